@@ -1,0 +1,282 @@
+/*
+ * netket_amd — C ABI of the MI355X (gfx950) VMC step.
+ *
+ * Every entry point takes caller-owned device buffers, plain sizes and a HIP
+ * stream (passed as void*), enqueues its kernels on that stream and returns
+ * 0 on success or a negative status (nka_last_error() holds the message).
+ * Nothing here allocates device memory or synchronises the stream; workspace
+ * buffers are passed in and their sizes are queryable.
+ *
+ * Element types are selected by the dtype codes NKA_F32 / NKA_F64.  Spin
+ * configurations are int8 arrays of +1/-1 (netket/hilbert/spin.py:165-178).
+ *
+ * The entry points replace the JAX functions of NetKet's VMC step; each
+ * declaration cites the reference function (file:line) it stands in for.
+ */
+#ifndef NETKET_AMD_H
+#define NETKET_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NKA_F32 0
+#define NKA_F64 1
+
+#define NKA_OK 0
+#define NKA_ERR_ARG -1
+#define NKA_ERR_HIP -2
+#define NKA_ERR_UNSUPPORTED -3
+
+/* Library identification and the message of the last failed call (per thread). */
+const char* nka_version(void);
+const char* nka_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Operators: connected elements (DiscreteJaxOperator.get_conn_padded,
+ * netket/operator/_discrete_operator_jax.py:117-139).
+ * ---------------------------------------------------------------------- */
+
+/* IsingJax.get_conn_padded (netket/operator/_ising/jax.py:82-88, 125-165).
+ * x: (n_batch, n_sites) int8; edges: (n_edges, 2) int32;
+ * xp: (n_batch, n_rows, n_sites) int8; mels: (n_batch, n_rows) of mel_dtype.
+ * n_rows is n_sites + 1, or 1 when h == 0 (the StaticZero branch). */
+int nka_ising_conn(const int8_t* x, int64_t n_batch, int n_sites, const int32_t* edges,
+                   int n_edges, double h, double J, int mel_dtype, int n_rows, int8_t* xp,
+                   void* mels, void* stream);
+
+/* Packed lookup tables of a spin-1/2 LocalOperator, the device image of
+ * pack_internals_jax (netket/operator/_local_operator/compile_helpers.py:29-218).
+ * Terms are ordered by acting size, then insertion order.  For term t with
+ * ns = op_nsites[t] sites op_sites[4t..4t+ns) (sorted), the row of a
+ * configuration is r = sum_k idx(x[site_k]) * 2^(ns-1-k) with idx(+1)=0,
+ * idx(-1)=1, and its table entry is row = op_row_off[t] + r.  row_diag[row] is
+ * the diagonal element; the off-diagonal elements with |mel| > mel_cutoff are
+ * conn_mel[row_conn_off[row] + k], k < row_nconn[row], with new local indices
+ * conn_newidx[...] (same bit order as r) and owning term conn_term[...].
+ * All pointers are device pointers. */
+typedef struct nka_localop {
+  int n_ops;
+  int max_conn_size;
+  int nonzero_diagonal;
+  int mel_dtype;
+  double constant;
+  double mel_cutoff;
+  const int32_t* op_sites;
+  const int32_t* op_nsites;
+  const int32_t* op_row_off;
+  const void* row_diag;
+  const int32_t* row_nconn;
+  const int32_t* row_conn_off;
+  const void* conn_mel;
+  const int32_t* conn_newidx;
+  const int32_t* conn_term;
+} nka_localop;
+
+/* LocalOperatorJax._get_conn_padded (netket/operator/_local_operator/jax.py:74-201,
+ * 256-288): xp (n_batch, max_conn_size, n_sites), mels (n_batch, max_conn_size),
+ * n_conn (n_batch,) may be NULL. */
+int nka_localop_conn(const int8_t* x, int64_t n_batch, int n_sites, const nka_localop* op,
+                     int8_t* xp, void* mels, int32_t* n_conn, void* stream);
+
+/* ------------------------------------------------------------------------
+ * RBM ansatz (netket/models/rbm.py:57-81, netket/nn/activation.py:78-84).
+ * Parameters live in one flat device buffer in the reference's ravel order
+ * (jax pytree key order): Dense/bias (M), Dense/kernel (N x M, row-major),
+ * visible_bias (N).  An absent bias has offset -1.
+ * ---------------------------------------------------------------------- */
+typedef struct nka_rbm {
+  int n_visible;
+  int n_hidden;
+  int dtype;
+  int reserved;
+  const void* params;
+  int64_t bias_off;
+  int64_t kernel_off;
+  int64_t visible_off;
+} nka_rbm;
+
+/* RBM.__call__: log psi of n configurations (netket/models/rbm.py:57-81). */
+int nka_rbm_log_psi(const nka_rbm* m, const int8_t* sigma, int64_t n, void* log_psi,
+                    void* stream);
+
+/* Operator selector for the fused local-energy kernel. */
+#define NKA_OP_ISING 0
+#define NKA_OP_LOCAL 1
+typedef struct nka_ising {
+  const int32_t* edges;
+  int n_edges;
+  int mel_dtype;
+  double h;
+  double J;
+} nka_ising;
+
+/* local_value_kernel_jax (netket/vqs/mc/kernels.py:62-71) fused with
+ * get_conn_padded: eloc[s] = sum_k mel_k exp(log psi(x'_k) - log psi(x_s)),
+ * evaluated with the hidden-unit update theta' = theta - 2 sigma_i W_i. so the
+ * connected configurations are never materialised.  eloc is FP64; log_psi
+ * (model dtype) may be NULL. */
+int nka_rbm_local_energy(const nka_rbm* m, int op_kind, const void* op, const int8_t* sigma,
+                         int64_t n, double* eloc, void* log_psi, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Sampler.
+ * ---------------------------------------------------------------------- */
+
+/* Random initial chain states (MetropolisSampler._init_state,
+ * netket/sampler/metropolis.py:353-380): uniform +-1 when n_down < 0, else a
+ * uniformly random placement of n_down spins -1 (the SumConstraint branch,
+ * netket/hilbert/random/homogeneous.py:50-72, netket/hilbert/random/fock.py:77-97). */
+int nka_random_states(int8_t* sigma, int n_chains, int n_sites, int n_down, uint64_t seed,
+                      uint64_t counter, void* stream);
+
+#define NKA_RULE_LOCAL 0
+#define NKA_RULE_EXCHANGE 1
+
+/* MetropolisSampler reset + sample(n_discard) + sample(chain_length)
+ * (netket/vqs/mc/mc_state/state.py:521-576; netket/sampler/metropolis.py:382-505)
+ * with LocalRule (netket/sampler/rules/local.py:40-49) or ExchangeRule
+ * (netket/sampler/rules/exchange.py:143-218) for an RBM.  One sweep is
+ * sweep_size proposals; after each of the last chain_length sweeps the chain
+ * state is written to samples (n_chains, chain_length, n_sites).  sigma
+ * (n_chains, n_sites) and log_prob (n_chains,, model dtype) are the persistent
+ * chain state.  Proposal q of chain c draws Philox4x32-10 at counter
+ * (counter + q, c, "METR") with key seed.  clusters: (n_clusters, 2) int32
+ * (exchange only).  n_accepted (n_chains,) is incremented. */
+int nka_rbm_metropolis(const nka_rbm* m, int rule, const int32_t* clusters, int n_clusters,
+                       int8_t* sigma, void* log_prob, int n_chains, int sweep_size,
+                       int n_discard, int chain_length, double machine_pow, uint64_t seed,
+                       uint64_t counter, int8_t* samples, void* log_prob_samples,
+                       int64_t* n_accepted, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Statistics (netket/stats/mc_stats_old.py:87-196): out[0..7] =
+ * {mean, var, var(chain means), n_blocks, var(block means), var(half-chain means),
+ *  block_size, n} of data (n_chains, chain_length) in FP64 (variances with ddof 0).
+ * ---------------------------------------------------------------------- */
+int nka_stats_moments(const double* data, int n_chains, int chain_length, double* out8,
+                      void* stream);
+
+/* ------------------------------------------------------------------------
+ * Jacobian of the RBM (nkjax.jacobian, netket/jax/_jacobian/logic.py:308-387)
+ * and the quantities built from it.
+ * ---------------------------------------------------------------------- */
+
+/* t = tanh(theta), theta = sigma W + b: t (ns, M) and tT (M, ns) in model dtype. */
+int nka_rbm_prepare(const nka_rbm* m, const int8_t* sigma, int64_t ns, void* t, void* tT,
+                    void* stream);
+
+/* Weighted sums over samples of the per-sample log-derivatives:
+ * out[v, p] = sum_s (O[s, p] - mean[p]) * w[v, s] for v < n_w <= 2 (FP64),
+ * O[s,p] = d log psi(sigma_s) / d theta_p in the ravel order, t = tanh(theta)
+ * (ns, M) from nka_rbm_prepare, and mean == NULL meaning no centring.  With
+ * w = 1/ns this is the column mean used for centring (logic.py:365-369); with
+ * w = 2 (E - mean E)/ns it is the gradient 2 Re F
+ * (netket/vqs/mc/mc_state/expect_forces.py:98-104, netket/vqs/mc/common.py:102-118);
+ * with centring and w = x / sqrt(ns) it is O_L^T x (netket/_src/ngd/srt.py:79,
+ * netket/_src/ngd/sr.py:43).  workspace holds nka_rbm_contract_workspace(...) bytes. */
+int64_t nka_rbm_contract_workspace(const nka_rbm* m, int64_t ns, int n_w);
+int nka_rbm_contract(const nka_rbm* m, const int8_t* sigma, const void* t, int64_t ns,
+                     const double* w, int n_w, const double* mean, double* out, void* workspace,
+                     void* stream);
+
+/* Centred, rescaled Jacobian O_L[s,p] = (O[s,p] - mean[p]) * scale
+ * (logic.py:365-373, netket/_src/ngd/sr_srt_common.py:75-80).
+ * layout NKA_LAYOUT_PS writes X = O_L^T (n_params x ns), NKA_LAYOUT_SP writes
+ * X = O_L (ns x n_params); both row-major.  kind NKA_OUT_BF16X2 writes the
+ * bf16 split X = hi + lo into out_a (hi) and out_b (lo); NKA_OUT_F32 /
+ * NKA_OUT_F64 write out_a only.  Rows of the output are ldo elements apart;
+ * columns past the row length are not touched. */
+#define NKA_LAYOUT_PS 0
+#define NKA_LAYOUT_SP 1
+#define NKA_OUT_BF16X2 0
+#define NKA_OUT_F32 1
+#define NKA_OUT_F64 2
+int nka_rbm_jacobian(const nka_rbm* m, const int8_t* sigma, const void* t, const void* tT,
+                     int64_t ns, const double* mean, double scale, int layout, int out_kind,
+                     void* out_a, void* out_b, int64_t ldo, void* stream);
+
+/* ------------------------------------------------------------------------
+ * QGT / NTK Gram matrices and dense FP64 linear algebra.
+ * ---------------------------------------------------------------------- */
+
+/* out = X X^T + shift I + offset 11^T (lower triangle incl. diagonal, FP64,
+ * row-major ldo) from the bf16 split X = hi + lo (n x k, row-major ldx) with
+ * three BF16 MFMA products (hi hi^T + hi lo^T + lo hi^T) accumulated in FP32.
+ * S = O_L^T O_L (QGTJacobianDenseT.to_dense,
+ * netket/optimizer/qgt/qgt_jacobian_dense.py:136-161; netket/_src/ngd/sr.py:58-63)
+ * or T = O_L O_L^T (netket/_src/ngd/srt.py:55-66). */
+int nka_gram_bf16x3(const uint16_t* x_hi, const uint16_t* x_lo, int64_t n, int64_t k,
+                    int64_t ldx, double shift, double offset, double* out, int64_t ldo,
+                    void* stream);
+
+/* C = beta C + alpha X X^T on the lower triangle (FP64 MFMA), X n x k row-major. */
+int nka_syrk_f64(const double* x, int64_t n, int64_t k, int64_t ldx, double alpha, double beta,
+                 double shift, double* c, int64_t ldc, void* stream);
+
+/* C = beta C + alpha A B^T (FP64 MFMA); A m x k, B n x k, C m x n, row-major. */
+int nka_gemm_nt_f64(const double* a, int64_t lda, const double* b, int64_t ldb, double* c,
+                    int64_t ldc, int64_t m, int64_t n, int64_t k, double alpha, double beta,
+                    void* stream);
+
+/* y = alpha A x + beta y (trans = 0) or alpha A^T x + beta y (trans = 1); A m x n
+ * row-major FP64, or the lower triangle of a symmetric matrix when trans = 2. */
+int nka_gemv_f64(const double* a, int64_t lda, int64_t m, int64_t n, int trans, const double* x,
+                 double alpha, double beta, double* y, void* workspace, void* stream);
+
+/* Cholesky factorisation A = L L^T of the lower triangle in place (FP64),
+ * cho_factor of netket/optimizer/solver/solvers.py:241-270.  workspace holds
+ * nka_potrf_workspace(n) bytes (inverses of the diagonal blocks).  info (one
+ * device int32) is 0 on success, or j+1 for the first non-positive or
+ * non-finite pivot j; later kernels of the same call skip their work then. */
+int64_t nka_potrf_workspace(int64_t n);
+int nka_potrf_f64(double* a, int64_t n, int64_t lda, void* workspace, int32_t* info,
+                  void* stream);
+
+/* cho_solve: b <- A^{-1} b from the factor of nka_potrf_f64 (same workspace);
+ * b is set to NaN when the factorisation failed (info != 0). */
+int nka_potrs_f64(const double* a, int64_t n, int64_t lda, const void* workspace, double* b,
+                  const int32_t* info, void* stream);
+
+/* Mirror the lower triangle of a row-major FP64 matrix onto its upper triangle. */
+int nka_symmetrize_f64(double* a, int64_t n, int64_t lda, void* stream);
+
+/* Flags non-finite entries: *flag |= 1 if any of x[0..n) is NaN/Inf (the
+ * nan_fallback test of netket/_src/solvers/nan_fallback.py:22-45). */
+int nka_check_finite(const double* x, int64_t n, int32_t* flag, void* stream);
+
+/* Conjugate gradient for A x = b (A full row-major FP64 n x n, e.g. S + eps I),
+ * with the semantics of jax.scipy.sparse.linalg.cg, the default solver of
+ * nk.optimizer.SR (netket/optimizer/sr.py:121): x0 = 0; iterate while
+ * r.r > max(tol^2 b.b, atol^2) and k < maxiter.  nka_cg_start initialises x
+ * and the workspace (nka_cg_workspace(n) bytes); each nka_cg_iterate call
+ * enqueues `iters` iterations that become no-ops once converged;
+ * nka_cg_status copies {r.r, p.Ap, alpha, beta, atol^2, k, done, maxiter} into
+ * a device buffer of 8 doubles. */
+int64_t nka_cg_workspace(int64_t n);
+int nka_cg_start(const double* b, int64_t n, double tol, double atol, int64_t maxiter, double* x,
+                 void* workspace, void* stream);
+int nka_cg_iterate(const double* a, int64_t lda, int64_t n, double* x, void* workspace, int iters,
+                   void* stream);
+int nka_cg_status(const void* workspace, int64_t n, double* out8, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Parameter update: optax.sgd via apply_gradient
+ * (netket/_src/driver/abstract_optimization_driver.py:86-92,
+ * netket/optimizer/__init__.py:33-60): p <- p + (-lr * (dtype)dp).
+ * ---------------------------------------------------------------------- */
+int nka_sgd_update(void* params, int dtype, const double* dp, int64_t n, double lr,
+                   void* stream);
+
+/* Elementwise helpers used by the host-side solvers (no reference counterpart:
+ * building blocks of the CG / dv preparation). */
+int nka_centre_scale(const double* e, int64_t n, const double* mean, double scale, double* out,
+                     void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NETKET_AMD_H */

@@ -1,0 +1,130 @@
+"""Device-side building blocks of the VMC step, shared by MCState, the QGT and
+the drivers.  Each function enqueues C-ABI kernels on the current stream and
+returns device tensors; nothing here synchronises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import torch
+
+from . import _native as nat
+from . import _ops
+
+
+def local_energies(model, flat, op, sigma2d: torch.Tensor, log_psi: bool = False):
+    """E_loc (FP64) of every row of ``sigma2d`` (netket/vqs/mc/kernels.py:62-71)."""
+    n, N = sigma2d.shape
+    eloc = torch.empty(n, dtype=torch.float64, device=sigma2d.device)
+    lp = torch.empty(n, dtype=flat.dtype, device=sigma2d.device) if log_psi else None
+    kind, op_ptr, keep = _ops.op_struct(op)
+    m = model.native(flat, N)
+    nat.call("nka_rbm_local_energy", C.byref(m), kind, op_ptr, _ops.ptr(sigma2d), n,
+             _ops.ptr(eloc), _ops.ptr(lp), _ops.stream())
+    del keep
+    return (eloc, lp) if log_psi else eloc
+
+
+def prepare(model, flat, sigma2d: torch.Tensor, row_major: bool = True, transposed: bool = False):
+    """tanh(theta) per sample: (ns, M) and/or (M, ns) in the model dtype."""
+    n, N = sigma2d.shape
+    M = model.n_hidden(N)
+    t = torch.empty((n, M), dtype=flat.dtype, device=sigma2d.device) if row_major else None
+    tT = torch.empty((M, n), dtype=flat.dtype, device=sigma2d.device) if transposed else None
+    m = model.native(flat, N)
+    nat.call("nka_rbm_prepare", C.byref(m), _ops.ptr(sigma2d), n, _ops.ptr(t), _ops.ptr(tT),
+             _ops.stream())
+    return t, tT
+
+
+def contract(model, flat, sigma2d: torch.Tensor, t: torch.Tensor, w: torch.Tensor,
+             mean: torch.Tensor | None = None) -> torch.Tensor:
+    """sum_s (O[s,p] - mean[p]) w[v,s] for each row v of ``w`` (FP64, (n_w, Np))."""
+    n, N = sigma2d.shape
+    w = w.reshape(-1, n).contiguous()
+    n_w = w.shape[0]
+    m = model.native(flat, N)
+    np_ = model.layout(N)["n_params"]
+    out = torch.empty((n_w, np_), dtype=torch.float64, device=sigma2d.device)
+    ws_bytes = nat.lib().nka_rbm_contract_workspace(C.byref(m), n, n_w)
+    ws = _ops.WORKSPACE.get("contract", ws_bytes)
+    nat.call("nka_rbm_contract", C.byref(m), _ops.ptr(sigma2d), _ops.ptr(t), n, _ops.ptr(w), n_w,
+             _ops.ptr(mean), _ops.ptr(out), _ops.ptr(ws), _ops.stream())
+    return out
+
+
+def _ld(k: int) -> int:
+    return ((k + 63) // 64) * 64
+
+
+class JacobianBuffers:
+    """Centred, rescaled Jacobian X (row-major, rows padded to a multiple of 64
+    columns with zeros) in bf16 hi/lo planes or in FP64."""
+
+    def __init__(self, rows: int, cols: int, kind: int, device):
+        self.rows, self.cols, self.kind = rows, cols, kind
+        self.ld = _ld(cols)
+        if kind == nat.OUT_BF16X2:
+            self.hi = torch.zeros((rows, self.ld), dtype=torch.uint16, device=device)
+            self.lo = torch.zeros((rows, self.ld), dtype=torch.uint16, device=device)
+        elif kind == nat.OUT_F64:
+            self.hi = torch.zeros((rows, self.ld), dtype=torch.float64, device=device)
+            self.lo = None
+        else:
+            self.hi = torch.zeros((rows, self.ld), dtype=torch.float32, device=device)
+            self.lo = None
+
+    def matches(self, rows, cols, kind):
+        return self.rows == rows and self.cols == cols and self.kind == kind
+
+
+def jacobian(model, flat, sigma2d, t, tT, mean, scale: float, layout: int, kind: int,
+             bufs: JacobianBuffers | None = None) -> JacobianBuffers:
+    """X = O_L^T (layout PS, Np x ns) or O_L (layout SP, ns x Np), O_L = (O - mean) scale."""
+    n, N = sigma2d.shape
+    np_ = model.layout(N)["n_params"]
+    rows, cols = (np_, n) if layout == nat.LAYOUT_PS else (n, np_)
+    if bufs is None or not bufs.matches(rows, cols, kind):
+        bufs = JacobianBuffers(rows, cols, kind, sigma2d.device)
+    m = model.native(flat, N)
+    nat.call("nka_rbm_jacobian", C.byref(m), _ops.ptr(sigma2d), _ops.ptr(t), _ops.ptr(tT), n,
+             _ops.ptr(mean), float(scale), layout, kind, _ops.ptr(bufs.hi), _ops.ptr(bufs.lo),
+             bufs.ld, _ops.stream())
+    return bufs
+
+
+def gram(bufs: JacobianBuffers, shift: float, offset: float = 0.0,
+         out: torch.Tensor | None = None) -> torch.Tensor:
+    """X X^T + shift I + offset 11^T on the lower triangle (FP64)."""
+    n = bufs.rows
+    if out is None or out.shape != (n, n):
+        out = torch.empty((n, n), dtype=torch.float64, device=bufs.hi.device)
+    if bufs.kind == nat.OUT_BF16X2:
+        nat.call("nka_gram_bf16x3", _ops.ptr(bufs.hi), _ops.ptr(bufs.lo), n, bufs.ld, bufs.ld,
+                 float(shift), float(offset), _ops.ptr(out), out.stride(0), _ops.stream())
+    elif bufs.kind == nat.OUT_F64:
+        nat.call("nka_syrk_f64", _ops.ptr(bufs.hi), n, bufs.ld, bufs.ld, 1.0, 0.0, float(shift),
+                 _ops.ptr(out), out.stride(0), _ops.stream())
+        if offset != 0.0:
+            raise NotImplementedError("offset with FP64 Jacobians")
+    else:
+        raise NotImplementedError("Gram of an FP32 Jacobian")
+    return out
+
+
+def centre_scale(e: torch.Tensor, mean_dev: torch.Tensor, scale: float) -> torch.Tensor:
+    out = torch.empty_like(e)
+    nat.call("nka_centre_scale", _ops.ptr(e), e.numel(), _ops.ptr(mean_dev), float(scale),
+             _ops.ptr(out), _ops.stream())
+    return out
+
+
+def sgd_update_(flat: torch.Tensor, dp: torch.Tensor, lr: float):
+    nat.call("nka_sgd_update", _ops.ptr(flat), _ops.dtype_code(flat.dtype), _ops.ptr(dp),
+             flat.numel(), float(lr), _ops.stream())
+
+
+def inv_sqrt(n: int) -> float:
+    return 1.0 / math.sqrt(n)

@@ -1,0 +1,145 @@
+"""ctypes binding of the C ABI declared in ``include/netket_amd.h``.
+
+The library ``netket_amd/_lib/libnetket_amd.so`` is built by
+``netket_amd.build``.  Loading fails loudly: there is no CPU fallback for any
+kernel of the VMC step.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libnetket_amd.so"
+
+F32 = 0
+F64 = 1
+OP_ISING = 0
+OP_LOCAL = 1
+RULE_LOCAL = 0
+RULE_EXCHANGE = 1
+LAYOUT_PS = 0
+LAYOUT_SP = 1
+OUT_BF16X2 = 0
+OUT_F32 = 1
+OUT_F64 = 2
+
+
+class NkaLocalOp(C.Structure):
+    _fields_ = [
+        ("n_ops", C.c_int),
+        ("max_conn_size", C.c_int),
+        ("nonzero_diagonal", C.c_int),
+        ("mel_dtype", C.c_int),
+        ("constant", C.c_double),
+        ("mel_cutoff", C.c_double),
+        ("op_sites", C.c_void_p),
+        ("op_nsites", C.c_void_p),
+        ("op_row_off", C.c_void_p),
+        ("row_diag", C.c_void_p),
+        ("row_nconn", C.c_void_p),
+        ("row_conn_off", C.c_void_p),
+        ("conn_mel", C.c_void_p),
+        ("conn_newidx", C.c_void_p),
+        ("conn_term", C.c_void_p),
+    ]
+
+
+class NkaRbm(C.Structure):
+    _fields_ = [
+        ("n_visible", C.c_int),
+        ("n_hidden", C.c_int),
+        ("dtype", C.c_int),
+        ("reserved", C.c_int),
+        ("params", C.c_void_p),
+        ("bias_off", C.c_int64),
+        ("kernel_off", C.c_int64),
+        ("visible_off", C.c_int64),
+    ]
+
+
+class NkaIsing(C.Structure):
+    _fields_ = [
+        ("edges", C.c_void_p),
+        ("n_edges", C.c_int),
+        ("mel_dtype", C.c_int),
+        ("h", C.c_double),
+        ("J", C.c_double),
+    ]
+
+
+vp = C.c_void_p
+i32 = C.c_int
+i64 = C.c_int64
+u64 = C.c_uint64
+f64 = C.c_double
+
+# name -> (restype, argtypes); must match include/netket_amd.h
+SIGNATURES = {
+    "nka_version": (C.c_char_p, []),
+    "nka_last_error": (C.c_char_p, []),
+    "nka_ising_conn": (i32, [vp, i64, i32, vp, i32, f64, f64, i32, i32, vp, vp, vp]),
+    "nka_localop_conn": (i32, [vp, i64, i32, C.POINTER(NkaLocalOp), vp, vp, vp, vp]),
+    "nka_rbm_log_psi": (i32, [C.POINTER(NkaRbm), vp, i64, vp, vp]),
+    "nka_rbm_local_energy": (i32, [C.POINTER(NkaRbm), i32, vp, vp, i64, vp, vp, vp]),
+    "nka_random_states": (i32, [vp, i32, i32, i32, u64, u64, vp]),
+    "nka_rbm_metropolis": (
+        i32,
+        [C.POINTER(NkaRbm), i32, vp, i32, vp, vp, i32, i32, i32, i32, f64, u64, u64, vp, vp, vp, vp],
+    ),
+    "nka_stats_moments": (i32, [vp, i32, i32, vp, vp]),
+    "nka_rbm_prepare": (i32, [C.POINTER(NkaRbm), vp, i64, vp, vp, vp]),
+    "nka_rbm_contract_workspace": (i64, [C.POINTER(NkaRbm), i64, i32]),
+    "nka_rbm_contract": (i32, [C.POINTER(NkaRbm), vp, vp, i64, vp, i32, vp, vp, vp, vp]),
+    "nka_rbm_jacobian": (i32, [C.POINTER(NkaRbm), vp, vp, vp, i64, vp, f64, i32, i32, vp, vp, i64, vp]),
+    "nka_gram_bf16x3": (i32, [vp, vp, i64, i64, i64, f64, f64, vp, i64, vp]),
+    "nka_syrk_f64": (i32, [vp, i64, i64, i64, f64, f64, f64, vp, i64, vp]),
+    "nka_gemm_nt_f64": (i32, [vp, i64, vp, i64, vp, i64, i64, i64, i64, f64, f64, vp]),
+    "nka_gemv_f64": (i32, [vp, i64, i64, i64, i32, vp, f64, f64, vp, vp, vp]),
+    "nka_potrf_workspace": (i64, [i64]),
+    "nka_potrf_f64": (i32, [vp, i64, i64, vp, vp, vp]),
+    "nka_potrs_f64": (i32, [vp, i64, i64, vp, vp, vp, vp]),
+    "nka_symmetrize_f64": (i32, [vp, i64, i64, vp]),
+    "nka_check_finite": (i32, [vp, i64, vp, vp]),
+    "nka_sgd_update": (i32, [vp, i32, vp, i64, f64, vp]),
+    "nka_centre_scale": (i32, [vp, i64, vp, f64, vp, vp]),
+    "nka_cg_workspace": (i64, [i64]),
+    "nka_cg_start": (i32, [vp, i64, f64, f64, i64, vp, vp, vp]),
+    "nka_cg_iterate": (i32, [vp, i64, i64, vp, vp, i32, vp]),
+    "nka_cg_status": (i32, [vp, i64, vp, vp]),
+}
+
+_lib = None
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise NativeError(
+                f"{LIB_PATH} is missing: build the gfx950 library first "
+                "(python -m netket_amd.build). There is no CPU fallback."
+            )
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status: int, what: str):
+    if status != 0:
+        msg = lib().nka_last_error().decode(errors="replace")
+        raise NativeError(f"{what} failed ({status}): {msg}")
+
+
+def call(name: str, *args):
+    fn = getattr(lib(), name)
+    check(fn(*args), name)

@@ -1,0 +1,156 @@
+// Shared device helpers for the gfx950 VMC-step kernels.
+//
+// * Philox4x32-10 counter-based generator (Salmon, Moraes, Dror, Shaw,
+//   "Parallel random numbers: as easy as 1, 2, 3", SC'11), used for every
+//   random draw of the sampler so that a draw is a pure function of
+//   (seed, chain, proposal index).
+// * log_cosh(x) = |x| + log1p(exp(-2|x|)) - log 2, the stable form used by
+//   the reference activation (netket/nn/activation.py:78-84).
+// * 64-lane wavefront reductions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+#define NKA_WAVE 64
+
+namespace nka {
+
+struct u32x4 {
+  uint32_t x, y, z, w;
+};
+
+__host__ __device__ inline u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c.x;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c.z;
+    u32x4 n;
+    n.x = (uint32_t)(p1 >> 32) ^ c.y ^ k0;
+    n.y = (uint32_t)p1;
+    n.z = (uint32_t)(p0 >> 32) ^ c.w ^ k1;
+    n.w = (uint32_t)p0;
+    c = n;
+  }
+  return c;
+}
+
+// Stream tags placed in the fourth counter word so that different uses of the
+// generator never share a counter.
+enum : uint32_t {
+  TAG_METROPOLIS = 0x4D455452u,  // "METR"
+  TAG_RANDSTATE = 0x52414E44u,   // "RAND"
+};
+
+// Uniform in [0, 1) from the top 24 bits (float) or 53 bits (double).
+template <class T> __device__ inline T uniform01(const u32x4& r);
+template <> __device__ inline float uniform01<float>(const u32x4& r) {
+  return (float)(r.y >> 8) * 5.9604644775390625e-08f;  // 2^-24
+}
+template <> __device__ inline double uniform01<double>(const u32x4& r) {
+  const uint64_t u = (((uint64_t)r.y << 32) | (uint64_t)r.z) >> 11;
+  return (double)u * 1.1102230246251565e-16;  // 2^-53
+}
+
+// floor(x * n / 2^32): an index in [0, n) from one 32-bit draw.
+__host__ __device__ inline uint32_t bounded(uint32_t x, uint32_t n) {
+  return (uint32_t)(((uint64_t)x * (uint64_t)n) >> 32);
+}
+
+template <class T> __device__ inline T log_cosh(T x);
+template <> __device__ inline float log_cosh<float>(float x) {
+  const float ax = fabsf(x);
+  return ax + log1pf(expf(-2.0f * ax)) - 0.69314718055994531f;
+}
+template <> __device__ inline double log_cosh<double>(double x) {
+  const double ax = fabs(x);
+  return ax + log1p(exp(-2.0 * ax)) - 0.69314718055994530942;
+}
+
+template <class T> __device__ inline T exp_t(T x);
+template <> __device__ inline float exp_t<float>(float x) { return expf(x); }
+template <> __device__ inline double exp_t<double>(double x) { return exp(x); }
+
+template <class T> __device__ inline T log_t(T x);
+template <> __device__ inline float log_t<float>(float x) { return logf(x); }
+template <> __device__ inline double log_t<double>(double x) { return log(x); }
+
+template <class T> __device__ inline T tanh_t(T x);
+template <> __device__ inline float tanh_t<float>(float x) { return tanhf(x); }
+template <> __device__ inline double tanh_t<double>(double x) { return tanh(x); }
+
+// Butterfly sum over the 64 lanes of a wavefront; every lane gets the total.
+template <class T> __device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, NKA_WAVE);
+  return v;
+}
+
+__device__ inline int lane_id() { return (int)(threadIdx.x & (NKA_WAVE - 1)); }
+
+// Order this lane's LDS writes before later LDS reads of other lanes in the
+// same wavefront.
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Lane 0's value in every lane (a scalar register), so that decisions taken on
+// it are wave-uniform by construction.
+template <class T> __device__ inline T uniform_value(T v);
+template <> __device__ inline int uniform_value<int>(int v) {
+  return __builtin_amdgcn_readfirstlane(v);
+}
+template <> __device__ inline uint32_t uniform_value<uint32_t>(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+template <> __device__ inline float uniform_value<float>(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+template <> __device__ inline double uniform_value<double>(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffLL));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+}  // namespace nka
+
+// ---------------------------------------------------------------------------
+// Host-side status plumbing shared by every C-ABI entry point.
+// ---------------------------------------------------------------------------
+namespace nka {
+void set_error(const char* fmt, ...);
+}
+
+#define NKA_OK 0
+#define NKA_ERR_ARG -1
+#define NKA_ERR_HIP -2
+#define NKA_ERR_UNSUPPORTED -3
+
+#define NKA_CHECK_ARG(cond, ...)      \
+  do {                                \
+    if (!(cond)) {                    \
+      nka::set_error(__VA_ARGS__);    \
+      return NKA_ERR_ARG;             \
+    }                                 \
+  } while (0)
+
+#define NKA_CHECK_LAUNCH(what)                                               \
+  do {                                                                       \
+    hipError_t e_ = hipGetLastError();                                       \
+    if (e_ != hipSuccess) {                                                  \
+      nka::set_error("%s: launch failed: %s", what, hipGetErrorString(e_)); \
+      return NKA_ERR_HIP;                                                    \
+    }                                                                        \
+  } while (0)
+
+#define NKA_F32 0
+#define NKA_F64 1

@@ -1,0 +1,734 @@
+// Dense linear algebra of the SR / minSR solve on gfx950 matrix cores.
+//
+// nka_gram_bf16x3  S = O_L^T O_L or T = O_L O_L^T (+ shift I + offset 11^T)
+//                  (netket/optimizer/qgt/qgt_jacobian_dense.py:136-161,
+//                   netket/_src/ngd/sr.py:58-63, netket/_src/ngd/srt.py:55-66)
+//                  from a bf16 hi/lo split with three v_mfma_f32_32x32x16_bf16
+//                  products per tile (hi.hi + hi.lo + lo.hi), FP32 accumulate.
+// nka_gemm_nt_f64 / nka_syrk_f64
+//                  FP64 v_mfma_f64_16x16x4 products, used by the Cholesky
+//                  trailing update, its panel solve and FP64 Gram matrices.
+// nka_potrf_f64 / nka_potrs_f64
+//                  blocked right-looking Cholesky and the two triangular
+//                  solves (jax.scipy.linalg.cho_factor / cho_solve as called by
+//                  netket/optimizer/solver/solvers.py:241-270).
+// nka_gemv_f64, nka_check_finite, nka_sgd_update, nka_centre_scale, nka_stats_moments
+//                  vector helpers of the step.
+#include "nka_common.hpp"
+#include "netket_amd.h"
+
+namespace nka {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+// Map a linear block id onto the lower-triangular tile pair (ti >= tj).
+__device__ inline void tri_tile(int64_t b, int& ti, int& tj) {
+  int64_t i = (int64_t)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
+  while ((i + 1) * (i + 2) / 2 <= b) ++i;
+  while (i * (i + 1) / 2 > b) --i;
+  ti = (int)i;
+  tj = (int)(b - i * (i + 1) / 2);
+}
+
+// Spread consecutive block ids over the 8 XCDs in contiguous chunks: the
+// hardware dispatches block b to XCD b % 8, so block b is given the work item
+// (b % 8) * ceil(G/8) + b / 8; neighbouring tiles (which share operand rows)
+// then run on the same XCD and share its L2.
+// The grid is launched with a multiple of 8 blocks (G = 8 * per), which makes
+// the map a bijection of [0, G); ids past the real work count exit early.
+__device__ inline int64_t xcd_remap(int64_t b, int64_t G) {
+  const int64_t per = G / 8;
+  return (b % 8) * per + b / 8;
+}
+inline unsigned grid_mult8(int64_t tiles) { return (unsigned)(((tiles + 7) / 8) * 8); }
+
+// ---------------------------------------------------------------------------
+// bf16x3 Gram: 128x128 output tile per workgroup, 4 waves of 64x64, K step 32.
+// ---------------------------------------------------------------------------
+constexpr int GB_T = 128;   // tile edge
+constexpr int GB_K = 32;    // K step (bf16 elements)
+constexpr int GB_LD = 40;   // LDS row stride in bf16 (32 + 8 pad: 80 bytes)
+
+__global__ void __launch_bounds__(256) gram_bf16x3_kernel(const uint16_t* __restrict__ xh,
+                                                          const uint16_t* __restrict__ xl, int64_t n,
+                                                          int64_t K, int64_t ldx, double shift,
+                                                          double offset, double* __restrict__ out,
+                                                          int64_t ldo, int64_t n_tiles_total) {
+  __shared__ __attribute__((aligned(16))) uint16_t sA[2][GB_T * GB_LD];  // [plane][row*LD + k]
+  __shared__ __attribute__((aligned(16))) uint16_t sB[2][GB_T * GB_LD];
+  const int64_t bid = xcd_remap(blockIdx.x, gridDim.x);
+  if (bid >= n_tiles_total) return;
+  int ti, tj;
+  tri_tile(bid, ti, tj);
+  const int64_t i0 = (int64_t)ti * GB_T, j0 = (int64_t)tj * GB_T;
+  const int tid = threadIdx.x, wid = tid / 64, lane = lane_id();
+  const int wm = (wid >> 1) * 64, wn = (wid & 1) * 64;  // wave tile origin in block tile
+  const bool diag_tile = (ti == tj);
+
+  floatx16 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[a][b][e] = 0.0f;
+
+  // global->LDS copy: each operand tile is 128 rows x 32 k = 512 chunks of 8 bf16
+  for (int64_t k0 = 0; k0 < K; k0 += GB_K) {
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+      const int chunk = tid + it * 256;  // 0..511
+      const int r = chunk >> 2, kc = (chunk & 3) * 8;
+      const int64_t kk = k0 + kc;
+      uint4 ah = {0, 0, 0, 0}, al = {0, 0, 0, 0}, bh = {0, 0, 0, 0}, bl = {0, 0, 0, 0};
+      if (i0 + r < n && kk < K) {
+        ah = *(const uint4*)(xh + (i0 + r) * ldx + kk);
+        al = *(const uint4*)(xl + (i0 + r) * ldx + kk);
+      }
+      if (!diag_tile && j0 + r < n && kk < K) {
+        bh = *(const uint4*)(xh + (j0 + r) * ldx + kk);
+        bl = *(const uint4*)(xl + (j0 + r) * ldx + kk);
+      }
+      *(uint4*)(&sA[0][r * GB_LD + kc]) = ah;
+      *(uint4*)(&sA[1][r * GB_LD + kc]) = al;
+      if (!diag_tile) {
+        *(uint4*)(&sB[0][r * GB_LD + kc]) = bh;
+        *(uint4*)(&sB[1][r * GB_LD + kc]) = bl;
+      }
+    }
+    __syncthreads();
+    const uint16_t* Bh = diag_tile ? sA[0] : sB[0];
+    const uint16_t* Bl = diag_tile ? sA[1] : sB[1];
+#pragma unroll
+    for (int ks = 0; ks < GB_K; ks += 16) {
+      const int kq = ks + (lane >> 5) * 8;
+      bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const int row = wm + a * 32 + (lane & 31);
+        ah[a] = *(const bf16x8*)(&sA[0][row * GB_LD + kq]);
+        al[a] = *(const bf16x8*)(&sA[1][row * GB_LD + kq]);
+      }
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int row = wn + b * 32 + (lane & 31);
+        bh[b] = *(const bf16x8*)(&Bh[row * GB_LD + kq]);
+        bl[b] = *(const bf16x8*)(&Bl[row * GB_LD + kq]);
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+        }
+    }
+  }
+  // store: element e of lane l in a 32x32 tile is (row (e/4)*8 + (l/32)*4 + e%4, col l%32)
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int64_t gi = i0 + wm + a * 32 + (e / 4) * 8 + (lane >> 5) * 4 + (e & 3);
+        const int64_t gj = j0 + wn + b * 32 + (lane & 31);
+        if (gi < n && gj <= gi) {
+          double v = (double)acc[a][b][e] + offset;
+          if (gi == gj) v += shift;
+          out[gi * ldo + gj] = v;
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 NT GEMM: C = beta C + alpha A B^T (+ shift on the diagonal),
+// 64x64 tile per workgroup, 4 waves of 32x32, K step 16.  lower_only skips
+// tiles above the diagonal and elements with j > i.  A and C may alias when
+// beta == 0 and the grid has a single column tile (in-place panel solve):
+// every workgroup reads all of its A rows before it writes them.
+// ---------------------------------------------------------------------------
+constexpr int GD_T = 64;
+constexpr int GD_K = 16;
+constexpr int GD_LD = 17;  // padded LDS stride (doubles)
+
+__global__ void __launch_bounds__(256) gemm_nt_f64_kernel(const double* a, int64_t lda, const double* b,
+                                                          int64_t ldb, double* c, int64_t ldc, int64_t m,
+                                                          int64_t n, int64_t K, double alpha, double beta,
+                                                          double shift, int lower_only, int64_t tiles_n,
+                                                          int64_t n_tiles_total, const int32_t* info) {
+  if (info && *info != 0) return;
+  __shared__ double sA[GD_T * GD_LD];
+  __shared__ double sB[GD_T * GD_LD];
+  const int64_t bid = xcd_remap(blockIdx.x, gridDim.x);
+  if (bid >= n_tiles_total) return;
+  int ti, tj;
+  if (lower_only) {
+    tri_tile(bid, ti, tj);
+  } else {
+    ti = (int)(bid / tiles_n);
+    tj = (int)(bid % tiles_n);
+  }
+  const int64_t i0 = (int64_t)ti * GD_T, j0 = (int64_t)tj * GD_T;
+  const int tid = threadIdx.x, wid = tid / 64, lane = lane_id();
+  const int wm = (wid >> 1) * 32, wn = (wid & 1) * 32;
+  doublex4 acc[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) acc[x][y] = (doublex4){0.0, 0.0, 0.0, 0.0};
+
+  for (int64_t k0 = 0; k0 < K; k0 += GD_K) {
+    __syncthreads();
+    // 64 rows x 16 k per operand = 1024 doubles; 256 threads x 4
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+      const int idx = tid + it * 256;
+      const int r = idx >> 4, kc = idx & 15;
+      const int64_t kk = k0 + kc;
+      sA[r * GD_LD + kc] = (i0 + r < m && kk < K) ? a[(i0 + r) * lda + kk] : 0.0;
+      sB[r * GD_LD + kc] = (j0 + r < n && kk < K) ? b[(j0 + r) * ldb + kk] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < GD_K; ks += 4) {
+      const int kq = ks + (lane >> 4);
+      double av[2], bv[2];
+#pragma unroll
+      for (int x = 0; x < 2; ++x) av[x] = sA[(wm + x * 16 + (lane & 15)) * GD_LD + kq];
+#pragma unroll
+      for (int y = 0; y < 2; ++y) bv[y] = sB[(wn + y * 16 + (lane & 15)) * GD_LD + kq];
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+          acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x], bv[y], acc[x][y], 0, 0, 0);
+    }
+  }
+  __syncthreads();
+  // element e of lane l in a 16x16 tile is (row l/16 + 4e, col l%16)
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t gi = i0 + wm + x * 16 + (lane >> 4) + 4 * e;
+        const int64_t gj = j0 + wn + y * 16 + (lane & 15);
+        if (gi < m && gj < n && (!lower_only || gj <= gi)) {
+          double v = alpha * acc[x][y][e];
+          if (beta != 0.0) v += beta * c[gi * ldc + gj];
+          if (gi == gj) v += shift;
+          c[gi * ldc + gj] = v;
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// Cholesky building blocks (block size 64).
+// ---------------------------------------------------------------------------
+constexpr int CH_NB = 64;
+
+// Factor the diagonal block A[k0:k0+nb, k0:k0+nb] in LDS, write L11 back and
+// L11^{-1} to inv (nb x nb).  A non-positive or non-finite pivot sets *info.
+__global__ void __launch_bounds__(256) potf2_inv_kernel(double* __restrict__ A, int64_t lda, int64_t n,
+                                                        int64_t k0, double* __restrict__ inv,
+                                                        int32_t* __restrict__ info) {
+  if (*info != 0) return;
+  __shared__ double L[CH_NB][CH_NB + 1];
+  __shared__ double X[CH_NB][CH_NB + 1];
+  __shared__ int bad;
+  const int nb = (int)min<int64_t>(CH_NB, n - k0);
+  const int tid = threadIdx.x;
+  if (tid == 0) bad = 0;
+  for (int idx = tid; idx < CH_NB * CH_NB; idx += blockDim.x) {
+    const int r = idx / CH_NB, cc = idx % CH_NB;
+    double v = 0.0;
+    if (r < nb && cc < nb) v = (cc <= r) ? A[(k0 + r) * lda + k0 + cc] : 0.0;
+    else if (r == cc) v = 1.0;  // identity padding keeps the padded factor regular
+    L[r][cc] = v;
+  }
+  __syncthreads();
+  for (int j = 0; j < CH_NB; ++j) {
+    if (tid == 0) {
+      const double d = L[j][j];
+      if (!(d > 0.0) || !isfinite(d)) {
+        if (!bad) bad = (int)(k0 + j + 1);
+        L[j][j] = 1.0;
+      } else {
+        L[j][j] = sqrt(d);
+      }
+    }
+    __syncthreads();
+    const double djj = L[j][j];
+    for (int i = j + 1 + tid; i < CH_NB; i += blockDim.x) L[i][j] /= djj;
+    __syncthreads();
+    const int rem = CH_NB - j - 1;
+    for (int idx = tid; idx < rem * rem; idx += blockDim.x) {
+      const int i = j + 1 + idx / rem, cc = j + 1 + idx % rem;
+      if (cc <= i) L[i][cc] -= L[i][j] * L[cc][j];
+    }
+    __syncthreads();
+  }
+  // inverse of the lower-triangular L: column cc solved by forward substitution,
+  // four threads per column sharing the dot products
+  const int col = tid >> 2, part = tid & 3;
+  for (int i = 0; i < CH_NB; ++i) {
+    double s = 0.0;
+    if (i >= col) {
+      for (int k = col + part; k < i; k += 4) s += L[i][k] * X[k][col];
+    }
+    s += __shfl_xor(s, 1, 64);
+    s += __shfl_xor(s, 2, 64);
+    if (part == 0) {
+      if (i < col) X[i][col] = 0.0;
+      else X[i][col] = ((i == col ? 1.0 : 0.0) - s) / L[i][i];
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < CH_NB * CH_NB; idx += blockDim.x) {
+    const int r = idx / CH_NB, cc = idx % CH_NB;
+    inv[idx] = X[r][cc];
+    if (r < nb && cc <= r) A[(k0 + r) * lda + k0 + cc] = L[r][cc];
+  }
+  if (tid == 0 && bad) atomicCAS(info, 0, bad);
+}
+
+// ---------------------------------------------------------------------------
+// Triangular solves with the blocked factor.  One launch per block column:
+// every workgroup recomputes y = inv_kk b_k (64x64) into LDS, workgroup 0
+// stores it, and each workgroup applies y to its slice of the remaining rows.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) trsv_fwd_kernel(const double* __restrict__ A, int64_t lda, int64_t n,
+                                                       int64_t k0, const double* __restrict__ inv,
+                                                       double* __restrict__ b, double* __restrict__ y_out,
+                                                       const int32_t* __restrict__ info) {
+  if (*info != 0) return;
+  __shared__ double y[CH_NB];
+  __shared__ double bk[CH_NB];
+  const int nb = (int)min<int64_t>(CH_NB, n - k0);
+  const int tid = threadIdx.x;
+  if (tid < CH_NB) bk[tid] = tid < nb ? b[k0 + tid] : 0.0;
+  __syncthreads();
+  if (tid < CH_NB) {
+    double s = 0.0;
+    for (int c = 0; c <= tid; ++c) s += inv[tid * CH_NB + c] * bk[c];
+    y[tid] = s;
+    if (blockIdx.x == 0 && tid < nb) y_out[k0 + tid] = s;
+  }
+  __syncthreads();
+  // rows below the block: b[r] -= sum_c A[r, k0+c] y[c]; one wave per row
+  const int wid = tid / 64, lane = lane_id();
+  const int64_t r0 = k0 + nb;
+  for (int64_t r = r0 + (int64_t)blockIdx.x * 4 + wid; r < n; r += (int64_t)gridDim.x * 4) {
+    double s = lane < nb ? A[r * lda + k0 + lane] * y[lane] : 0.0;
+    s = wave_sum(s);
+    if (lane == 0) b[r] -= s;
+  }
+}
+
+__global__ void __launch_bounds__(256) trsv_bwd_kernel(const double* __restrict__ A, int64_t lda, int64_t n,
+                                                       int64_t k0, const double* __restrict__ inv,
+                                                       double* __restrict__ b, double* __restrict__ x_out,
+                                                       const int32_t* __restrict__ info) {
+  if (*info != 0) return;
+  __shared__ double y[CH_NB];
+  __shared__ double bk[CH_NB];
+  const int nb = (int)min<int64_t>(CH_NB, n - k0);
+  const int tid = threadIdx.x;
+  if (tid < CH_NB) bk[tid] = tid < nb ? b[k0 + tid] : 0.0;
+  __syncthreads();
+  if (tid < CH_NB) {
+    // x_k = inv^T b_k
+    double s = 0.0;
+    for (int c = tid; c < CH_NB; ++c) s += inv[c * CH_NB + tid] * bk[c];
+    y[tid] = tid < nb ? s : 0.0;
+    if (blockIdx.x == 0 && tid < nb) x_out[k0 + tid] = s;
+  }
+  __syncthreads();
+  // rows above: b[r] -= sum_c A[k0+c, r] y[c]; thread per row (coalesced along r)
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + tid; r < k0; r += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int c = 0; c < nb; ++c) s += A[(k0 + c) * lda + r] * y[c];
+    b[r] -= s;
+  }
+}
+
+// A failed factorisation leaves NaN in the solution, as LAPACK-based
+// cho_solve does on a non-positive-definite matrix.
+__global__ void nan_on_info_kernel(double* __restrict__ x, int64_t n, const int32_t* __restrict__ info) {
+  if (*info == 0) return;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = __longlong_as_double(0x7FF8000000000000LL);
+}
+
+// ---------------------------------------------------------------------------
+// Vector helpers
+// ---------------------------------------------------------------------------
+// y = alpha op(A) x + beta y.  trans 0: row dot products (one wave per row);
+// trans 1: column sums (thread per column); trans 2: symmetric from lower.
+__global__ void __launch_bounds__(256) gemv_n_kernel(const double* __restrict__ A, int64_t lda, int64_t m,
+                                                     int64_t n, const double* __restrict__ x, double alpha,
+                                                     double beta, double* __restrict__ y, int sym_lower) {
+  const int wid = threadIdx.x / 64, lane = lane_id();
+  for (int64_t r = (int64_t)blockIdx.x * 4 + wid; r < m; r += (int64_t)gridDim.x * 4) {
+    double s = 0.0;
+    if (!sym_lower) {
+      for (int64_t c = lane; c < n; c += 64) s += A[r * lda + c] * x[c];
+    } else {
+      for (int64_t c = lane; c <= r; c += 64) s += A[r * lda + c] * x[c];
+      for (int64_t c = r + 1 + lane; c < n; c += 64) s += A[c * lda + r] * x[c];
+    }
+    s = wave_sum(s);
+    if (lane == 0) y[r] = alpha * s + (beta != 0.0 ? beta * y[r] : 0.0);
+  }
+}
+
+__global__ void __launch_bounds__(256) gemv_t_partial(const double* __restrict__ A, int64_t lda, int64_t m,
+                                                      int64_t n, const double* __restrict__ x,
+                                                      double* __restrict__ part, int64_t rows_per) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.y * rows_per;
+  const int64_t r1 = min(m, r0 + rows_per);
+  if (c >= n) return;
+  double s = 0.0;
+  for (int64_t r = r0; r < r1; ++r) s += A[r * lda + c] * x[r];
+  part[(int64_t)blockIdx.y * n + c] = s;
+}
+
+__global__ void gemv_t_reduce(const double* __restrict__ part, int nparts, int64_t n, double alpha,
+                              double beta, double* __restrict__ y) {
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int p = 0; p < nparts; ++p) s += part[(int64_t)p * n + c];
+    y[c] = alpha * s + (beta != 0.0 ? beta * y[c] : 0.0);
+  }
+}
+
+__global__ void check_finite_kernel(const double* __restrict__ x, int64_t n, int32_t* __restrict__ flag) {
+  int bad = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    bad |= !isfinite(x[i]);
+  if (__any(bad) && lane_id() == 0) atomicOr(flag, 1);
+}
+
+template <class T>
+__global__ void sgd_kernel(T* __restrict__ p, const double* __restrict__ dp, int64_t n, T lr) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const T g = (T)dp[i];
+    const T u = -lr * g;
+    p[i] = p[i] + u;
+  }
+}
+
+__global__ void centre_scale_kernel(const double* __restrict__ e, int64_t n, const double* __restrict__ mean,
+                                    double scale, double* __restrict__ out) {
+  const double mu = *mean;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (e[i] - mu) * scale;
+}
+
+// Moments of data (C, L) for the blocking statistics: one workgroup.
+__device__ inline double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  const int wid = threadIdx.x / 64;
+  __syncthreads();
+  if (lane_id() == 0) red[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+  const int nw = blockDim.x / 64;
+  for (int w = 0; w < nw; ++w) t += red[w];
+  return t;
+}
+
+// Mean of the group means and variance around it (two passes); group g of
+// size gs starts at element start(g) of data (C, L).
+struct GroupSpec {
+  int kind;  // 0: chains (size L); 1: blocks of lb within each chain; 2: half chains
+  int C, L, lb, nbc, h;
+};
+__device__ inline int64_t group_count(const GroupSpec& g) {
+  return g.kind == 0 ? g.C : (g.kind == 1 ? (int64_t)g.C * g.nbc : 2 * (int64_t)g.C);
+}
+__device__ inline double group_mean(const double* d, const GroupSpec& g, int64_t q) {
+  int64_t start;
+  int sz;
+  if (g.kind == 0) {
+    start = q * g.L;
+    sz = g.L;
+  } else if (g.kind == 1) {
+    start = (q / g.nbc) * g.L + (q % g.nbc) * g.lb;
+    sz = g.lb;
+  } else {
+    start = (q / 2) * g.L + (q % 2) * g.h;
+    sz = g.h;
+  }
+  double a = 0.0;
+  for (int l = 0; l < sz; ++l) a += d[start + l];
+  return sz > 0 ? a / (double)sz : 0.0;
+}
+__device__ inline double group_var(const double* d, const GroupSpec& g, double* red) {
+  const int64_t cnt = group_count(g);
+  double s = 0.0;
+  for (int64_t q = threadIdx.x; q < cnt; q += blockDim.x) s += group_mean(d, g, q);
+  const double mu = block_sum(s, red) / (double)cnt;
+  double v = 0.0;
+  for (int64_t q = threadIdx.x; q < cnt; q += blockDim.x) {
+    const double x = group_mean(d, g, q) - mu;
+    v += x * x;
+  }
+  return block_sum(v, red) / (double)cnt;
+}
+
+__global__ void __launch_bounds__(1024) stats_kernel(const double* __restrict__ d, int C, int L,
+                                                     double* __restrict__ out) {
+  __shared__ double red[16];
+  const int64_t n = (int64_t)C * L;
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s += d[i];
+  const double mean = block_sum(s, red) / (double)n;
+  double v = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const double x = d[i] - mean;
+    v += x * x;
+  }
+  const double var = block_sum(v, red) / (double)n;
+  // block size l_block = max(1, L / 32) and floor(L / l_block) blocks per chain;
+  // split R-hat halves drop the last element when L is odd
+  GroupSpec g;
+  g.C = C;
+  g.L = L;
+  g.lb = max(1, L / 32);
+  g.nbc = L / g.lb;
+  g.h = L / 2;
+  g.kind = 0;
+  const double cvar = group_var(d, g, red);
+  g.kind = 1;
+  const double bvar = group_var(d, g, red);
+  g.kind = 2;
+  const double hvar = g.h > 0 ? group_var(d, g, red) : 0.0;
+  if (threadIdx.x == 0) {
+    out[0] = mean;
+    out[1] = var;
+    out[2] = cvar;
+    out[3] = (double)((int64_t)C * g.nbc);
+    out[4] = bvar;
+    out[5] = hvar;
+    out[6] = (double)g.lb;
+    out[7] = (double)n;
+  }
+}
+
+// Copy the lower triangle onto the upper one (row-major, FP64).
+__global__ void symmetrize_kernel(double* __restrict__ a, int64_t n, int64_t lda) {
+  const int64_t total = n * n;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = idx / n, j = idx % n;
+    if (j > i) a[i * lda + j] = a[j * lda + i];
+  }
+}
+
+}  // namespace nka
+
+using namespace nka;
+
+extern "C" {
+
+int nka_gram_bf16x3(const uint16_t* x_hi, const uint16_t* x_lo, int64_t n, int64_t k, int64_t ldx,
+                    double shift, double offset, double* out, int64_t ldo, void* stream) {
+  NKA_CHECK_ARG(n >= 0 && k >= 0 && ldx >= k && ldo >= n, "nka_gram_bf16x3: bad sizes");
+  NKA_CHECK_ARG(ldx % 8 == 0 && k % 8 == 0,
+                "nka_gram_bf16x3: k and ldx must be multiples of 8 (pad with zeros)");
+  NKA_CHECK_ARG(((uintptr_t)x_hi % 16) == 0 && ((uintptr_t)x_lo % 16) == 0,
+                "nka_gram_bf16x3: operands must be 16-byte aligned");
+  if (n == 0) return NKA_OK;
+  const int64_t T = (n + GB_T - 1) / GB_T;
+  const int64_t tiles = T * (T + 1) / 2;
+  NKA_CHECK_ARG(tiles < (1LL << 31), "nka_gram_bf16x3: matrix too large");
+  hipLaunchKernelGGL(gram_bf16x3_kernel, dim3(grid_mult8(tiles)), dim3(256), 0, (hipStream_t)stream,
+                     x_hi, x_lo, n, k, ldx, shift, offset, out, ldo, tiles);
+  NKA_CHECK_LAUNCH("nka_gram_bf16x3");
+  return NKA_OK;
+}
+
+static int gemm_launch(const double* a, int64_t lda, const double* b, int64_t ldb, double* c,
+                       int64_t ldc, int64_t m, int64_t n, int64_t k, double alpha, double beta,
+                       double shift, int lower, const int32_t* info, hipStream_t s) {
+  const int64_t tm = (m + GD_T - 1) / GD_T, tn = (n + GD_T - 1) / GD_T;
+  const int64_t tiles = lower ? tm * (tm + 1) / 2 : tm * tn;
+  if (tiles == 0) return NKA_OK;
+  hipLaunchKernelGGL(gemm_nt_f64_kernel, dim3(grid_mult8(tiles)), dim3(256), 0, s, a, lda, b, ldb, c,
+                     ldc, m, n, k, alpha, beta, shift, lower, tn, tiles, info);
+  return NKA_OK;
+}
+
+int nka_syrk_f64(const double* x, int64_t n, int64_t k, int64_t ldx, double alpha, double beta,
+                 double shift, double* c, int64_t ldc, void* stream) {
+  NKA_CHECK_ARG(n >= 0 && k >= 0 && ldx >= k && ldc >= n, "nka_syrk_f64: bad sizes");
+  gemm_launch(x, ldx, x, ldx, c, ldc, n, n, k, alpha, beta, shift, 1, nullptr, (hipStream_t)stream);
+  NKA_CHECK_LAUNCH("nka_syrk_f64");
+  return NKA_OK;
+}
+
+int nka_gemm_nt_f64(const double* a, int64_t lda, const double* b, int64_t ldb, double* c,
+                    int64_t ldc, int64_t m, int64_t n, int64_t k, double alpha, double beta,
+                    void* stream) {
+  NKA_CHECK_ARG(m >= 0 && n >= 0 && k >= 0 && lda >= k && ldb >= k && ldc >= n,
+                "nka_gemm_nt_f64: bad sizes");
+  gemm_launch(a, lda, b, ldb, c, ldc, m, n, k, alpha, beta, 0.0, 0, nullptr, (hipStream_t)stream);
+  NKA_CHECK_LAUNCH("nka_gemm_nt_f64");
+  return NKA_OK;
+}
+
+int64_t nka_potrf_workspace(int64_t n) {
+  if (n <= 0) return 0;
+  const int64_t nblk = (n + CH_NB - 1) / CH_NB;
+  return nblk * CH_NB * CH_NB * (int64_t)sizeof(double) + (n + 64) * (int64_t)sizeof(double);
+}
+
+int nka_potrf_f64(double* a, int64_t n, int64_t lda, void* workspace, int32_t* info,
+                  void* stream) {
+  NKA_CHECK_ARG(n >= 0 && lda >= n && a && workspace && info, "nka_potrf_f64: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  double* inv = (double*)workspace;
+  for (int64_t k0 = 0; k0 < n; k0 += CH_NB) {
+    const int64_t nb = std::min<int64_t>(CH_NB, n - k0);
+    hipLaunchKernelGGL(potf2_inv_kernel, dim3(1), dim3(256), 0, s, a, lda, n, k0,
+                       inv + (k0 / CH_NB) * CH_NB * CH_NB, info);
+    const int64_t rest = n - k0 - nb;
+    if (rest > 0) {
+      // panel: L21 = A21 L11^{-T}  (in place, single column tile of width nb)
+      double* a21 = a + (k0 + nb) * lda + k0;
+      gemm_launch(a21, lda, inv + (k0 / CH_NB) * CH_NB * CH_NB, CH_NB, a21, lda, rest, nb, nb, 1.0,
+                  0.0, 0.0, 0, info, s);
+      // trailing update: A22 -= L21 L21^T (lower)
+      double* a22 = a + (k0 + nb) * lda + (k0 + nb);
+      gemm_launch(a21, lda, a21, lda, a22, lda, rest, rest, nb, -1.0, 1.0, 0.0, 1, info, s);
+    }
+  }
+  NKA_CHECK_LAUNCH("nka_potrf_f64");
+  return NKA_OK;
+}
+
+int nka_potrs_f64(const double* a, int64_t n, int64_t lda, const void* workspace, double* b,
+                  const int32_t* info, void* stream) {
+  NKA_CHECK_ARG(n >= 0 && lda >= n && a && workspace && b && info, "nka_potrs_f64: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  const double* inv = (const double*)workspace;
+  const int64_t nblk = (n + CH_NB - 1) / CH_NB;
+  double* y = (double*)((char*)workspace + nblk * CH_NB * CH_NB * sizeof(double));
+  // forward: L y = b  (b is consumed; y collects the solution blocks)
+  for (int64_t kb = 0; kb < nblk; ++kb) {
+    const int64_t k0 = kb * CH_NB;
+    const int64_t rows = n - k0 - CH_NB;
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((rows + 3) / 4, 1024));
+    hipLaunchKernelGGL(trsv_fwd_kernel, dim3(g), dim3(256), 0, s, a, lda, n, k0,
+                       inv + kb * CH_NB * CH_NB, b, y, info);
+  }
+  // backward: L^T x = y  (y is consumed; x written into b)
+  for (int64_t kb = nblk - 1; kb >= 0; --kb) {
+    const int64_t k0 = kb * CH_NB;
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((k0 + 255) / 256, 1024));
+    hipLaunchKernelGGL(trsv_bwd_kernel, dim3(g), dim3(256), 0, s, a, lda, n, k0,
+                       inv + kb * CH_NB * CH_NB, y, b, info);
+  }
+  hipLaunchKernelGGL(nan_on_info_kernel, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 1024)),
+                     dim3(256), 0, s, b, n, info);
+  NKA_CHECK_LAUNCH("nka_potrs_f64");
+  return NKA_OK;
+}
+
+int nka_gemv_f64(const double* a, int64_t lda, int64_t m, int64_t n, int trans, const double* x,
+                 double alpha, double beta, double* y, void* workspace, void* stream) {
+  NKA_CHECK_ARG(m >= 0 && n >= 0 && a && x && y, "nka_gemv_f64: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  if (trans == 0 || trans == 2) {
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((m + 3) / 4, 4096));
+    hipLaunchKernelGGL(gemv_n_kernel, dim3(g), dim3(256), 0, s, a, lda, m, n, x, alpha, beta, y,
+                       trans == 2 ? 1 : 0);
+  } else if (trans == 1) {
+    NKA_CHECK_ARG(workspace != nullptr, "nka_gemv_f64: transposed product needs a workspace");
+    const int64_t rows_per = 256;
+    const int64_t parts = (m + rows_per - 1) / rows_per;
+    NKA_CHECK_ARG(parts <= 65535, "nka_gemv_f64: too many rows");
+    dim3 grid((unsigned)((n + 255) / 256), (unsigned)std::max<int64_t>(parts, 1));
+    hipLaunchKernelGGL(gemv_t_partial, grid, dim3(256), 0, s, a, lda, m, n, x, (double*)workspace,
+                       rows_per);
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 4096));
+    hipLaunchKernelGGL(gemv_t_reduce, dim3(g), dim3(256), 0, s, (const double*)workspace,
+                       (int)std::max<int64_t>(parts, 1), n, alpha, beta, y);
+  } else {
+    set_error("nka_gemv_f64: bad trans %d", trans);
+    return NKA_ERR_ARG;
+  }
+  NKA_CHECK_LAUNCH("nka_gemv_f64");
+  return NKA_OK;
+}
+
+int nka_check_finite(const double* x, int64_t n, int32_t* flag, void* stream) {
+  NKA_CHECK_ARG(n >= 0 && flag && (n == 0 || x), "nka_check_finite: bad arguments");
+  if (n == 0) return NKA_OK;
+  const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 1024);
+  hipLaunchKernelGGL(check_finite_kernel, dim3(g), dim3(256), 0, (hipStream_t)stream, x, n, flag);
+  NKA_CHECK_LAUNCH("nka_check_finite");
+  return NKA_OK;
+}
+
+int nka_sgd_update(void* params, int dtype, const double* dp, int64_t n, double lr, void* stream) {
+  NKA_CHECK_ARG(n >= 0 && (n == 0 || (params && dp)), "nka_sgd_update: bad arguments");
+  if (n == 0) return NKA_OK;
+  const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+  if (dtype == NKA_F32)
+    hipLaunchKernelGGL(sgd_kernel<float>, dim3(g), dim3(256), 0, (hipStream_t)stream,
+                       (float*)params, dp, n, (float)lr);
+  else if (dtype == NKA_F64)
+    hipLaunchKernelGGL(sgd_kernel<double>, dim3(g), dim3(256), 0, (hipStream_t)stream,
+                       (double*)params, dp, n, lr);
+  else {
+    set_error("nka_sgd_update: bad dtype");
+    return NKA_ERR_ARG;
+  }
+  NKA_CHECK_LAUNCH("nka_sgd_update");
+  return NKA_OK;
+}
+
+int nka_centre_scale(const double* e, int64_t n, const double* mean, double scale, double* out,
+                     void* stream) {
+  NKA_CHECK_ARG(n >= 0 && (n == 0 || (e && mean && out)), "nka_centre_scale: bad arguments");
+  if (n == 0) return NKA_OK;
+  const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(centre_scale_kernel, dim3(g), dim3(256), 0, (hipStream_t)stream, e, n, mean,
+                     scale, out);
+  NKA_CHECK_LAUNCH("nka_centre_scale");
+  return NKA_OK;
+}
+
+int nka_symmetrize_f64(double* a, int64_t n, int64_t lda, void* stream) {
+  NKA_CHECK_ARG(n >= 0 && lda >= n && (n == 0 || a), "nka_symmetrize_f64: bad arguments");
+  if (n == 0) return NKA_OK;
+  const unsigned g = (unsigned)std::min<int64_t>((n * n + 255) / 256, 65536);
+  hipLaunchKernelGGL(symmetrize_kernel, dim3(g), dim3(256), 0, (hipStream_t)stream, a, n, lda);
+  NKA_CHECK_LAUNCH("nka_symmetrize_f64");
+  return NKA_OK;
+}
+
+int nka_stats_moments(const double* data, int n_chains, int chain_length, double* out8,
+                      void* stream) {
+  NKA_CHECK_ARG(n_chains > 0 && chain_length > 0 && data && out8, "nka_stats_moments: bad arguments");
+  hipLaunchKernelGGL(stats_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, data, n_chains,
+                     chain_length, out8);
+  NKA_CHECK_LAUNCH("nka_stats_moments");
+  return NKA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,650 @@
+// RBM forward, fused local energy and Jacobian kernels.
+//
+// nka_rbm_log_psi       RBM.__call__ (netket/models/rbm.py:57-81)
+// nka_rbm_local_energy  local_value_kernel_jax + get_conn_padded
+//                       (netket/vqs/mc/kernels.py:62-71)
+// nka_rbm_prepare       tanh(theta) used by the analytic Jacobian
+// nka_rbm_contract      sum_s O[s,p] w[v,s]: centring means and forces
+//                       (netket/jax/_jacobian/logic.py:365-369,
+//                        netket/vqs/mc/mc_state/expect_forces.py:98-104)
+// nka_rbm_jacobian      centred, rescaled Jacobian in either layout
+//                       (netket/jax/_jacobian/logic.py:356-373)
+#include "nka_rbm.hpp"
+
+namespace nka {
+
+// ---------------------------------------------------------------------------
+// log psi
+// ---------------------------------------------------------------------------
+template <class T, int JL, bool STAGE>
+__global__ void __launch_bounds__(256) rbm_log_psi_kernel(nka_rbm m, const int8_t* __restrict__ sigma,
+                                                          int64_t n, T* __restrict__ out) {
+  extern __shared__ unsigned char smem[];
+  const RbmView<T> v = rbm_view<T>(m);
+  const int wid = threadIdx.x / NKA_WAVE, waves = blockDim.x / NKA_WAVE;
+  const size_t wbytes = STAGE ? pad16((size_t)v.N * v.M * sizeof(T)) : 0;
+  int8_t* xs = (int8_t*)(smem + wbytes) + wid * pad16(v.N);
+  const T* W = v.W;
+  if (STAGE) {
+    stage_w<T>((T*)smem, v.W, v.N, v.M);
+    W = (const T*)smem;
+  }
+  for (int64_t b = (int64_t)blockIdx.x * waves + wid; b < n; b += (int64_t)gridDim.x * waves) {
+    for (int i = lane_id(); i < v.N; i += NKA_WAVE) xs[i] = sigma[b * v.N + i];
+    wave_lds_sync();
+    HiddenState<T, JL> h;
+    compute_theta<T, JL>(h, xs, W, v.b, v.N, v.M);
+    const T lp = sum_log_cosh<T, JL>(h) + visible_term<T>(xs, v.a, v.N);
+    if (lane_id() == 0) out[b] = lp;
+    wave_lds_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Local energy, one wavefront per sample.
+// ---------------------------------------------------------------------------
+struct OpArgs {
+  int kind;
+  nka_ising ising;
+  nka_localop local;
+};
+
+__device__ inline double load_mel(const void* p, int64_t i, int dtype) {
+  return dtype == NKA_F64 ? ((const double*)p)[i] : (double)((const float*)p)[i];
+}
+
+template <class T, int JL, bool STAGE>
+__global__ void __launch_bounds__(256) rbm_eloc_kernel(nka_rbm m, OpArgs op, const int8_t* __restrict__ sigma,
+                                                       int64_t n, double* __restrict__ eloc,
+                                                       T* __restrict__ log_psi) {
+  extern __shared__ unsigned char smem[];
+  const RbmView<T> v = rbm_view<T>(m);
+  const int N = v.N, M = v.M;
+  const int wid = threadIdx.x / NKA_WAVE, waves = blockDim.x / NKA_WAVE;
+  const int lane = lane_id();
+  const size_t wbytes = STAGE ? pad16((size_t)N * M * sizeof(T)) : 0;
+  int8_t* xs = (int8_t*)(smem + wbytes) + wid * pad16(N);
+  const T* W = v.W;
+  if (STAGE) {
+    stage_w<T>((T*)smem, v.W, N, M);
+    W = (const T*)smem;
+  }
+  for (int64_t b = (int64_t)blockIdx.x * waves + wid; b < n; b += (int64_t)gridDim.x * waves) {
+    for (int i = lane; i < N; i += NKA_WAVE) xs[i] = sigma[b * N + i];
+    wave_lds_sync();
+    HiddenState<T, JL> h;
+    compute_theta<T, JL>(h, xs, W, v.b, N, M);
+    if (log_psi) {
+      const T lp = sum_log_cosh<T, JL>(h) + visible_term<T>(xs, v.a, N);
+      if (lane == 0) log_psi[b] = lp;
+    }
+    T nth[JL], nlc[JL];
+    double E = 0.0;
+    if (op.kind == NKA_OP_ISING) {
+      const nka_ising& is = op.ising;
+      int zz = 0;
+      for (int e = lane; e < is.n_edges; e += NKA_WAVE)
+        zz += (xs[is.edges[2 * e]] == xs[is.edges[2 * e + 1]]) ? 1 : -1;
+      zz = wave_sum(zz);
+      // mel of row 0 is J * (integer sum) in the operator dtype
+      const double diag =
+          is.mel_dtype == NKA_F64 ? is.J * (double)zz : (double)((float)is.J * (float)zz);
+      E += diag;
+      if (is.h != 0.0) {
+        const double mh = is.mel_dtype == NKA_F64 ? -is.h : (double)(-(float)is.h);
+        for (int r = 0; r < N; ++r) {
+          const int sites[1] = {r};
+          const T d = flip_delta<T, JL>(h, xs, W, v.a, N, M, sites, 1, nth, nlc);
+          E += mh * (double)exp_t<T>(d);
+        }
+      }
+    } else {
+      const nka_localop& lo = op.local;
+      // diagonal: constant + sum over terms of the selected row's diagonal element
+      double dpart = 0.0;
+      for (int t = lane; t < lo.n_ops; t += NKA_WAVE) {
+        const int ns = lo.op_nsites[t];
+        int r = 0;
+        for (int k = 0; k < ns; ++k) r = 2 * r + (xs[lo.op_sites[4 * t + k]] < 0 ? 1 : 0);
+        dpart += load_mel(lo.row_diag, lo.op_row_off[t] + r, lo.mel_dtype);
+      }
+      const double diag = lo.constant + wave_sum(dpart);
+      if (lo.nonzero_diagonal && fabs(diag) > lo.mel_cutoff) E += diag;
+      for (int t = 0; t < lo.n_ops; ++t) {
+        const int ns = lo.op_nsites[t];
+        int st[4];
+        int r = 0;
+        for (int k = 0; k < ns; ++k) {
+          st[k] = lo.op_sites[4 * t + k];
+          r = 2 * r + (xs[st[k]] < 0 ? 1 : 0);
+        }
+        r = uniform_value<int>(r);
+        const int row = lo.op_row_off[t] + r;
+        const int nc = lo.row_nconn[row];
+        const int off = lo.row_conn_off[row];
+        for (int c = 0; c < nc; ++c) {
+          const int nidx = lo.conn_newidx[off + c];
+          int fl[4];
+          int nf = 0;
+          for (int k = 0; k < ns; ++k)
+            if ((((nidx ^ r) >> (ns - 1 - k)) & 1) != 0) fl[nf++] = st[k];
+          const T d = flip_delta<T, JL>(h, xs, W, v.a, N, M, fl, nf, nth, nlc);
+          E += load_mel(lo.conn_mel, off + c, lo.mel_dtype) * (double)exp_t<T>(d);
+        }
+      }
+    }
+    if (lane == 0) eloc[b] = E;
+    wave_lds_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// t = tanh(theta): row-major (ns, M) and transposed (M, ns).
+// ---------------------------------------------------------------------------
+template <class T, int JL, bool STAGE>
+__global__ void __launch_bounds__(256) rbm_prepare_kernel(nka_rbm m, const int8_t* __restrict__ sigma, int64_t ns,
+                                                          T* __restrict__ t, T* __restrict__ tT) {
+  extern __shared__ unsigned char smem[];
+  const RbmView<T> v = rbm_view<T>(m);
+  const int N = v.N, M = v.M;
+  const int wid = threadIdx.x / NKA_WAVE, waves = blockDim.x / NKA_WAVE;
+  const int lane = lane_id();
+  const size_t wbytes = STAGE ? pad16((size_t)N * M * sizeof(T)) : 0;
+  int8_t* xs = (int8_t*)(smem + wbytes) + wid * pad16(N);
+  const T* W = v.W;
+  if (STAGE) {
+    stage_w<T>((T*)smem, v.W, N, M);
+    W = (const T*)smem;
+  }
+  for (int64_t b = (int64_t)blockIdx.x * waves + wid; b < ns; b += (int64_t)gridDim.x * waves) {
+    for (int i = lane; i < N; i += NKA_WAVE) xs[i] = sigma[b * N + i];
+    wave_lds_sync();
+    HiddenState<T, JL> h;
+    compute_theta<T, JL>(h, xs, W, v.b, N, M);
+#pragma unroll
+    for (int jj = 0; jj < JL; ++jj) {
+      const int j = lane + NKA_WAVE * jj;
+      if (j < M) {
+        const T tv = tanh_t<T>(h.th[jj]);
+        if (t) t[b * M + j] = tv;
+        if (tT) tT[(int64_t)j * ns + b] = tv;
+      }
+    }
+    wave_lds_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Weighted contraction over samples, split over sample chunks:
+//   part[c, v, p] = sum_{s in chunk c} O[s,p] w[v,s]
+// O[s, b_j] = t_sj, O[s, W_ij] = s_i t_sj, O[s, a_i] = s_i.
+// Each block owns a chunk of samples and a tile of hidden units; threads own
+// hidden units j and loop over the visible index i for the W block.
+// ---------------------------------------------------------------------------
+constexpr int kContractChunk = 256;   // samples per partial
+constexpr int kContractMaxW = 2;      // weight vectors per call
+
+template <class T, int NW>
+__global__ void __launch_bounds__(256) rbm_contract_partial(nka_rbm m, const int8_t* __restrict__ sigma,
+                                                            const T* __restrict__ t, int64_t ns,
+                                                            const double* __restrict__ w,
+                                                            double* __restrict__ part, int64_t np) {
+  // grid: x = chunk of kContractChunk samples, y = tile of 64 hidden units;
+  // block = 4 waves, lane = hidden unit j, wave wid takes visible rows
+  // i0 + wid + 4r of each 64-row slab.  The tanh values of the thread's hidden
+  // unit are read once per slab; partial sums stay in registers.
+  __shared__ double ws[NW][kContractChunk];
+  __shared__ int8_t sg[kContractChunk][64];
+  const int N = m.n_visible, M = m.n_hidden;
+  const int64_t s0 = (int64_t)blockIdx.x * kContractChunk;
+  const int cn = (int)min((int64_t)kContractChunk, ns - s0);
+  const int j = blockIdx.y * 64 + lane_id();
+  const int wid = threadIdx.x / 64;
+  for (int idx = threadIdx.x; idx < cn * NW; idx += blockDim.x) {
+    const int vv = idx / cn, s = idx % cn;
+    ws[vv][s] = w[(int64_t)vv * ns + s0 + s];
+  }
+  // partial layout: [chunk][v][np + 1]; slot np holds the chunk's sum of w[v]
+  const int64_t ld = np + 1;
+  double* out = part + (int64_t)blockIdx.x * NW * ld;
+  for (int i0 = 0; i0 < N; i0 += 64) {
+    const int ni = min(64, N - i0);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < cn * 64; idx += blockDim.x) {
+      const int s = idx / 64, ii = idx % 64;
+      sg[s][ii] = ii < ni ? sigma[(s0 + s) * N + i0 + ii] : (int8_t)0;
+    }
+    __syncthreads();
+    double acc[NW][16];
+#pragma unroll
+    for (int vv = 0; vv < NW; ++vv)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[vv][r] = 0.0;
+    if (j < M) {
+      for (int s = 0; s < cn; ++s) {
+        const double tv = (double)t[(s0 + s) * M + j];
+        double tw[NW];
+#pragma unroll
+        for (int vv = 0; vv < NW; ++vv) tw[vv] = tv * ws[vv][s];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int8_t sv = sg[s][wid + 4 * r];
+#pragma unroll
+          for (int vv = 0; vv < NW; ++vv) acc[vv][r] += sv > 0 ? tw[vv] : (sv < 0 ? -tw[vv] : 0.0);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int ii = wid + 4 * r;
+        if (ii < ni) {
+          const int64_t p = m.kernel_off + (int64_t)(i0 + ii) * M + j;
+#pragma unroll
+          for (int vv = 0; vv < NW; ++vv) out[(int64_t)vv * ld + p] = acc[vv][r];
+        }
+      }
+    }
+    // visible-bias entries of this slab (tile 0 only)
+    if (m.visible_off >= 0 && blockIdx.y == 0 && threadIdx.x < ni) {
+      const int ii = threadIdx.x;
+      double a2[NW];
+#pragma unroll
+      for (int vv = 0; vv < NW; ++vv) a2[vv] = 0.0;
+      for (int s = 0; s < cn; ++s) {
+        const double sv = sg[s][ii] > 0 ? 1.0 : -1.0;
+#pragma unroll
+        for (int vv = 0; vv < NW; ++vv) a2[vv] += sv * ws[vv][s];
+      }
+#pragma unroll
+      for (int vv = 0; vv < NW; ++vv) out[(int64_t)vv * ld + m.visible_off + i0 + ii] = a2[vv];
+    }
+  }
+  // per-chunk weight sums (used to centre the contraction)
+  if (blockIdx.y == 0 && wid == 1) {
+#pragma unroll
+    for (int vv = 0; vv < NW; ++vv) {
+      double sw = 0.0;
+      for (int s = lane_id(); s < cn; s += 64) sw += ws[vv][s];
+      sw = wave_sum(sw);
+      if (lane_id() == 0) out[(int64_t)vv * ld + np] = sw;
+    }
+  }
+  // hidden-bias entries
+  if (m.bias_off >= 0 && wid == 0 && j < M) {
+    double b2[NW];
+#pragma unroll
+    for (int vv = 0; vv < NW; ++vv) b2[vv] = 0.0;
+    for (int s = 0; s < cn; ++s) {
+      const double tv = (double)t[(s0 + s) * M + j];
+#pragma unroll
+      for (int vv = 0; vv < NW; ++vv) b2[vv] += tv * ws[vv][s];
+    }
+#pragma unroll
+    for (int vv = 0; vv < NW; ++vv) out[(int64_t)vv * ld + m.bias_off + j] = b2[vv];
+  }
+}
+
+__global__ void __launch_bounds__(256) contract_reduce(const double* __restrict__ part, int n_chunks, int n_w,
+                                                       int64_t np, const double* __restrict__ mean,
+                                                       double* __restrict__ out) {
+  // out[v, p] = sum_c part[c, v, p] - mean[p] * sum_c part[c, v, np]
+  const int64_t ld = np + 1;
+  const int64_t len = (int64_t)n_w * np;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < len;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = q / np, p = q % np;
+    double acc = 0.0, sw = 0.0;
+    for (int c = 0; c < n_chunks; ++c) {
+      acc += part[((int64_t)c * n_w + v) * ld + p];
+      if (mean) sw += part[((int64_t)c * n_w + v) * ld + np];
+    }
+    out[q] = mean ? acc - mean[p] * sw : acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Centred Jacobian writer.
+// ---------------------------------------------------------------------------
+__device__ inline uint16_t bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ inline float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ inline void store_out(int kind, void* a, void* b, int64_t idx, double val) {
+  if (kind == NKA_OUT_F64) {
+    ((double*)a)[idx] = val;
+  } else if (kind == NKA_OUT_F32) {
+    ((float*)a)[idx] = (float)val;
+  } else {
+    const float f = (float)val;
+    const uint16_t hi = bf16_rne(f);
+    const uint16_t lo = bf16_rne(f - bf16_to_f(hi));
+    ((uint16_t*)a)[idx] = hi;
+    ((uint16_t*)b)[idx] = lo;
+  }
+}
+
+// Parameter-major layout: X[p, s]; one thread per (p, s) with s fastest.
+template <class T>
+__global__ void __launch_bounds__(256) rbm_jac_ps_kernel(nka_rbm m, const int8_t* __restrict__ sigma,
+                                                         const T* __restrict__ tT, int64_t ns,
+                                                         const double* __restrict__ mean, double scale,
+                                                         int kind, void* out_a, void* out_b, int64_t np,
+                                                         int64_t ldo) {
+  const int N = m.n_visible, M = m.n_hidden;
+  const int64_t total = np * ns;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = idx / ns, s = idx % ns;
+    const int64_t o_idx = p * ldo + s;
+    double o;
+    if (m.bias_off >= 0 && p >= m.bias_off && p < m.bias_off + M) {
+      o = (double)tT[(p - m.bias_off) * ns + s];
+    } else if (p >= m.kernel_off && p < m.kernel_off + (int64_t)N * M) {
+      const int64_t q = p - m.kernel_off;
+      const int i = (int)(q / M), j = (int)(q % M);
+      const double tv = (double)tT[(int64_t)j * ns + s];
+      o = sigma[s * N + i] > 0 ? tv : -tv;
+    } else {
+      const int i = (int)(p - m.visible_off);
+      o = sigma[s * N + i] > 0 ? 1.0 : -1.0;
+    }
+    store_out(kind, out_a, out_b, o_idx, (o - mean[p]) * scale);
+  }
+}
+
+// Sample-major layout: X[s, p]; one thread per (s, p) with p fastest.
+template <class T>
+__global__ void __launch_bounds__(256) rbm_jac_sp_kernel(nka_rbm m, const int8_t* __restrict__ sigma,
+                                                         const T* __restrict__ t, int64_t ns,
+                                                         const double* __restrict__ mean, double scale,
+                                                         int kind, void* out_a, void* out_b, int64_t np,
+                                                         int64_t ldo) {
+  const int N = m.n_visible, M = m.n_hidden;
+  const int64_t total = np * ns;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = idx / np, p = idx % np;
+    const int64_t o_idx = s * ldo + p;
+    double o;
+    if (m.bias_off >= 0 && p >= m.bias_off && p < m.bias_off + M) {
+      o = (double)t[s * M + (p - m.bias_off)];
+    } else if (p >= m.kernel_off && p < m.kernel_off + (int64_t)N * M) {
+      const int64_t q = p - m.kernel_off;
+      const int i = (int)(q / M), j = (int)(q % M);
+      const double tv = (double)t[s * M + j];
+      o = sigma[s * N + i] > 0 ? tv : -tv;
+    } else {
+      const int i = (int)(p - m.visible_off);
+      o = sigma[s * N + i] > 0 ? 1.0 : -1.0;
+    }
+    store_out(kind, out_a, out_b, o_idx, (o - mean[p]) * scale);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+static int64_t n_params(const nka_rbm& m) {
+  return (m.bias_off >= 0 ? m.n_hidden : 0) + (int64_t)m.n_visible * m.n_hidden +
+         (m.visible_off >= 0 ? m.n_visible : 0);
+}
+
+static int check_rbm(const nka_rbm* m, const char* who) {
+  if (!m || !m->params) {
+    set_error("%s: null model", who);
+    return NKA_ERR_ARG;
+  }
+  if (m->n_visible <= 0 || m->n_hidden <= 0 || m->n_hidden > kMaxJL * NKA_WAVE) {
+    set_error("%s: unsupported sizes N=%d M=%d (need 0 < M <= %d)", who, m->n_visible,
+              m->n_hidden, kMaxJL * NKA_WAVE);
+    return NKA_ERR_ARG;
+  }
+  if (m->dtype != NKA_F32 && m->dtype != NKA_F64) {
+    set_error("%s: bad dtype", who);
+    return NKA_ERR_ARG;
+  }
+  if (m->kernel_off < 0) {
+    set_error("%s: kernel offset missing", who);
+    return NKA_ERR_ARG;
+  }
+  return NKA_OK;
+}
+
+template <class T>
+static bool stage_fits(const nka_rbm& m, int waves) {
+  return pad16((size_t)m.n_visible * m.n_hidden * sizeof(T)) + waves * pad16(m.n_visible) <=
+         kLdsBudget;
+}
+
+static unsigned grid_for(int64_t n, int waves, int64_t cap) {
+  int64_t g = (n + waves - 1) / waves;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+template <int JL, class T>
+static void launch_log_psi(const nka_rbm& m, const int8_t* sigma, int64_t n, T* out, hipStream_t s) {
+  const int waves = 4;
+  const bool stage = stage_fits<T>(m, waves);
+  const size_t lds = (stage ? pad16((size_t)m.n_visible * m.n_hidden * sizeof(T)) : 0) +
+                     waves * pad16(m.n_visible);
+  const unsigned g = grid_for(n, waves, 2048);
+  if (stage)
+    hipLaunchKernelGGL((rbm_log_psi_kernel<T, JL, true>), dim3(g), dim3(64 * waves), lds, s, m,
+                       sigma, n, out);
+  else
+    hipLaunchKernelGGL((rbm_log_psi_kernel<T, JL, false>), dim3(g), dim3(64 * waves), lds, s, m,
+                       sigma, n, out);
+}
+template <int JL>
+static void launch_log_psi_f32(const nka_rbm& m, const int8_t* sg, int64_t n, void* o, hipStream_t s) {
+  launch_log_psi<JL, float>(m, sg, n, (float*)o, s);
+}
+template <int JL>
+static void launch_log_psi_f64(const nka_rbm& m, const int8_t* sg, int64_t n, void* o, hipStream_t s) {
+  launch_log_psi<JL, double>(m, sg, n, (double*)o, s);
+}
+
+template <int JL, class T>
+static void launch_eloc(const nka_rbm& m, const OpArgs& op, const int8_t* sigma, int64_t n,
+                        double* eloc, T* lp, hipStream_t s) {
+  const int waves = 4;
+  const bool stage = stage_fits<T>(m, waves);
+  const size_t lds = (stage ? pad16((size_t)m.n_visible * m.n_hidden * sizeof(T)) : 0) +
+                     waves * pad16(m.n_visible);
+  const unsigned g = grid_for(n, waves, 2048);
+  if (stage)
+    hipLaunchKernelGGL((rbm_eloc_kernel<T, JL, true>), dim3(g), dim3(64 * waves), lds, s, m, op,
+                       sigma, n, eloc, lp);
+  else
+    hipLaunchKernelGGL((rbm_eloc_kernel<T, JL, false>), dim3(g), dim3(64 * waves), lds, s, m, op,
+                       sigma, n, eloc, lp);
+}
+template <int JL>
+static void launch_eloc_f32(const nka_rbm& m, const OpArgs& op, const int8_t* sg, int64_t n,
+                            double* e, void* lp, hipStream_t s) {
+  launch_eloc<JL, float>(m, op, sg, n, e, (float*)lp, s);
+}
+template <int JL>
+static void launch_eloc_f64(const nka_rbm& m, const OpArgs& op, const int8_t* sg, int64_t n,
+                            double* e, void* lp, hipStream_t s) {
+  launch_eloc<JL, double>(m, op, sg, n, e, (double*)lp, s);
+}
+
+template <int JL, class T>
+static void launch_prepare(const nka_rbm& m, const int8_t* sigma, int64_t ns, T* t, T* tT,
+                           hipStream_t s) {
+  const int waves = 4;
+  const bool stage = stage_fits<T>(m, waves);
+  const size_t lds = (stage ? pad16((size_t)m.n_visible * m.n_hidden * sizeof(T)) : 0) +
+                     waves * pad16(m.n_visible);
+  const unsigned g = grid_for(ns, waves, 2048);
+  if (stage)
+    hipLaunchKernelGGL((rbm_prepare_kernel<T, JL, true>), dim3(g), dim3(64 * waves), lds, s, m,
+                       sigma, ns, t, tT);
+  else
+    hipLaunchKernelGGL((rbm_prepare_kernel<T, JL, false>), dim3(g), dim3(64 * waves), lds, s, m,
+                       sigma, ns, t, tT);
+}
+template <int JL>
+static void launch_prepare_f32(const nka_rbm& m, const int8_t* sg, int64_t ns, void* t, void* tT,
+                               hipStream_t s) {
+  launch_prepare<JL, float>(m, sg, ns, (float*)t, (float*)tT, s);
+}
+template <int JL>
+static void launch_prepare_f64(const nka_rbm& m, const int8_t* sg, int64_t ns, void* t, void* tT,
+                               hipStream_t s) {
+  launch_prepare<JL, double>(m, sg, ns, (double*)t, (double*)tT, s);
+}
+
+}  // namespace nka
+
+using namespace nka;
+
+extern "C" {
+
+int nka_rbm_log_psi(const nka_rbm* m, const int8_t* sigma, int64_t n, void* out, void* stream) {
+  int st = check_rbm(m, "nka_rbm_log_psi");
+  if (st) return st;
+  NKA_CHECK_ARG(n >= 0 && (n == 0 || (sigma && out)), "nka_rbm_log_psi: bad buffers");
+  if (n == 0) return NKA_OK;
+  const int jl = pick_jl(m->n_hidden);
+  hipStream_t s = (hipStream_t)stream;
+  if (m->dtype == NKA_F32) {
+    NKA_DISPATCH_JL(jl, launch_log_psi_f32, *m, sigma, n, out, s);
+  } else {
+    NKA_DISPATCH_JL(jl, launch_log_psi_f64, *m, sigma, n, out, s);
+  }
+  NKA_CHECK_LAUNCH("nka_rbm_log_psi");
+  return NKA_OK;
+}
+
+int nka_rbm_local_energy(const nka_rbm* m, int op_kind, const void* op, const int8_t* sigma,
+                         int64_t n, double* eloc, void* log_psi, void* stream) {
+  int st = check_rbm(m, "nka_rbm_local_energy");
+  if (st) return st;
+  NKA_CHECK_ARG(op != nullptr, "nka_rbm_local_energy: null operator");
+  NKA_CHECK_ARG(n >= 0 && (n == 0 || (sigma && eloc)), "nka_rbm_local_energy: bad buffers");
+  OpArgs a{};
+  a.kind = op_kind;
+  if (op_kind == NKA_OP_ISING) {
+    a.ising = *(const nka_ising*)op;
+    NKA_CHECK_ARG(a.ising.n_edges >= 0 && (a.ising.n_edges == 0 || a.ising.edges),
+                  "nka_rbm_local_energy: bad Ising edges");
+  } else if (op_kind == NKA_OP_LOCAL) {
+    a.local = *(const nka_localop*)op;
+    NKA_CHECK_ARG(a.local.n_ops >= 0, "nka_rbm_local_energy: bad local operator");
+  } else {
+    set_error("nka_rbm_local_energy: unknown operator kind %d", op_kind);
+    return NKA_ERR_ARG;
+  }
+  if (n == 0) return NKA_OK;
+  const int jl = pick_jl(m->n_hidden);
+  hipStream_t s = (hipStream_t)stream;
+  if (m->dtype == NKA_F32) {
+    NKA_DISPATCH_JL(jl, launch_eloc_f32, *m, a, sigma, n, eloc, log_psi, s);
+  } else {
+    NKA_DISPATCH_JL(jl, launch_eloc_f64, *m, a, sigma, n, eloc, log_psi, s);
+  }
+  NKA_CHECK_LAUNCH("nka_rbm_local_energy");
+  return NKA_OK;
+}
+
+int nka_rbm_prepare(const nka_rbm* m, const int8_t* sigma, int64_t ns, void* t, void* tT,
+                    void* stream) {
+  int st = check_rbm(m, "nka_rbm_prepare");
+  if (st) return st;
+  NKA_CHECK_ARG(ns >= 0 && (ns == 0 || sigma), "nka_rbm_prepare: bad buffers");
+  if (ns == 0) return NKA_OK;
+  const int jl = pick_jl(m->n_hidden);
+  hipStream_t s = (hipStream_t)stream;
+  if (m->dtype == NKA_F32) {
+    NKA_DISPATCH_JL(jl, launch_prepare_f32, *m, sigma, ns, t, tT, s);
+  } else {
+    NKA_DISPATCH_JL(jl, launch_prepare_f64, *m, sigma, ns, t, tT, s);
+  }
+  NKA_CHECK_LAUNCH("nka_rbm_prepare");
+  return NKA_OK;
+}
+
+int64_t nka_rbm_contract_workspace(const nka_rbm* m, int64_t ns, int n_w) {
+  if (!m || ns <= 0 || n_w <= 0) return 0;
+  const int64_t chunks = (ns + kContractChunk - 1) / kContractChunk;
+  return chunks * n_w * (n_params(*m) + 1) * (int64_t)sizeof(double);
+}
+
+int nka_rbm_contract(const nka_rbm* m, const int8_t* sigma, const void* t, int64_t ns,
+                     const double* w, int n_w, const double* mean, double* out, void* workspace,
+                     void* stream) {
+  int st = check_rbm(m, "nka_rbm_contract");
+  if (st) return st;
+  NKA_CHECK_ARG(n_w >= 1 && n_w <= kContractMaxW, "nka_rbm_contract: n_w must be 1 or 2");
+  NKA_CHECK_ARG(ns > 0 && sigma && t && w && out && workspace, "nka_rbm_contract: bad buffers");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t np = n_params(*m);
+  const int chunks = (int)((ns + kContractChunk - 1) / kContractChunk);
+  dim3 grid(chunks, (m->n_hidden + 63) / 64);
+  double* ws = (double*)workspace;
+  if (m->dtype == NKA_F32) {
+    if (n_w == 1)
+      hipLaunchKernelGGL((rbm_contract_partial<float, 1>), grid, dim3(256), 0, s, *m, sigma,
+                         (const float*)t, ns, w, ws, np);
+    else
+      hipLaunchKernelGGL((rbm_contract_partial<float, 2>), grid, dim3(256), 0, s, *m, sigma,
+                         (const float*)t, ns, w, ws, np);
+  } else {
+    if (n_w == 1)
+      hipLaunchKernelGGL((rbm_contract_partial<double, 1>), grid, dim3(256), 0, s, *m, sigma,
+                         (const double*)t, ns, w, ws, np);
+    else
+      hipLaunchKernelGGL((rbm_contract_partial<double, 2>), grid, dim3(256), 0, s, *m, sigma,
+                         (const double*)t, ns, w, ws, np);
+  }
+  const int64_t len = (int64_t)n_w * np;
+  unsigned g = (unsigned)std::min<int64_t>((len + 255) / 256, 4096);
+  hipLaunchKernelGGL(contract_reduce, dim3(g), dim3(256), 0, s, (const double*)workspace, chunks,
+                     n_w, np, mean, out);
+  NKA_CHECK_LAUNCH("nka_rbm_contract");
+  return NKA_OK;
+}
+
+int nka_rbm_jacobian(const nka_rbm* m, const int8_t* sigma, const void* t, const void* tT,
+                     int64_t ns, const double* mean, double scale, int layout, int out_kind,
+                     void* out_a, void* out_b, int64_t ldo, void* stream) {
+  int st = check_rbm(m, "nka_rbm_jacobian");
+  if (st) return st;
+  NKA_CHECK_ARG(ns > 0 && sigma && mean && out_a, "nka_rbm_jacobian: bad buffers");
+  NKA_CHECK_ARG(out_kind == NKA_OUT_F32 || out_kind == NKA_OUT_F64 ||
+                    (out_kind == NKA_OUT_BF16X2 && out_b),
+                "nka_rbm_jacobian: bad output kind");
+  NKA_CHECK_ARG(layout == NKA_LAYOUT_PS ? tT != nullptr : (layout == NKA_LAYOUT_SP && t),
+                "nka_rbm_jacobian: bad layout / missing t");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t np = n_params(*m);
+  NKA_CHECK_ARG(ldo >= (layout == NKA_LAYOUT_PS ? ns : np), "nka_rbm_jacobian: ldo too small");
+  const int64_t total = np * ns;
+  const unsigned g = (unsigned)std::min<int64_t>((total + 255) / 256, 65536);
+  if (layout == NKA_LAYOUT_PS) {
+    if (m->dtype == NKA_F32)
+      hipLaunchKernelGGL(rbm_jac_ps_kernel<float>, dim3(g), dim3(256), 0, s, *m, sigma,
+                         (const float*)tT, ns, mean, scale, out_kind, out_a, out_b, np, ldo);
+    else
+      hipLaunchKernelGGL(rbm_jac_ps_kernel<double>, dim3(g), dim3(256), 0, s, *m, sigma,
+                         (const double*)tT, ns, mean, scale, out_kind, out_a, out_b, np, ldo);
+  } else {
+    if (m->dtype == NKA_F32)
+      hipLaunchKernelGGL(rbm_jac_sp_kernel<float>, dim3(g), dim3(256), 0, s, *m, sigma,
+                         (const float*)t, ns, mean, scale, out_kind, out_a, out_b, np, ldo);
+    else
+      hipLaunchKernelGGL(rbm_jac_sp_kernel<double>, dim3(g), dim3(256), 0, s, *m, sigma,
+                         (const double*)t, ns, mean, scale, out_kind, out_a, out_b, np, ldo);
+  }
+  NKA_CHECK_LAUNCH("nka_rbm_jacobian");
+  return NKA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,318 @@
+// Metropolis-Hastings sampler for the RBM, one wavefront per Markov chain.
+//
+// Semantics (netket/sampler/metropolis.py:382-505, netket/vqs/mc/mc_state/state.py:521-576):
+//   reset: log_prob = machine_pow * Re log psi(sigma)          (metropolis.py:398-403)
+//   repeat n_discard + chain_length times a sweep of sweep_size proposals;
+//   each proposal sigma' is accepted when u < exp(log_prob' - log_prob + corr)
+//   (metropolis.py:441-450); after each of the last chain_length sweeps the
+//   chain state is recorded (metropolis.py:462-464, 492-499).
+// Rules:
+//   LocalRule    : uniform site, spin flipped (rules/local.py:40-49,
+//                  hilbert/random/homogeneous.py:146-170)
+//   ExchangeRule : a cluster (i, j) uniformly among those with s_i != s_j,
+//                  swapped; corr = log n_diff(sigma) - log n_diff(sigma')
+//                  (rules/exchange.py:143-218)
+// The wavefront keeps theta_j and log_cosh(theta_j) of its chain in registers
+// (64 hidden units per register slot) and the spins in LDS, so a proposal
+// reads only the W rows of the flipped sites.
+#include "nka_rbm.hpp"
+
+namespace nka {
+
+// ---------------------------------------------------------------------------
+// Initial states
+// ---------------------------------------------------------------------------
+__global__ void random_states_kernel(int8_t* __restrict__ sigma, int C, int N, int n_down,
+                                     uint32_t k0, uint32_t k1, uint64_t counter) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  int8_t* x = sigma + (int64_t)c * N;
+  if (n_down < 0) {
+    for (int i0 = 0; i0 < N; i0 += 128) {
+      const uint64_t q = counter + (uint64_t)(i0 / 128);
+      u32x4 r = philox4x32_10({(uint32_t)q, (uint32_t)(q >> 32), (uint32_t)c, TAG_RANDSTATE}, k0, k1);
+      const uint32_t words[4] = {r.x, r.y, r.z, r.w};
+      for (int i = i0; i < min(N, i0 + 128); ++i) {
+        const int bit = (words[(i - i0) >> 5] >> ((i - i0) & 31)) & 1;
+        x[i] = bit ? (int8_t)-1 : (int8_t)1;
+      }
+    }
+  } else {
+    // n_down spins -1 at uniformly random positions: Fisher-Yates shuffle of
+    // [-1]*n_down + [+1]*(N-n_down), swap partner j uniform in [0, i].
+    for (int i = 0; i < N; ++i) x[i] = (i < n_down) ? (int8_t)-1 : (int8_t)1;
+    for (int i = N - 1; i > 0; --i) {
+      const uint64_t q = counter + (uint64_t)i;
+      u32x4 r = philox4x32_10({(uint32_t)q, (uint32_t)(q >> 32), (uint32_t)c, TAG_RANDSTATE}, k0, k1);
+      const int j = (int)bounded(r.x, (uint32_t)(i + 1));
+      const int8_t tmp = x[i];
+      x[i] = x[j];
+      x[j] = tmp;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sweeps
+// ---------------------------------------------------------------------------
+constexpr int kMaxClusterSlots = 16;  // clusters per lane: n_clusters <= 1024
+
+struct SweepArgs {
+  int rule;
+  const int32_t* clusters;
+  int n_clusters;
+  int C, sweep_size, n_discard, chain_length;
+  double machine_pow;
+  uint32_t k0, k1;
+  uint64_t counter;
+};
+
+template <class T, int JL, bool STAGE>
+__global__ void __launch_bounds__(256) rbm_metropolis_kernel(nka_rbm m, SweepArgs sa, int8_t* __restrict__ sigma,
+                                                             T* __restrict__ log_prob, int8_t* __restrict__ samples,
+                                                             T* __restrict__ lp_samples,
+                                                             int64_t* __restrict__ n_accepted) {
+  extern __shared__ unsigned char smem[];
+  const RbmView<T> v = rbm_view<T>(m);
+  const int N = v.N, M = v.M;
+  const int wid = threadIdx.x / NKA_WAVE, waves = blockDim.x / NKA_WAVE;
+  const int lane = lane_id();
+  const size_t wbytes = STAGE ? pad16((size_t)N * M * sizeof(T)) : 0;
+  int8_t* xs = (int8_t*)(smem + wbytes) + wid * pad16(N);
+  const T* W = v.W;
+  if (STAGE) {
+    stage_w<T>((T*)smem, v.W, N, M);
+    W = (const T*)smem;
+  }
+  const int c = blockIdx.x * waves + wid;
+  if (c >= sa.C) return;
+
+  for (int i = lane; i < N; i += NKA_WAVE) xs[i] = sigma[(int64_t)c * N + i];
+  wave_lds_sync();
+  HiddenState<T, JL> h;
+  compute_theta<T, JL>(h, xs, W, v.b, N, M);
+  const T pw = (T)sa.machine_pow;
+  T lp = pw * (sum_log_cosh<T, JL>(h) + visible_term<T>(xs, v.a, N));
+
+  // exchange clusters owned by this lane: cl = lane + 64 * r
+  const bool exch = sa.rule == NKA_RULE_EXCHANGE;
+  const int RS = exch ? (sa.n_clusters + NKA_WAVE - 1) / NKA_WAVE : 0;
+  int cu[kMaxClusterSlots], cv[kMaxClusterSlots];
+#pragma unroll
+  for (int r = 0; r < kMaxClusterSlots; ++r) {
+    const int cl = lane + NKA_WAVE * r;
+    if (r < RS && cl < sa.n_clusters) {
+      cu[r] = sa.clusters[2 * cl];
+      cv[r] = sa.clusters[2 * cl + 1];
+    } else {
+      cu[r] = -1;
+      cv[r] = -1;
+    }
+  }
+
+  int64_t nacc = 0;
+  uint64_t q = sa.counter;
+  T nth[JL], nlc[JL];
+  const int n_sweeps = sa.n_discard + sa.chain_length;
+  for (int sw = 0; sw < n_sweeps; ++sw) {
+    for (int p = 0; p < sa.sweep_size; ++p, ++q) {
+      const u32x4 rnd = philox4x32_10({(uint32_t)q, (uint32_t)(q >> 32), (uint32_t)c, TAG_METROPOLIS},
+                                      sa.k0, sa.k1);
+      int sites[2];
+      int nf;
+      T corr = (T)0;
+      bool valid = true;
+      if (!exch) {
+        sites[0] = (int)bounded(uniform_value<uint32_t>(rnd.x), (uint32_t)N);
+        nf = 1;
+      } else {
+        // differing clusters of the current state
+        uint64_t bal[kMaxClusterSlots];
+        int n_diff = 0;
+#pragma unroll
+        for (int r = 0; r < kMaxClusterSlots; ++r) {
+          if (r < RS) {
+            const bool d = cu[r] >= 0 && xs[cu[r]] != xs[cv[r]];
+            bal[r] = __ballot(d);
+            n_diff += __popcll(bal[r]);
+          } else {
+            bal[r] = 0;
+          }
+        }
+        n_diff = uniform_value<int>(n_diff);
+        if (n_diff == 0) {
+          valid = false;
+          nf = 0;
+        } else {
+          int k = (int)bounded(uniform_value<uint32_t>(rnd.x), (uint32_t)n_diff);
+          int sel = -1;
+#pragma unroll
+          for (int r = 0; r < kMaxClusterSlots; ++r) {
+            if (sel < 0 && r < RS) {
+              const int cnt = __popcll(bal[r]);
+              if (k < cnt) {
+                uint64_t mb = bal[r];
+                for (int z = 0; z < k; ++z) mb &= mb - 1;
+                sel = r * NKA_WAVE + (int)__builtin_ctzll(mb);
+              } else {
+                k -= cnt;
+              }
+            }
+          }
+          sel = uniform_value<int>(sel);
+          if (sel < 0 || sel >= sa.n_clusters) sel = 0;  // unreachable: k < n_diff
+          const int su = sa.clusters[2 * sel], sv = sa.clusters[2 * sel + 1];
+          sites[0] = su;
+          sites[1] = sv;
+          nf = 2;
+          // differing clusters after the swap (sigma'_u = sigma_v, sigma'_v = sigma_u)
+          int n_new = 0;
+#pragma unroll
+          for (int r = 0; r < kMaxClusterSlots; ++r) {
+            if (r < RS) {
+              bool d = false;
+              if (cu[r] >= 0) {
+                const int a = cu[r], bb = cv[r];
+                const int8_t xa = a == su ? xs[sv] : (a == sv ? xs[su] : xs[a]);
+                const int8_t xb = bb == su ? xs[sv] : (bb == sv ? xs[su] : xs[bb]);
+                d = xa != xb;
+              }
+              n_new += __popcll(__ballot(d));
+            }
+          }
+          n_new = uniform_value<int>(n_new);
+          corr = uniform_value<T>(log_t<T>((T)n_diff) - log_t<T>((T)n_new));
+        }
+      }
+      if (valid) {
+        const T delta =
+            uniform_value<T>(flip_delta<T, JL>(h, xs, W, v.a, N, M, sites, nf, nth, nlc));
+        const T u = uniform_value<T>(uniform01<T>(rnd));
+        const bool acc = u < exp_t<T>(pw * delta + corr);
+        if (acc) {
+#pragma unroll
+          for (int jj = 0; jj < JL; ++jj) {
+            h.th[jj] = nth[jj];
+            h.lc[jj] = nlc[jj];
+          }
+          if (lane == 0) {
+            for (int f = 0; f < nf; ++f) xs[sites[f]] = (int8_t)(-xs[sites[f]]);
+          }
+          lp += pw * delta;
+          ++nacc;
+          wave_lds_sync();
+        }
+      }
+    }
+    if (sw >= sa.n_discard) {
+      const int t = sw - sa.n_discard;
+      int8_t* dst = samples + ((int64_t)c * sa.chain_length + t) * N;
+      for (int i = lane; i < N; i += NKA_WAVE) dst[i] = xs[i];
+      if (lp_samples && lane == 0) lp_samples[(int64_t)c * sa.chain_length + t] = lp;
+    }
+  }
+  for (int i = lane; i < N; i += NKA_WAVE) sigma[(int64_t)c * N + i] = xs[i];
+  if (lane == 0) {
+    log_prob[c] = lp;
+    n_accepted[c] += nacc;
+  }
+}
+
+template <class T>
+static bool stage_fits_s(const nka_rbm& m, int waves) {
+  return pad16((size_t)m.n_visible * m.n_hidden * sizeof(T)) + waves * pad16(m.n_visible) <=
+         kLdsBudget;
+}
+
+template <int JL, class T>
+static void launch_sweep(const nka_rbm& m, const SweepArgs& sa, int8_t* sigma, T* lp, int8_t* smp,
+                         T* lps, int64_t* nacc, hipStream_t s) {
+  const int waves = 4;
+  const bool stage = stage_fits_s<T>(m, waves);
+  const size_t lds = (stage ? pad16((size_t)m.n_visible * m.n_hidden * sizeof(T)) : 0) +
+                     waves * pad16(m.n_visible);
+  const unsigned g = (unsigned)((sa.C + waves - 1) / waves);
+  if (stage)
+    hipLaunchKernelGGL((rbm_metropolis_kernel<T, JL, true>), dim3(g), dim3(64 * waves), lds, s, m,
+                       sa, sigma, lp, smp, lps, nacc);
+  else
+    hipLaunchKernelGGL((rbm_metropolis_kernel<T, JL, false>), dim3(g), dim3(64 * waves), lds, s,
+                       m, sa, sigma, lp, smp, lps, nacc);
+}
+template <int JL>
+static void launch_sweep_f32(const nka_rbm& m, const SweepArgs& sa, int8_t* sg, void* lp,
+                             int8_t* smp, void* lps, int64_t* na, hipStream_t s) {
+  launch_sweep<JL, float>(m, sa, sg, (float*)lp, smp, (float*)lps, na, s);
+}
+template <int JL>
+static void launch_sweep_f64(const nka_rbm& m, const SweepArgs& sa, int8_t* sg, void* lp,
+                             int8_t* smp, void* lps, int64_t* na, hipStream_t s) {
+  launch_sweep<JL, double>(m, sa, sg, (double*)lp, smp, (double*)lps, na, s);
+}
+
+}  // namespace nka
+
+using namespace nka;
+
+extern "C" {
+
+int nka_random_states(int8_t* sigma, int n_chains, int n_sites, int n_down, uint64_t seed,
+                      uint64_t counter, void* stream) {
+  NKA_CHECK_ARG(n_chains >= 0 && n_sites > 0 && (n_chains == 0 || sigma),
+                "nka_random_states: bad sizes");
+  NKA_CHECK_ARG(n_down <= n_sites, "nka_random_states: n_down > n_sites");
+  if (n_chains == 0) return NKA_OK;
+  hipLaunchKernelGGL(random_states_kernel, dim3((n_chains + 127) / 128), dim3(128), 0,
+                     (hipStream_t)stream, sigma, n_chains, n_sites, n_down, (uint32_t)seed,
+                     (uint32_t)(seed >> 32), counter);
+  NKA_CHECK_LAUNCH("nka_random_states");
+  return NKA_OK;
+}
+
+int nka_rbm_metropolis(const nka_rbm* m, int rule, const int32_t* clusters, int n_clusters,
+                       int8_t* sigma, void* log_prob, int n_chains, int sweep_size,
+                       int n_discard, int chain_length, double machine_pow, uint64_t seed,
+                       uint64_t counter, int8_t* samples, void* log_prob_samples,
+                       int64_t* n_accepted, void* stream) {
+  NKA_CHECK_ARG(m && m->params, "nka_rbm_metropolis: null model");
+  NKA_CHECK_ARG(m->n_visible > 0 && m->n_hidden > 0 && m->n_hidden <= kMaxJL * NKA_WAVE,
+                "nka_rbm_metropolis: unsupported sizes");
+  NKA_CHECK_ARG(m->dtype == NKA_F32 || m->dtype == NKA_F64, "nka_rbm_metropolis: bad dtype");
+  NKA_CHECK_ARG(rule == NKA_RULE_LOCAL || rule == NKA_RULE_EXCHANGE,
+                "nka_rbm_metropolis: unknown rule %d", rule);
+  NKA_CHECK_ARG(rule == NKA_RULE_LOCAL ||
+                    (clusters && n_clusters > 0 && n_clusters <= kMaxClusterSlots * NKA_WAVE),
+                "nka_rbm_metropolis: exchange rule needs 1..%d clusters",
+                kMaxClusterSlots * NKA_WAVE);
+  NKA_CHECK_ARG(n_chains >= 0 && sweep_size >= 0 && n_discard >= 0 && chain_length >= 0,
+                "nka_rbm_metropolis: bad sizes");
+  NKA_CHECK_ARG(n_chains == 0 || (sigma && log_prob && n_accepted &&
+                                  (chain_length == 0 || samples)),
+                "nka_rbm_metropolis: null buffer");
+  if (n_chains == 0) return NKA_OK;
+  SweepArgs sa;
+  sa.rule = rule;
+  sa.clusters = clusters;
+  sa.n_clusters = rule == NKA_RULE_EXCHANGE ? n_clusters : 0;
+  sa.C = n_chains;
+  sa.sweep_size = sweep_size;
+  sa.n_discard = n_discard;
+  sa.chain_length = chain_length;
+  sa.machine_pow = machine_pow;
+  sa.k0 = (uint32_t)seed;
+  sa.k1 = (uint32_t)(seed >> 32);
+  sa.counter = counter;
+  const int jl = pick_jl(m->n_hidden);
+  hipStream_t s = (hipStream_t)stream;
+  if (m->dtype == NKA_F32) {
+    NKA_DISPATCH_JL(jl, launch_sweep_f32, *m, sa, sigma, log_prob, samples, log_prob_samples,
+                    n_accepted, s);
+  } else {
+    NKA_DISPATCH_JL(jl, launch_sweep_f64, *m, sa, sigma, log_prob, samples, log_prob_samples,
+                    n_accepted, s);
+  }
+  NKA_CHECK_LAUNCH("nka_rbm_metropolis");
+  return NKA_OK;
+}
+
+}  // extern "C"

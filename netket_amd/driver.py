@@ -1,0 +1,211 @@
+"""Variational drivers.
+
+* ``VMC``     netket/driver/vmc.py:35-161: expect_and_grad, preconditioner,
+              update (netket/_src/driver/abstract_optimization_driver.py:86-92).
+* ``VMC_SR``  netket/_src/driver/vmc_sr.py:86-510: SR (use_ntk=False) or
+              minSR/NTK (use_ntk=True) with the linear algebra of
+              netket/_src/ngd/sr_srt_common.py:44-188, sr.py:16-80, srt.py:14-96.
+* ``run``     the step loop of AbstractDriver.run
+              (netket/_src/driver/abstract_variational_driver.py:349-499) with
+              loggers and callbacks (a callback returning False stops the run).
+"""
+
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import _engine as eng
+from . import _native as nat
+from . import _ops
+from .logging import RuntimeLog
+from .optimizer import solver as solvers
+from .optimizer.sr import Sgd, identity_preconditioner
+from .stats import statistics
+
+
+class AbstractVariationalDriver:
+    def __init__(self, variational_state, optimizer, minimized_quantity_name: str = "Energy"):
+        if not isinstance(optimizer, Sgd):
+            raise NotImplementedError("only the Sgd optimiser is supported on the GPU path")
+        self.state = variational_state
+        self.optimizer = optimizer
+        self._loss_name = minimized_quantity_name
+        self._step_count = 0
+        self._loss_stats = None
+        self._dp = None
+
+    @property
+    def step_count(self) -> int:
+        return self._step_count
+
+    def reset(self):
+        self.state.reset()
+        self._step_count = 0
+
+    def reset_step(self):
+        self.state.reset()
+
+    def compute_loss_and_update(self):
+        raise NotImplementedError
+
+    def update_parameters(self, dp_flat: torch.Tensor):
+        """p <- p + (-lr * (dtype)dp) on the flat parameter buffer."""
+        eng.sgd_update_(self.state.parameters_flat, dp_flat, self.optimizer.lr_at(self._step_count))
+        self.state.reset()
+
+    def _log_additional_data(self, log_data: dict):
+        acc = self.state.sampler_state.acceptance
+        if acc is not None:
+            log_data["acceptance"] = acc
+
+    def run(self, n_iter: int, out=None, obs=None, step_size: int = 1, show_progress: bool = False,
+            callback=None, timeit: bool = False):
+        if out is None:
+            loggers = ()
+        elif isinstance(out, (list, tuple)):
+            loggers = tuple(out)
+        else:
+            loggers = (out,)
+        callbacks = () if callback is None else (callback if isinstance(callback, (list, tuple)) else (callback,))
+        for _ in range(int(n_iter)):
+            self.reset_step()
+            self._loss_stats, dp = self.compute_loss_and_update()
+            log_data = {}
+            if self._loss_stats is not None:
+                log_data[self._loss_name] = self._loss_stats
+            if obs:
+                for name, op in obs.items():
+                    log_data[name] = self.state.expect(op)
+            self._log_additional_data(log_data)
+            stop = False
+            for cb in callbacks:
+                if cb(self._step_count, log_data, self) is False:
+                    stop = True
+            if self._step_count % step_size == 0:
+                for lg in loggers:
+                    lg(self._step_count, log_data, self.state)
+            self.update_parameters(dp)
+            self._step_count += 1
+            if stop:
+                break
+        for lg in loggers:
+            if hasattr(lg, "flush"):
+                lg.flush(self.state)
+        return loggers
+
+    @property
+    def energy(self):
+        return self._loss_stats
+
+
+class VMC(AbstractVariationalDriver):
+    """Energy minimisation by VMC with an optional preconditioner (e.g. SR)."""
+
+    def __init__(self, hamiltonian, optimizer, *, variational_state, preconditioner=None):
+        super().__init__(variational_state, optimizer)
+        self._ham = hamiltonian
+        self.preconditioner = preconditioner if preconditioner is not None else identity_preconditioner
+
+    def compute_loss_and_update(self):
+        self.state.reset()
+        st, g, _ = self.state.expect_and_forces_flat(self._ham)
+        if self.preconditioner is identity_preconditioner:
+            dp = g
+        else:
+            dp = self.preconditioner(self.state, g, self._step_count)
+            if isinstance(dp, dict):
+                dp = self.state.flatten_tree(dp)
+        return st, dp
+
+
+class VMC_SR(AbstractVariationalDriver):
+    """VMC with SR or minSR (NTK) natural-gradient updates."""
+
+    def __init__(self, hamiltonian, optimizer, *, diag_shift, variational_state, proj_reg=None,
+                 momentum=None, linear_solver=solvers.cholesky_with_fallback, linear_solver_fn=None,
+                 chunk_size_bwd=None, mode=None, use_ntk=None, on_the_fly=None):
+        super().__init__(variational_state, optimizer)
+        if linear_solver_fn is not None:
+            linear_solver = linear_solver_fn
+        if mode not in (None, "real"):
+            raise NotImplementedError("only mode='real' is supported")
+        if use_ntk is None:
+            use_ntk = variational_state.n_parameters > variational_state.n_samples
+        if on_the_fly is None:
+            on_the_fly = False
+        elif on_the_fly and not use_ntk:
+            raise ValueError("`on_the_fly=True` is only supported when `use_ntk=True`.")
+        if on_the_fly:
+            raise NotImplementedError("on-the-fly NTK is not supported; pass on_the_fly=False")
+        if momentum is not None and not use_ntk:
+            pass
+        self._ham = hamiltonian
+        self.diag_shift = diag_shift
+        self.proj_reg = proj_reg
+        self.momentum = momentum
+        self.use_ntk = bool(use_ntk)
+        self.on_the_fly = False
+        self.linear_solver = linear_solver
+        self._old_updates = None
+        self._jac = None
+        self.info = None
+
+    def _hyper(self, v):
+        return v(self._step_count) if callable(v) else v
+
+    def compute_loss_and_update(self):
+        vs = self.state
+        eloc = vs.local_estimators(self._ham)
+        st = statistics(eloc)
+        e = eloc.reshape(-1)
+        ns = e.numel()
+        diag_shift = float(self._hyper(self.diag_shift))
+        proj_reg = self._hyper(self.proj_reg)
+        momentum = self._hyper(self.momentum)
+        pc = vs._prepared()
+        sig, t = pc["sigma"], pc["t"]
+        model, flat = vs.model, vs.parameters_flat
+        need_tT = not self.use_ntk
+        tT = eng.prepare(model, flat, sig, row_major=False, transposed=True)[1] if need_tT else None
+        ones = torch.full((1, ns), 1.0 / ns, dtype=torch.float64, device=e.device)
+        mean = eng.contract(model, flat, sig, t, ones)[0]
+        # dv = 2 (E - <E>) / sqrt(Ns)   (sr_srt_common.py:75-77)
+        dv = eng.centre_scale(e, st.mean_device, 2.0 / math.sqrt(ns))
+        kind = nat.OUT_BF16X2 if flat.dtype == torch.float32 else nat.OUT_F64
+        layout = nat.LAYOUT_SP if self.use_ntk else nat.LAYOUT_PS
+        self._jac = eng.jacobian(model, flat, sig, t, tT, mean, 1.0 / math.sqrt(ns), layout, kind,
+                                 self._jac)
+        X = self._jac
+        if not self.use_ntk:
+            if proj_reg is not None:
+                raise ValueError("proj_reg not implemented for SR")
+            # F = O_L^T dv  (the contraction weights are dv / sqrt(Ns))
+            F = eng.contract(model, flat, sig, t, (dv / math.sqrt(ns)).reshape(1, -1), mean)[0]
+            if momentum is not None and self._old_updates is not None:
+                F = F + diag_shift * momentum * self._old_updates
+            system = solvers.DenseSystem(builder=lambda: eng.gram(X, diag_shift))
+            x, info = self._solve(system, F)
+            if momentum is not None:
+                self._old_updates = x
+            dp = x
+        else:
+            if momentum is not None and self._old_updates is not None:
+                raise NotImplementedError("SPRING momentum with the NTK is not supported")
+            offset = 0.0 if proj_reg is None else float(proj_reg) / ns
+            system = solvers.DenseSystem(builder=lambda: eng.gram(X, diag_shift, offset))
+            y, info = self._solve(system, dv)
+            # updates = O_L^T y (srt.py:79), from the analytic Jacobian
+            dp = eng.contract(model, flat, sig, t, (y / math.sqrt(ns)).reshape(1, -1), mean)[0]
+        self.info = info
+        return st, dp
+
+    def _solve(self, system, b):
+        res = self.linear_solver(system, b)
+        if isinstance(res, tuple):
+            x, info = res
+            return x, (info or {})
+        return res, {}

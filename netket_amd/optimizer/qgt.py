@@ -1,0 +1,96 @@
+"""Dense Jacobian quantum geometric tensor (QGTJacobianDense).
+
+Follows QGTJacobian_DefaultConstructor and QGTJacobianDenseT
+(netket/optimizer/qgt/qgt_jacobian.py:30-206,
+netket/optimizer/qgt/qgt_jacobian_dense.py:29-161, 251-270) for real
+parameters ("real" mode): O_L = (O - <O>) / sqrt(Ns) with FP64 centring,
+S = O_L^T O_L + diag_shift I.
+
+On the GPU O_L is stored parameter-major (X = O_L^T, Np x Ns); for FP32
+models as a bf16 hi/lo split, and S is formed with three BF16 MFMA products
+per tile (FP32 accumulate); FP64 models keep O_L and S in FP64 (FP64 MFMA).
+S is built on demand and not kept after a Cholesky solve consumed it.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .. import _engine as eng
+from .. import _native as nat
+from .. import _ops
+
+
+class LinearOperator:
+    def __call__(self, vec):
+        return self @ vec
+
+    def solve(self, solve_fun, y, *, x0=None, **kwargs):
+        """Solve S x = y with ``solve_fun(S, y, x0=...) -> (x, info)``
+        (netket/optimizer/linear_operator.py:86-103)."""
+        return solve_fun(self, y, x0=x0, **kwargs)
+
+
+class QGTJacobianDenseT(LinearOperator):
+    def __init__(self, vstate, *, diag_shift: float = 0.0, diag_scale=None, mode=None,
+                 holomorphic=None, chunk_size=None):
+        if diag_scale is not None and diag_scale != 0.0:
+            raise NotImplementedError("diag_scale (scale-invariant regularisation) is not supported")
+        if mode not in (None, "real"):
+            raise NotImplementedError("only real-parameter ('real' mode) QGTs are supported")
+        self.vstate = vstate
+        self.diag_shift = float(diag_shift)
+        s = vstate.samples
+        C, L, N = s.shape
+        self.n_samples = C * L
+        sig = s.reshape(C * L, N)
+        model, flat = vstate.model, vstate.parameters_flat
+        t, tT = eng.prepare(model, flat, sig, row_major=True, transposed=True)
+        ones = torch.full((1, self.n_samples), 1.0 / self.n_samples, dtype=torch.float64,
+                          device=sig.device)
+        self.mean = eng.contract(model, flat, sig, t, ones)[0]
+        kind = nat.OUT_BF16X2 if flat.dtype == torch.float32 else nat.OUT_F64
+        self.X = eng.jacobian(model, flat, sig, t, tT, self.mean, eng.inv_sqrt(self.n_samples),
+                              nat.LAYOUT_PS, kind)
+        self.n_params = self.X.rows
+        self._dense = None
+
+    # -- matrices ---------------------------------------------------------------
+    def lower(self) -> torch.Tensor:
+        """A fresh FP64 matrix whose lower triangle is S + diag_shift I."""
+        return eng.gram(self.X, self.diag_shift)
+
+    def to_dense(self) -> torch.Tensor:
+        if self._dense is None:
+            a = self.lower()
+            _ops.symmetrize_(a)
+            self._dense = a
+        return self._dense
+
+    def __matmul__(self, vec):
+        v = self.vstate.flatten_tree(vec) if not isinstance(vec, torch.Tensor) else vec.to(torch.float64)
+        out = _ops.gemv(self.to_dense(), v.contiguous())
+        if isinstance(vec, dict):
+            return self.vstate.unflatten(out)
+        return out
+
+    def __repr__(self):
+        return f"QGTJacobianDense(diag_shift={self.diag_shift}, mode=real)"
+
+
+def QGTJacobianDense(vstate=None, *, diag_shift: float = 0.0, diag_scale=None, mode=None,
+                     holomorphic=None, chunk_size=None, **kwargs):
+    """Constructor (or partial when called without a state)."""
+    if vstate is None:
+        def ctor(vs, **kw):
+            args = dict(diag_shift=diag_shift, diag_scale=diag_scale, mode=mode,
+                        holomorphic=holomorphic, chunk_size=chunk_size)
+            args.update(kw)
+            return QGTJacobianDenseT(vs, **args)
+        return ctor
+    return QGTJacobianDenseT(vstate, diag_shift=diag_shift, diag_scale=diag_scale, mode=mode,
+                             holomorphic=holomorphic, chunk_size=chunk_size)
+
+
+QGTAuto = QGTJacobianDense

@@ -1,0 +1,105 @@
+"""Metropolis sampling — oracle (test infrastructure only).
+
+``metropolis`` restates MetropolisSampler._sample_chain/_sample_next
+(netket/sampler/metropolis.py:416-505) with LocalRule
+(netket/sampler/rules/local.py:40-49) or ExchangeRule
+(netket/sampler/rules/exchange.py:143-218), driven by the same Philox4x32-10
+draws as the GPU kernel (proposal q of chain c at counter (q, c, "METR")):
+word 0 picks the site / the k-th differing cluster, words 1-2 the uniform.
+log psi is recomputed with a full forward per proposal (the reference
+algorithm), in FP64.
+
+``random_states`` restates the kernel's initial-state draw (uniform spins, or
+a Fisher-Yates placement of the down spins).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import philox
+from .rbm import log_psi
+
+
+def random_states(C: int, N: int, n_down: int | None, seed: int, counter: int = 0) -> np.ndarray:
+    out = np.empty((C, N), dtype=np.int8)
+    for c in range(C):
+        if n_down is None:
+            x = np.empty(N, dtype=np.int8)
+            for i0 in range(0, N, 128):
+                r = philox.draw(counter + i0 // 128, c, philox.TAG_RANDSTATE, seed)
+                for i in range(i0, min(N, i0 + 128)):
+                    bit = (int(r[(i - i0) >> 5]) >> ((i - i0) & 31)) & 1
+                    x[i] = -1 if bit else 1
+        else:
+            x = np.array([-1] * n_down + [1] * (N - n_down), dtype=np.int8)
+            for i in range(N - 1, 0, -1):
+                r = philox.draw(counter + i, c, philox.TAG_RANDSTATE, seed)
+                j = philox.bounded(r[0], i + 1)
+                x[i], x[j] = x[j], x[i]
+        out[c] = x
+    return out
+
+
+def exchange_moves(x: np.ndarray, clusters: np.ndarray):
+    """Proposals of the exchange rule from x: (list of x', log-correction list).
+
+    Each x' swaps the two sites of one cluster whose spins differ; it is drawn
+    with probability 1 / len(list); the correction is
+    log n_diff(x) - log n_diff(x') (netket/sampler/rules/exchange.py:143-184).
+    """
+    diff = np.nonzero(x[clusters[:, 0]] != x[clusters[:, 1]])[0]
+    moves, corrs = [], []
+    for c in diff:
+        u_, v_ = clusters[c]
+        xp = x.copy()
+        xp[u_], xp[v_] = x[v_], x[u_]
+        n_new = int(np.count_nonzero(xp[clusters[:, 0]] != xp[clusters[:, 1]]))
+        moves.append(xp)
+        corrs.append(np.log(diff.size) - np.log(n_new))
+    return moves, corrs
+
+
+def local_moves(x: np.ndarray):
+    """Proposals of the local rule: flip one of the N sites, each with probability 1/N."""
+    moves = []
+    for i in range(x.size):
+        xp = x.copy()
+        xp[i] = -xp[i]
+        moves.append(xp)
+    return moves, [0.0] * x.size
+
+
+def metropolis(params, sigma0: np.ndarray, *, rule: str, clusters=None, n_discard: int,
+               chain_length: int, sweep_size: int, seed: int, counter: int,
+               machine_pow: float = 2.0):
+    """Returns (samples (C, L, N), final sigma, n_accepted per chain)."""
+    sigma = np.array(sigma0, dtype=np.int8, copy=True)
+    C, N = sigma.shape
+    samples = np.empty((C, chain_length, N), dtype=np.int8)
+    nacc = np.zeros(C, dtype=np.int64)
+    if clusters is not None:
+        clusters = np.asarray(clusters, dtype=np.int64)
+    for c in range(C):
+        x = sigma[c].astype(np.int8).copy()
+        lp = machine_pow * log_psi(params, x[None])[0]
+        q = counter
+        for sw in range(n_discard + chain_length):
+            for _ in range(sweep_size):
+                r = philox.draw(q, c, philox.TAG_METROPOLIS, seed)
+                q += 1
+                moves, corrs = local_moves(x) if rule == "local" else exchange_moves(x, clusters)
+                if not moves:
+                    continue
+                k = philox.bounded(r[0], len(moves))
+                xp, corr = moves[k], corrs[k]
+                lpp = machine_pow * log_psi(params, xp[None])[0]
+                u = philox.uniform01_f64(r)
+                if u < np.exp(lpp - lp + corr):
+                    x = xp
+                    lp = lpp
+                    nacc[c] += 1
+            if sw >= n_discard:
+                samples[c, sw - n_discard] = x
+        sigma[c] = x
+    return samples, sigma, nacc

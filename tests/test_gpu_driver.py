@@ -1,0 +1,56 @@
+"""End-to-end VMC on the GPU: convergence to the exact ground state and the
+SR == minSR equivalence of whole runs (test/driver/test_srt.py:98-211)."""
+
+import numpy as np
+import pytest
+
+import netket_amd as nk
+from netket_amd.optimizer import solver as gsolver
+from oracle import operators as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _tfim_state(L, n_chains, n_samples, n_discard, seed, alpha=1, dtype=np.float64):
+    g = nk.graph.Chain(L)
+    hi = nk.hilbert.Spin(0.5, N=L)
+    H = nk.operator.Ising(hi, g, h=1.0, J=1.0)
+    sa = nk.sampler.MetropolisLocal(hi, n_chains=n_chains)
+    vs = nk.vqs.MCState(sa, nk.models.RBM(alpha=alpha, param_dtype=dtype), n_samples=n_samples,
+                        n_discard_per_chain=n_discard, seed=seed)
+    return H, vs
+
+
+def test_vmc_sr_tfim_l16_converges(gpu):
+    # Config 1: TFIM L=16 PBC h=J=1, FP64 RBM alpha=1, MetropolisLocal 16 chains,
+    # 1008 samples, 16 discarded sweeps, SR (QGTJacobianDense, diag_shift 0.01,
+    # Cholesky), SGD; the learning rate is raised from 0.01 to 0.05 so that 300
+    # steps reach the converged regime.
+    H, vs = _tfim_state(16, 16, 1008, 16, seed=0)
+    e0 = O.tfim_chain_e0(16, 1.0)
+    sr = nk.optimizer.SR(nk.optimizer.QGTJacobianDense, solver=gsolver.cholesky, diag_shift=0.01)
+    drv = nk.VMC(H, nk.optimizer.Sgd(0.05), variational_state=vs, preconditioner=sr)
+    log = nk.logging.RuntimeLog()
+    drv.run(300, out=log)
+    e = log.data["Energy"]["Mean"]
+    tail = float(np.mean(e[-50:]))
+    rel = abs(tail - e0) / abs(e0)
+    print(f"E(last 50) = {tail:.6f}, E0 = {e0:.6f}, rel err = {rel:.2e}", flush=True)
+    assert e[0] > e[-1]
+    assert rel < 2e-3
+
+
+def test_vmc_sr_and_minsr_runs_agree(gpu):
+    # same initial state and sampler seed: SR and NTK updates give the same trajectory
+    energies = {}
+    for ntk in (False, True):
+        H, vs = _tfim_state(8, 8, 64, 4, seed=3)
+        drv = nk.VMC_SR(H, nk.optimizer.Sgd(0.05), diag_shift=0.01, variational_state=vs,
+                        use_ntk=ntk, linear_solver=gsolver.cholesky)
+        log = nk.logging.RuntimeLog()
+        drv.run(5, out=log)
+        energies[ntk] = log.data["Energy"]["Mean"]
+        params = vs.parameters_flat.cpu().numpy()
+        energies[(ntk, "p")] = params
+    np.testing.assert_allclose(energies[False], energies[True], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(energies[(False, "p")], energies[(True, "p")], rtol=1e-8, atol=1e-12)

@@ -1,0 +1,118 @@
+"""GPU parity of the Metropolis sampler: identical chains to the oracle on the
+same Philox stream (FP64), and the sampled distribution vs exact |psi|^2."""
+
+import numpy as np
+import pytest
+import torch
+
+import netket_amd as nk
+from oracle import operators as O
+from oracle import rbm as orbm
+from oracle import sampler as osamp
+
+pytestmark = pytest.mark.gpu
+
+
+def _model(N, alpha, dtype, seed, scale):
+    model = nk.models.RBM(alpha=alpha, param_dtype=dtype)
+    lay = model.layout(N)
+    rng = np.random.default_rng(seed)
+    flat_np = rng.normal(scale=scale, size=lay["n_params"]).astype(dtype)
+    return model, torch.as_tensor(flat_np, device="cuda"), orbm.RBMParams(
+        flat_np.astype(np.float64), N, lay["M"])
+
+
+@pytest.mark.parametrize("N,n_down", [(10, None), (12, 6), (64, 32), (7, 3)])
+def test_random_states_match_oracle(gpu, N, n_down):
+    print(f"case random_states N={N} n_down={n_down}", flush=True)
+    hi = nk.hilbert.Spin(0.5, N=N, total_sz=None if n_down is None else (N - 2 * n_down) / 2)
+    sa = nk.sampler.MetropolisLocal(hi, n_chains=37)
+    st = sa.init_state(None, None, seed=1234)
+    ref = osamp.random_states(37, N, n_down, 1234, 0)
+    np.testing.assert_array_equal(st.sigma.cpu().numpy(), ref)
+    if n_down is not None:
+        assert np.all((st.sigma.cpu().numpy() < 0).sum(axis=1) == n_down)
+
+
+@pytest.mark.parametrize("rule", ["local", "exchange"])
+def test_chains_identical_to_oracle_fp64(gpu, rule):
+    print(f"case chains {rule}", flush=True)
+    N = 10
+    model, flat, p = _model(N, 2, np.float64, 3, 0.4)
+    g = nk.graph.Chain(N)
+    if rule == "local":
+        hi = nk.hilbert.Spin(0.5, N=N)
+        sa = nk.sampler.MetropolisLocal(hi, n_chains=6, sweep_size=7)
+        clusters = None
+    else:
+        hi = nk.hilbert.Spin(0.5, N=N, total_sz=0)
+        sa = nk.sampler.MetropolisExchange(hi, graph=g, n_chains=6, sweep_size=7)
+        clusters = sa.rule.clusters
+    st = sa.init_state(model, flat, seed=99)
+    sigma0 = st.sigma.cpu().numpy().copy()
+    counter0 = st.counter
+    samples, st = sa.sample_fused(model, flat, st, 3, 9)
+    ref, ref_sigma, ref_acc = osamp.metropolis(p, sigma0, rule=rule, clusters=clusters, n_discard=3,
+                                               chain_length=9, sweep_size=7, seed=99, counter=counter0)
+    np.testing.assert_array_equal(samples.cpu().numpy(), ref)
+    np.testing.assert_array_equal(st.sigma.cpu().numpy(), ref_sigma)
+    np.testing.assert_array_equal(st.n_accepted_proc.cpu().numpy(), ref_acc)
+    # log_prob of the chain state is 2 Re log psi of the final configurations
+    np.testing.assert_allclose(st.log_prob.cpu().numpy(), 2 * orbm.log_psi(p, ref_sigma), rtol=1e-12)
+
+
+def _kl(counts, probs):
+    q = counts / counts.sum()
+    m = q > 0
+    return float(np.sum(q[m] * np.log(q[m] / probs[m])))
+
+
+@pytest.mark.parametrize("rule,dtype", [("local", np.float64), ("local", np.float32),
+                                        ("exchange", np.float64), ("exchange", np.float32)])
+def test_distribution_matches_exact(gpu, rule, dtype):
+    # sampled frequencies vs exact |psi|^2 (FullSumState semantics), KL divergence
+    print(f"case distribution {rule} {np.dtype(dtype)}", flush=True)
+    # weights small enough that |psi|^2 is not sharply multimodal, so chains
+    # started from random states equilibrate within the discarded sweeps
+    N = 8
+    model, flat, p = _model(N, 1, dtype, 7, 0.25)
+    if rule == "local":
+        hi = nk.hilbert.Spin(0.5, N=N)
+        sa = nk.sampler.MetropolisLocal(hi, n_chains=1024)
+        states = O.states_all(N)
+    else:
+        hi = nk.hilbert.Spin(0.5, N=N, total_sz=0)
+        sa = nk.sampler.MetropolisExchange(hi, graph=nk.graph.Chain(N), n_chains=1024)
+        states = O.states_all(N, n_down=4)
+    st = sa.init_state(model, flat, seed=5)
+    samples, st = sa.sample_fused(model, flat, st, 100, 200)
+    s = samples.cpu().numpy().reshape(-1, N)
+    keys = O.state_index(s)
+    order = np.argsort(O.state_index(states))
+    skeys = O.state_index(states)[order]
+    pos = np.searchsorted(skeys, keys)
+    assert np.all(skeys[pos] == keys), "sample outside the Hilbert space"
+    counts = np.bincount(pos, minlength=len(states)).astype(float)
+    probs = orbm.exact_probability(p, states[order])
+    kl = _kl(counts, probs)
+    print(f"KL = {kl:.3e} over {len(states)} states, {s.shape[0]} samples", flush=True)
+    # expected KL of an exact sampler ~ (K-1)/(2 n) = 6e-4 (local), 2e-4 (exchange)
+    assert kl < 4e-3
+    if rule == "exchange":
+        assert np.all((s < 0).sum(axis=1) == N // 2)
+
+
+def test_mcstate_sample_shapes_and_reset(gpu):
+    N = 12
+    hi = nk.hilbert.Spin(0.5, N=N)
+    sa = nk.sampler.MetropolisLocal(hi, n_chains=16)
+    vs = nk.vqs.MCState(sa, nk.models.RBM(alpha=1), n_samples=1008, n_discard_per_chain=16, seed=0)
+    assert vs.chain_length == 63 and vs.n_samples == 1008
+    s1 = vs.samples
+    assert tuple(s1.shape) == (16, 63, N)
+    assert vs.samples is s1
+    vs.reset()
+    s2 = vs.samples
+    assert s2 is not s1
+    acc = vs.sampler_state.acceptance
+    assert 0.0 < acc <= 1.0

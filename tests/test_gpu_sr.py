@@ -1,0 +1,188 @@
+"""GPU parity of statistics, energy gradient, QGT/NTK Gram matrices and the
+dense solvers vs the oracle, on identical sample sets."""
+
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import torch
+
+import netket_amd as nk
+from netket_amd import _engine as eng
+from netket_amd import _native as nat
+from netket_amd import _ops
+from netket_amd.optimizer import solver as gsolver
+from oracle import ngd as ongd
+from oracle import operators as O
+from oracle import rbm as orbm
+from oracle import stats as ostats
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("C,L", [(16, 63), (4096, 4), (4, 256), (4, 4), (1, 100), (33, 7)])
+def test_statistics(gpu, C, L):
+    print(f"case stats {C}x{L}", flush=True)
+    rng = np.random.default_rng(C * 1000 + L)
+    data = rng.normal(loc=-3.0, size=(C, L)) + 0.3 * np.cumsum(rng.normal(size=(C, L)), axis=1)
+    st = nk.stats.statistics(torch.as_tensor(data, device="cuda"))
+    ref = ostats.statistics(data)
+    for k, v in (("mean", st.mean), ("variance", st.variance), ("error_of_mean", st.error_of_mean),
+                 ("tau_corr", st.tau_corr), ("R_hat", st.R_hat)):
+        if np.isnan(ref[k]):
+            assert np.isnan(v), k
+        else:
+            assert v == pytest.approx(ref[k], rel=1e-10, abs=1e-12), k
+
+
+def _state(kind, dtype, n_chains, n_samples, seed=0, scale=0.05):
+    if kind == "ising":
+        N = 12
+        hi = nk.hilbert.Spin(0.5, N=N)
+        H = nk.operator.Ising(hi, nk.graph.Chain(N), h=1.0)
+        sa = nk.sampler.MetropolisLocal(hi, n_chains=n_chains)
+        conn = lambda x: O.ising_conn(x, H.edges, 1.0, 1.0)
+    else:
+        N = 16
+        hi = nk.hilbert.Spin(0.5, N=N, total_sz=0)
+        H = nk.operator.Heisenberg(hi, nk.graph.Chain(N))
+        sa = nk.sampler.MetropolisExchange(hi, graph=nk.graph.Chain(N), n_chains=n_chains)
+        terms = dict(zip([tuple(a) for a in H.acting_on], H.operators))
+        conn = lambda x: O.local_conn(x, terms)
+    model = nk.models.RBM(alpha=2, param_dtype=dtype)
+    vs = nk.vqs.MCState(sa, model, n_samples=n_samples, n_discard_per_chain=4, seed=seed)
+    # larger random parameters than the default init so the gradient is not tiny
+    rng = np.random.default_rng(seed + 1)
+    flat = rng.normal(scale=scale, size=vs.n_parameters).astype(dtype)
+    vs.parameters = torch.as_tensor(flat, device="cuda")
+    p = orbm.RBMParams(flat.astype(np.float64), N, model.n_hidden(N))
+    return vs, H, p, conn
+
+
+@pytest.mark.parametrize("kind,dtype", [("ising", np.float64), ("ising", np.float32),
+                                        ("heis", np.float64), ("heis", np.float32)])
+def test_expect_and_grad(gpu, kind, dtype):
+    print(f"case expect_and_grad {kind} {np.dtype(dtype)}", flush=True)
+    vs, H, p, conn = _state(kind, dtype, 32, 2048, scale=0.1)
+    st, grad = vs.expect_and_grad(H)
+    samples = vs.samples.cpu().numpy()
+    ref_st, ref_g, _ = orbm.expect_and_grad(p, samples, conn)
+    tol = 1e-10 if dtype == np.float64 else 1e-5
+    assert st.mean == pytest.approx(ref_st["mean"], rel=tol)
+    assert st.variance == pytest.approx(ref_st["variance"], rel=10 * tol)
+    g = vs.flatten_tree(grad).cpu().numpy()
+    np.testing.assert_allclose(g, ref_g, rtol=tol, atol=tol * np.abs(ref_g).max())
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_qgt_dense(gpu, dtype):
+    print(f"case qgt {np.dtype(dtype)}", flush=True)
+    vs, H, p, conn = _state("ising", dtype, 16, 1008, scale=0.2)
+    qgt = nk.optimizer.QGTJacobianDense(vs, diag_shift=0.01)
+    S = qgt.to_dense().cpu().numpy()
+    x = vs.samples.cpu().numpy().reshape(-1, vs.hilbert.size)
+    OL = ongd.centred_jacobian(orbm.jacobian(p, x))
+    S_ref = ongd.qgt_dense(OL, 0.01)
+    np.testing.assert_array_equal(S, S.T)
+    if dtype == np.float64:
+        np.testing.assert_allclose(S, S_ref, rtol=1e-12, atol=1e-14)
+    else:
+        # bf16x3: each product keeps ~16 mantissa bits of X (hi + lo), so
+        # |dS_pq| <= c 2^-16 sum_s |X_ps X_qs| <= c 2^-16 sqrt(S_pp S_qq)
+        d = np.sqrt(np.diag(S_ref - 0.01 * np.eye(len(S_ref))))
+        bound = 3e-5 * np.outer(d, d) + 1e-9
+        err = np.abs(S - S_ref)
+        print(f"max |dS|/bound = {(err / bound).max():.3f}", flush=True)
+        assert np.all(err <= bound)
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 200, 333])
+def test_cholesky_solve(gpu, n):
+    print(f"case cholesky n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    A = rng.normal(size=(n, n))
+    A = A @ A.T / n + 0.05 * np.eye(n)
+    b = rng.normal(size=n)
+    x, info = gsolver.cholesky(torch.as_tensor(A, device="cuda"), torch.as_tensor(b, device="cuda"))
+    ref = sla.cho_solve(sla.cho_factor(A), b)
+    np.testing.assert_allclose(x.cpu().numpy(), ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max())
+
+
+def test_cholesky_failure_and_fallback(gpu):
+    # netket/_src/solvers/nan_fallback.py: NaN output triggers pinv_smooth
+    n = 96
+    rng = np.random.default_rng(3)
+    U = rng.normal(size=(n, 40))
+    A = U @ U.T  # rank 40: not positive definite
+    A[70, 70] = -1.0
+    b = rng.normal(size=n)
+    x, _ = gsolver.cholesky(torch.as_tensor(A, device="cuda"), torch.as_tensor(b, device="cuda"))
+    assert np.all(np.isnan(x.cpu().numpy()))
+    x2, info = gsolver.cholesky_with_fallback(torch.as_tensor(A, device="cuda"),
+                                              torch.as_tensor(b, device="cuda"))
+    assert info["solver_fallback"] is True
+    np.testing.assert_allclose(x2.cpu().numpy(), ongd.pinv_smooth(A, b), rtol=1e-7, atol=1e-8)
+    # a good system does not fall back
+    B = U @ U.T + np.eye(n) * 5.0
+    x3, info3 = gsolver.cholesky_with_fallback(torch.as_tensor(B, device="cuda"),
+                                               torch.as_tensor(b, device="cuda"))
+    assert info3["solver_fallback"] is False
+
+
+def test_cg(gpu):
+    n = 150
+    rng = np.random.default_rng(11)
+    A = rng.normal(size=(n, n))
+    A = A @ A.T / n + 0.5 * np.eye(n)
+    b = rng.normal(size=n)
+    x, info = gsolver.cg(torch.as_tensor(A, device="cuda"), torch.as_tensor(b, device="cuda"))
+    ref, k = ongd.cg(lambda v: A @ v, b)
+    assert info["iterations"] == k
+    np.testing.assert_allclose(x.cpu().numpy(), ref, rtol=1e-8, atol=1e-10)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_vmc_sr_update_matches_oracle_and_minsr(gpu, dtype):
+    # VMC_SR with use_ntk=False and use_ntk=True give the same update
+    # (test/driver/test_srt.py:98-211), and both match the oracle's SR update
+    print(f"case sr/minsr {np.dtype(dtype)}", flush=True)
+    vs, H, p, conn = _state("ising", dtype, 16, 256, scale=0.1)
+    samples = vs.samples.cpu().numpy()
+    x = samples.reshape(-1, vs.hilbert.size)
+    e = orbm.local_energy(p, x, conn)
+    OL, dv = ongd.prepare(orbm.jacobian(p, x), e)
+    ref_sr, _, _ = ongd.sr_update(OL, dv, 1e-3)
+    ref_srt, _, _ = ongd.srt_update(OL, dv, 1e-3)
+    np.testing.assert_allclose(ref_sr, ref_srt, rtol=1e-8, atol=1e-10)
+    out = {}
+    for ntk in (False, True):
+        drv = nk.VMC_SR(H, nk.optimizer.Sgd(0.01), diag_shift=1e-3, variational_state=vs,
+                        use_ntk=ntk, linear_solver=gsolver.cholesky)
+        st, dp = drv.compute_loss_and_update()
+        out[ntk] = dp.cpu().numpy()
+        assert vs.samples.cpu().numpy() is not None
+    scale = np.abs(ref_sr).max()
+    tol = 1e-8 if dtype == np.float64 else 2e-3
+    print(f"max |sr - ref| / max|ref| = {np.abs(out[False] - ref_sr).max() / scale:.2e}, "
+          f"ntk: {np.abs(out[True] - ref_sr).max() / scale:.2e}", flush=True)
+    np.testing.assert_allclose(out[False], ref_sr, rtol=0, atol=tol * scale)
+    np.testing.assert_allclose(out[True], ref_sr, rtol=0, atol=tol * scale)
+
+
+def test_vmc_sr_preconditioner_step(gpu):
+    # VMC + SR(QGTJacobianDense, cholesky): dp = (S + eps I)^-1 grad on the same samples
+    vs, H, p, conn = _state("heis", np.float64, 16, 512, scale=0.1)
+    sr = nk.optimizer.SR(nk.optimizer.QGTJacobianDense, solver=gsolver.cholesky, diag_shift=0.01)
+    drv = nk.VMC(H, nk.optimizer.Sgd(0.01), variational_state=vs, preconditioner=sr)
+    st, dp = drv.compute_loss_and_update()
+    x = vs.samples.cpu().numpy().reshape(-1, vs.hilbert.size)
+    e = orbm.local_energy(p, x, conn)
+    OL, dv = ongd.prepare(orbm.jacobian(p, x), e)
+    ref, _, _ = ongd.sr_update(OL, dv, 0.01)
+    np.testing.assert_allclose(dp.cpu().numpy(), ref, rtol=1e-8, atol=1e-10 * np.abs(ref).max())
+    # the default SR solver (CG) converges to the same update within its tolerance
+    sr_cg = nk.optimizer.SR(nk.optimizer.QGTJacobianDense, diag_shift=0.01)
+    g = vs.expect_and_forces_flat(H)[1]  # same cached samples
+    dp2 = sr_cg(vs, g)
+    assert np.linalg.norm(dp2.cpu().numpy() - ref) <= 1e-3 * np.linalg.norm(ref)
