@@ -1,0 +1,245 @@
+#!/usr/bin/env python
+"""Benchmark of the VMC step on one MI355X.
+
+Workload (timed): one VMC_SR step of the 1-D antiferromagnetic Heisenberg
+chain, L=64, PBC, J=1, total Sz=0, Marshall sign rule; FP32 RBM alpha=2
+(8384 parameters); MetropolisExchange on nearest-neighbour bonds (64
+clusters), 4096 chains, 16384 samples per step, 5 discarded sweeps (NetKet's
+default), sweep size 64; SR with the dense S (8384 x 8384) from the bf16x3
+MFMA Gram and an FP64 Cholesky solve; diag_shift 0.01, SGD lr 0.01.
+Synthetic data: parameters drawn N(0, 0.01^2) from seed 0, chains from the
+sampler.  A step is sample -> E_loc -> statistics -> Jacobian -> S -> solve
+-> update, all on the GPU.
+
+Reported beside it: per-stage GPU times from HIP events on the launch stream,
+the roofline of the dominant kernel, the CPU restatement of the reference
+algorithm (FP64 NumPy/LAPACK) timed on one step of the same workload, and an
+accuracy run (TFIM L=16, outside the timed region) against the exact energy.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def build_cfg2(seed=0):
+    import numpy as np
+
+    import netket_amd as nk
+    from netket_amd.optimizer import solver
+
+    L = 64
+    g = nk.graph.Chain(L)
+    hi = nk.hilbert.Spin(0.5, N=L, total_sz=0)
+    H = nk.operator.Heisenberg(hi, g)
+    sa = nk.sampler.MetropolisExchange(hi, graph=g, n_chains=4096)
+    model = nk.models.RBM(alpha=2, param_dtype=np.float32)
+    vs = nk.vqs.MCState(sa, model, n_samples=16384, n_discard_per_chain=5, seed=seed)
+    drv = nk.VMC_SR(H, nk.optimizer.Sgd(0.01), diag_shift=0.01, variational_state=vs,
+                    use_ntk=False, linear_solver=solver.cholesky)
+    return drv, H, vs
+
+
+def step(drv):
+    drv.reset_step()
+    st, dp = drv.compute_loss_and_update()
+    drv.update_parameters(dp)
+    drv._step_count += 1
+    return st
+
+
+def accuracy_run(n_steps=300):
+    """Config 1 (TFIM L=16, FP64 RBM alpha=1, 16 chains, 1008 samples, 16 discarded
+    sweeps, SR diag_shift 0.01 with Cholesky) against the exact energy."""
+    import numpy as np
+    import torch
+
+    import netket_amd as nk
+    from netket_amd.optimizer import solver
+
+    L = 16
+    e0 = -float(np.sum(np.sqrt(2.0 - 2.0 * np.cos(np.pi * (2 * np.arange(L) + 1) / L))))
+    g = nk.graph.Chain(L)
+    hi = nk.hilbert.Spin(0.5, N=L)
+    H = nk.operator.Ising(hi, g, h=1.0, J=1.0)
+    sa = nk.sampler.MetropolisLocal(hi, n_chains=16)
+    vs = nk.vqs.MCState(sa, nk.models.RBM(alpha=1, param_dtype=np.float64), n_samples=1008,
+                        n_discard_per_chain=16, seed=0)
+    sr = nk.optimizer.SR(nk.optimizer.QGTJacobianDense, solver=solver.cholesky, diag_shift=0.01)
+    lr = 0.05
+    drv = nk.VMC(H, nk.optimizer.Sgd(lr), variational_state=vs, preconditioner=sr)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = []
+    for _ in range(n_steps):
+        stats.append(step(drv))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    e = np.array([s.mean for s in stats])
+    tail = float(np.mean(e[-50:]))
+    return {
+        "workload": "tfim-1d-L16-h1 fp64-rbm-a1 metropolis-local 16ch 1008smp sr-cholesky "
+                    f"eps=0.01 sgd-lr={lr} {n_steps} steps",
+        "exact_e0": e0,
+        "energy_last50": tail,
+        "rel_err": abs(tail - e0) / abs(e0),
+        "steps_per_s": n_steps / dt,
+    }
+
+
+def cpu_baseline_cfg2(budget_s: float = 25.0):
+    """One CPU step of the reference algorithm (FP64) on the same workload."""
+    import numpy as np
+
+    import netket_amd as nk
+    from oracle import cpu_baseline as cb
+
+    L, M = 64, 128
+    g = nk.graph.Chain(L)
+    hi = nk.hilbert.Spin(0.5, N=L, total_sz=0)
+    H = nk.operator.Heisenberg(hi, g)
+    terms = dict(zip([tuple(a) for a in H.acting_on], H.operators))
+    clusters = nk.sampler.compute_clusters(g, 1).astype(np.int64)
+    rng = np.random.default_rng(0)
+    flat = rng.normal(scale=0.01, size=M + L * M + L)
+    C = 4096
+    sigma = np.ones((C, L), dtype=np.int8)
+    for c in range(C):
+        sigma[c, rng.permutation(L)[: L // 2]] = -1
+    timings = {}
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        ts = time.perf_counter()
+        flat, sigma, e = cb.vmc_sr_step(flat, L, M, sigma, terms, clusters, n_discard=5,
+                                        chain_length=4, sweep_size=L, diag_shift=0.01, lr=0.01,
+                                        rng=rng, timings=timings)
+        n += 1
+        last = time.perf_counter() - ts
+        if time.perf_counter() - t0 + last > budget_s or n >= 3:
+            break
+    dt = time.perf_counter() - t0
+    return n / dt, n, timings
+
+
+def roofline_for(stage: str, mean_ms: float):
+    Ns, Np, n_chains = 16384, 8384, 4096
+    secs = mean_ms * 1e-3
+    if stage == "gram_bf16x3":
+        flops = 2.0 * Ns * Np * Np
+        ach = flops / secs / 1e12
+        return {"bound": "mfma", "achieved": ach, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": ach / BF16_PEAK_TFLOPS, "traffic": None,
+                "kernel": "gram_bf16x3_kernel",
+                "per_launch": f"2*Ns*Np^2 = {flops:.3e} logical flop (x3 MFMA work for the bf16 split)"}
+    if stage == "potrf":
+        flops = Np**3 / 3.0
+        ach = flops / secs / 1e12
+        return {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": ach / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "nka_potrf_f64 (all launches)",
+                "per_launch": f"n^3/3 = {flops:.3e} flop, n = {Np}"}
+    if stage == "metropolis":
+        props = n_chains * (5 + 4) * 64
+        byts = props * 2 * 4 * 128 + Ns * 64
+        ach = byts / secs / 1e9
+        return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "rbm_metropolis_kernel",
+                "per_launch": f"{props} proposals x 2 W rows x 512 B + samples"}
+    if stage == "local_energy":
+        byts = Ns * 64 * 4 * 128 * 2 + Ns * (64 + 8)
+        ach = byts / secs / 1e9
+        return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "rbm_eloc_kernel",
+                "per_launch": "64 bonds x 2 W rows x 512 B per sample"}
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=25.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the VMC step here runs on a single MI355X)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from netket_amd import _ops
+
+    drv, H, vs = build_cfg2()
+    for _ in range(args.warmup):
+        step(drv)
+    torch.cuda.synchronize()
+    _ops.TIMER.reset()
+    _ops.TIMER.enabled = True
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = [step(drv) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    _ops.TIMER.enabled = False
+    stages = _ops.TIMER.summary()
+    dt = t1 - t0
+    sps = args.steps / dt
+    energies = [s.mean for s in stats]
+    gpu_ms_total = sum(v["total_ms"] for v in stages.values()) / args.steps
+    dom = max(stages.items(), key=lambda kv: kv[1]["total_ms"])
+    roof = roofline_for(dom[0], dom[1]["mean_ms"])
+    acc = None if args.no_accuracy else accuracy_run()
+    cpu = None
+    if not args.no_cpu_baseline:
+        cores = len(os.sched_getaffinity(0))
+        rate, n_cpu, tim = cpu_baseline_cfg2(args.cpu_budget)
+        threads = int(os.environ.get("OMP_NUM_THREADS", cores))
+        cpu = {"value": rate, "unit": "SR steps/s", "cores": min(cores, threads), "kind": "port",
+               "sample": f"{n_cpu} full step(s) of the same workload (L=64 Heisenberg, 16384 samples, "
+                         f"S 8384^2) in FP64 NumPy/LAPACK with the reference algorithm; "
+                         f"stage seconds {json.dumps({k: round(v, 2) for k, v in tim.items()})}"}
+    out = {
+        "metric": "SR steps/s & samples/s at <=1e-4 rel. energy err vs ED; p50/p99 job latency",
+        "value": sps,
+        "unit": "SR steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "samples_per_s": sps * 16384,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (bf16x3 MFMA Gram, FP64 Cholesky/means)",
+        "data": "synthetic (random-init RBM, sampler-generated configurations)",
+        "config": {"workload": "heisenberg-1d-L64-sz0 fp32-rbm-a2 metropolis-exchange 4096ch "
+                               "16384smp sr-dense-S8384 bf16x3 fp64-cholesky",
+                   "n_samples": 16384, "n_params": 8384, "n_chains": 4096, "n_discard": 5,
+                   "diag_shift": 0.01, "lr": 0.01, "parallelism": "single-gpu"},
+        "stages_ms_per_step": {k: round(v["total_ms"] / args.steps, 4) for k, v in sorted(
+            stages.items(), key=lambda kv: -kv[1]["total_ms"])},
+        "gpu_busy_ms_per_step": gpu_ms_total,
+        "dominant_stage": dom[0],
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "accuracy": acc,
+        "energy_first_last": [energies[0], energies[-1]],
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

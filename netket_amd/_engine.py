@@ -21,8 +21,9 @@ def local_energies(model, flat, op, sigma2d: torch.Tensor, log_psi: bool = False
     lp = torch.empty(n, dtype=flat.dtype, device=sigma2d.device) if log_psi else None
     kind, op_ptr, keep = _ops.op_struct(op)
     m = model.native(flat, N)
-    nat.call("nka_rbm_local_energy", C.byref(m), kind, op_ptr, _ops.ptr(sigma2d), n,
-             _ops.ptr(eloc), _ops.ptr(lp), _ops.stream())
+    with _ops.TIMER.region("local_energy"):
+        nat.call("nka_rbm_local_energy", C.byref(m), kind, op_ptr, _ops.ptr(sigma2d), n,
+                 _ops.ptr(eloc), _ops.ptr(lp), _ops.stream())
     del keep
     return (eloc, lp) if log_psi else eloc
 
@@ -34,8 +35,9 @@ def prepare(model, flat, sigma2d: torch.Tensor, row_major: bool = True, transpos
     t = torch.empty((n, M), dtype=flat.dtype, device=sigma2d.device) if row_major else None
     tT = torch.empty((M, n), dtype=flat.dtype, device=sigma2d.device) if transposed else None
     m = model.native(flat, N)
-    nat.call("nka_rbm_prepare", C.byref(m), _ops.ptr(sigma2d), n, _ops.ptr(t), _ops.ptr(tT),
-             _ops.stream())
+    with _ops.TIMER.region("prepare"):
+        nat.call("nka_rbm_prepare", C.byref(m), _ops.ptr(sigma2d), n, _ops.ptr(t), _ops.ptr(tT),
+                 _ops.stream())
     return t, tT
 
 
@@ -50,8 +52,9 @@ def contract(model, flat, sigma2d: torch.Tensor, t: torch.Tensor, w: torch.Tenso
     out = torch.empty((n_w, np_), dtype=torch.float64, device=sigma2d.device)
     ws_bytes = nat.lib().nka_rbm_contract_workspace(C.byref(m), n, n_w)
     ws = _ops.WORKSPACE.get("contract", ws_bytes)
-    nat.call("nka_rbm_contract", C.byref(m), _ops.ptr(sigma2d), _ops.ptr(t), n, _ops.ptr(w), n_w,
-             _ops.ptr(mean), _ops.ptr(out), _ops.ptr(ws), _ops.stream())
+    with _ops.TIMER.region("contract"):
+        nat.call("nka_rbm_contract", C.byref(m), _ops.ptr(sigma2d), _ops.ptr(t), n, _ops.ptr(w),
+                 n_w, _ops.ptr(mean), _ops.ptr(out), _ops.ptr(ws), _ops.stream())
     return out
 
 
@@ -89,9 +92,10 @@ def jacobian(model, flat, sigma2d, t, tT, mean, scale: float, layout: int, kind:
     if bufs is None or not bufs.matches(rows, cols, kind):
         bufs = JacobianBuffers(rows, cols, kind, sigma2d.device)
     m = model.native(flat, N)
-    nat.call("nka_rbm_jacobian", C.byref(m), _ops.ptr(sigma2d), _ops.ptr(t), _ops.ptr(tT), n,
-             _ops.ptr(mean), float(scale), layout, kind, _ops.ptr(bufs.hi), _ops.ptr(bufs.lo),
-             bufs.ld, _ops.stream())
+    with _ops.TIMER.region("jacobian"):
+        nat.call("nka_rbm_jacobian", C.byref(m), _ops.ptr(sigma2d), _ops.ptr(t), _ops.ptr(tT), n,
+                 _ops.ptr(mean), float(scale), layout, kind, _ops.ptr(bufs.hi),
+                 _ops.ptr(bufs.lo), bufs.ld, _ops.stream())
     return bufs
 
 
@@ -102,13 +106,15 @@ def gram(bufs: JacobianBuffers, shift: float, offset: float = 0.0,
     if out is None or out.shape != (n, n):
         out = torch.empty((n, n), dtype=torch.float64, device=bufs.hi.device)
     if bufs.kind == nat.OUT_BF16X2:
-        nat.call("nka_gram_bf16x3", _ops.ptr(bufs.hi), _ops.ptr(bufs.lo), n, bufs.ld, bufs.ld,
-                 float(shift), float(offset), _ops.ptr(out), out.stride(0), _ops.stream())
+        with _ops.TIMER.region("gram_bf16x3"):
+            nat.call("nka_gram_bf16x3", _ops.ptr(bufs.hi), _ops.ptr(bufs.lo), n, bufs.ld, bufs.ld,
+                     float(shift), float(offset), _ops.ptr(out), out.stride(0), _ops.stream())
     elif bufs.kind == nat.OUT_F64:
-        nat.call("nka_syrk_f64", _ops.ptr(bufs.hi), n, bufs.ld, bufs.ld, 1.0, 0.0, float(shift),
-                 _ops.ptr(out), out.stride(0), _ops.stream())
         if offset != 0.0:
             raise NotImplementedError("offset with FP64 Jacobians")
+        with _ops.TIMER.region("gram_f64"):
+            nat.call("nka_syrk_f64", _ops.ptr(bufs.hi), n, bufs.ld, bufs.ld, 1.0, 0.0,
+                     float(shift), _ops.ptr(out), out.stride(0), _ops.stream())
     else:
         raise NotImplementedError("Gram of an FP32 Jacobian")
     return out
@@ -116,14 +122,16 @@ def gram(bufs: JacobianBuffers, shift: float, offset: float = 0.0,
 
 def centre_scale(e: torch.Tensor, mean_dev: torch.Tensor, scale: float) -> torch.Tensor:
     out = torch.empty_like(e)
-    nat.call("nka_centre_scale", _ops.ptr(e), e.numel(), _ops.ptr(mean_dev), float(scale),
-             _ops.ptr(out), _ops.stream())
+    with _ops.TIMER.region("centre"):
+        nat.call("nka_centre_scale", _ops.ptr(e), e.numel(), _ops.ptr(mean_dev), float(scale),
+                 _ops.ptr(out), _ops.stream())
     return out
 
 
 def sgd_update_(flat: torch.Tensor, dp: torch.Tensor, lr: float):
-    nat.call("nka_sgd_update", _ops.ptr(flat), _ops.dtype_code(flat.dtype), _ops.ptr(dp),
-             flat.numel(), float(lr), _ops.stream())
+    with _ops.TIMER.region("sgd"):
+        nat.call("nka_sgd_update", _ops.ptr(flat), _ops.dtype_code(flat.dtype), _ops.ptr(dp),
+                 flat.numel(), float(lr), _ops.stream())
 
 
 def inv_sqrt(n: int) -> float:

@@ -8,6 +8,7 @@ return torch tensors resident on the GPU.
 from __future__ import annotations
 
 import ctypes as C
+from contextlib import contextmanager
 
 import numpy as np
 import torch
@@ -17,6 +18,46 @@ from . import _native as nat
 
 class DeviceUnavailable(RuntimeError):
     pass
+
+
+class KernelTimer:
+    """Per-stage GPU time from HIP events recorded on the launch stream.
+
+    Disabled by default; when enabled, every wrapped region records a start
+    and an end event around its kernel launches (no synchronisation).
+    """
+
+    def __init__(self):
+        self.enabled = False
+        self._events = {}
+
+    @contextmanager
+    def region(self, name: str):
+        if not self.enabled:
+            yield
+            return
+        s = torch.cuda.Event(enable_timing=True)
+        e = torch.cuda.Event(enable_timing=True)
+        s.record()
+        try:
+            yield
+        finally:
+            e.record()
+            self._events.setdefault(name, []).append((s, e))
+
+    def reset(self):
+        self._events = {}
+
+    def summary(self) -> dict:
+        torch.cuda.synchronize()
+        out = {}
+        for name, evs in self._events.items():
+            ts = [s.elapsed_time(e) for s, e in evs]
+            out[name] = {"count": len(ts), "mean_ms": float(np.mean(ts)), "total_ms": float(np.sum(ts))}
+        return out
+
+
+TIMER = KernelTimer()
 
 
 def device() -> torch.device:
@@ -223,13 +264,15 @@ def potrf_(a: torch.Tensor, info: torch.Tensor, ws: torch.Tensor | None = None):
     n = a.shape[0]
     if ws is None:
         ws = WORKSPACE.get("potrf", nat.lib().nka_potrf_workspace(n))
-    nat.call("nka_potrf_f64", ptr(a), n, a.stride(0), ptr(ws), ptr(info), stream())
+    with TIMER.region("potrf"):
+        nat.call("nka_potrf_f64", ptr(a), n, a.stride(0), ptr(ws), ptr(info), stream())
     return ws
 
 
 def potrs_(a: torch.Tensor, ws: torch.Tensor, b: torch.Tensor, info: torch.Tensor):
     n = a.shape[0]
-    nat.call("nka_potrs_f64", ptr(a), n, a.stride(0), ptr(ws), ptr(b), ptr(info), stream())
+    with TIMER.region("potrs"):
+        nat.call("nka_potrs_f64", ptr(a), n, a.stride(0), ptr(ws), ptr(b), ptr(info), stream())
 
 
 def symmetrize_(a: torch.Tensor):

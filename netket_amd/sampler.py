@@ -202,11 +202,12 @@ class MetropolisSampler:
             cl_ptr, n_cl = _ops.ptr(cl), int(cl.shape[0])
         else:
             cl_ptr, n_cl = C.c_void_p(0), 0
-        nat.call("nka_rbm_metropolis", C.byref(m), self.rule.kind, cl_ptr, n_cl,
-                 _ops.ptr(state.sigma), _ops.ptr(state.log_prob), self.n_chains, self.sweep_size,
-                 int(n_discard), int(chain_length), float(self.machine_pow),
-                 state.seed & (2**64 - 1), state.counter, _ops.ptr(samples), _ops.ptr(lps),
-                 _ops.ptr(state.n_accepted_proc), _ops.stream())
+        with _ops.TIMER.region("metropolis"):
+            nat.call("nka_rbm_metropolis", C.byref(m), self.rule.kind, cl_ptr, n_cl,
+                     _ops.ptr(state.sigma), _ops.ptr(state.log_prob), self.n_chains,
+                     self.sweep_size, int(n_discard), int(chain_length), float(self.machine_pow),
+                     state.seed & (2**64 - 1), state.counter, _ops.ptr(samples), _ops.ptr(lps),
+                     _ops.ptr(state.n_accepted_proc), _ops.stream())
         n_prop = (int(n_discard) + int(chain_length)) * self.sweep_size
         state.counter += n_prop
         state.n_steps_proc += n_prop * self.n_chains

@@ -122,5 +122,6 @@ def statistics(data) -> Stats:
     d = data.to(device=_ops.device(), dtype=torch.float64).contiguous()
     C, L = d.shape
     out = torch.empty(8, dtype=torch.float64, device=d.device)
-    nat.call("nka_stats_moments", _ops.ptr(d), C, L, _ops.ptr(out), _ops.stream())
+    with _ops.TIMER.region("stats"):
+        nat.call("nka_stats_moments", _ops.ptr(d), C, L, _ops.ptr(out), _ops.stream())
     return Stats(_moments=out, _shape=(C, L))
