@@ -64,9 +64,13 @@ __host__ __device__ inline uint32_t bounded(uint32_t x, uint32_t n) {
 }
 
 template <class T> __device__ inline T log_cosh(T x);
+// FP32: hardware exp2/log2 (v_exp_f32, v_log_f32).  With e = exp(-2|x|) in
+// (0, 1], log(1 + e) is evaluated directly; the rounding of 1 + e costs at
+// most 2^-24 absolute, below the FP32 resolution of the result.
 template <> __device__ inline float log_cosh<float>(float x) {
   const float ax = fabsf(x);
-  return ax + log1pf(expf(-2.0f * ax)) - 0.69314718055994531f;
+  const float e = __builtin_amdgcn_exp2f(-2.8853900817779268f * ax);  // exp(-2|x|)
+  return ax + 0.69314718055994531f * (__builtin_amdgcn_logf(1.0f + e) - 1.0f);
 }
 template <> __device__ inline double log_cosh<double>(double x) {
   const double ax = fabs(x);
@@ -74,7 +78,7 @@ template <> __device__ inline double log_cosh<double>(double x) {
 }
 
 template <class T> __device__ inline T exp_t(T x);
-template <> __device__ inline float exp_t<float>(float x) { return expf(x); }
+template <> __device__ inline float exp_t<float>(float x) { return __expf(x); }
 template <> __device__ inline double exp_t<double>(double x) { return exp(x); }
 
 template <class T> __device__ inline T log_t(T x);
@@ -85,14 +89,48 @@ template <class T> __device__ inline T tanh_t(T x);
 template <> __device__ inline float tanh_t<float>(float x) { return tanhf(x); }
 template <> __device__ inline double tanh_t<double>(double x) { return tanh(x); }
 
-// Butterfly sum over the 64 lanes of a wavefront; every lane gets the total.
+// Sum over the 64 lanes of a wavefront; every lane gets the total.
+// Generic: butterfly over cross-lane shuffles.
 template <class T> __device__ inline T wave_sum(T v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, NKA_WAVE);
   return v;
 }
 
+// FP32: DPP row operations inside 16-lane rows, then row broadcasts, so the
+// partial sums never leave the VALU; the total lands in lane 63 and is read
+// back as a scalar.  Lanes masked off by a DPP step receive 0.
+template <int CTRL, int ROW_MASK, int BANK_MASK>
+__device__ inline float dpp_f32(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, BANK_MASK, false));
+}
+template <> __device__ inline float wave_sum<float>(float v) {
+  v += dpp_f32<0xb1, 0xf, 0xf>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f32<0x4e, 0xf, 0xf>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f32<0x114, 0xf, 0xe>(v);  // row_shr:4
+  v += dpp_f32<0x118, 0xf, 0xc>(v);  // row_shr:8
+  v += dpp_f32<0x142, 0xa, 0xf>(v);  // row_bcast:15
+  v += dpp_f32<0x143, 0xc, 0xf>(v);  // row_bcast:31
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 __device__ inline int lane_id() { return (int)(threadIdx.x & (NKA_WAVE - 1)); }
+
+// Position of the k-th (0-based) set bit of m (m has more than k set bits).
+__device__ inline int kth_set_bit(uint64_t m, int k) {
+  int pos = 0;
+#pragma unroll
+  for (int w = 32; w >= 1; w >>= 1) {
+    const int c = __popcll(m & ((1ull << w) - 1ull));
+    if (k >= c) {
+      k -= c;
+      m >>= w;
+      pos += w;
+    }
+  }
+  return pos;
+}
 
 // Order this lane's LDS writes before later LDS reads of other lanes in the
 // same wavefront.

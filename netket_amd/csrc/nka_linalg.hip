@@ -75,31 +75,47 @@ __global__ void __launch_bounds__(256) gram_bf16x3_kernel(const uint16_t* __rest
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[a][b][e] = 0.0f;
 
-  // global->LDS copy: each operand tile is 128 rows x 32 k = 512 chunks of 8 bf16
+  // Each operand slab is 128 rows x 32 k = 512 chunks of 8 bf16; thread tid
+  // moves chunks tid and tid + 256 of each of the (up to) four planes.  The
+  // next slab is fetched into registers while the current one is used.
+  uint4 pf[2][4];
+  auto fetch = [&](int64_t k0) {
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+      const int chunk = tid + it * 256;
+      const int r = chunk >> 2, kc = (chunk & 3) * 8;
+      const int64_t kk = k0 + kc;
+      uint4 z = {0, 0, 0, 0};
+      pf[it][0] = z;
+      pf[it][1] = z;
+      pf[it][2] = z;
+      pf[it][3] = z;
+      if (i0 + r < n && kk < K) {
+        pf[it][0] = *(const uint4*)(xh + (i0 + r) * ldx + kk);
+        pf[it][1] = *(const uint4*)(xl + (i0 + r) * ldx + kk);
+      }
+      if (!diag_tile && j0 + r < n && kk < K) {
+        pf[it][2] = *(const uint4*)(xh + (j0 + r) * ldx + kk);
+        pf[it][3] = *(const uint4*)(xl + (j0 + r) * ldx + kk);
+      }
+    }
+  };
+  fetch(0);
   for (int64_t k0 = 0; k0 < K; k0 += GB_K) {
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < 2; ++it) {
-      const int chunk = tid + it * 256;  // 0..511
+      const int chunk = tid + it * 256;
       const int r = chunk >> 2, kc = (chunk & 3) * 8;
-      const int64_t kk = k0 + kc;
-      uint4 ah = {0, 0, 0, 0}, al = {0, 0, 0, 0}, bh = {0, 0, 0, 0}, bl = {0, 0, 0, 0};
-      if (i0 + r < n && kk < K) {
-        ah = *(const uint4*)(xh + (i0 + r) * ldx + kk);
-        al = *(const uint4*)(xl + (i0 + r) * ldx + kk);
-      }
-      if (!diag_tile && j0 + r < n && kk < K) {
-        bh = *(const uint4*)(xh + (j0 + r) * ldx + kk);
-        bl = *(const uint4*)(xl + (j0 + r) * ldx + kk);
-      }
-      *(uint4*)(&sA[0][r * GB_LD + kc]) = ah;
-      *(uint4*)(&sA[1][r * GB_LD + kc]) = al;
+      *(uint4*)(&sA[0][r * GB_LD + kc]) = pf[it][0];
+      *(uint4*)(&sA[1][r * GB_LD + kc]) = pf[it][1];
       if (!diag_tile) {
-        *(uint4*)(&sB[0][r * GB_LD + kc]) = bh;
-        *(uint4*)(&sB[1][r * GB_LD + kc]) = bl;
+        *(uint4*)(&sB[0][r * GB_LD + kc]) = pf[it][2];
+        *(uint4*)(&sB[1][r * GB_LD + kc]) = pf[it][3];
       }
     }
     __syncthreads();
+    if (k0 + GB_K < K) fetch(k0 + GB_K);
     const uint16_t* Bh = diag_tile ? sA[0] : sB[0];
     const uint16_t* Bl = diag_tile ? sA[1] : sB[1];
 #pragma unroll
@@ -118,14 +134,23 @@ __global__ void __launch_bounds__(256) gram_bf16x3_kernel(const uint16_t* __rest
         bh[b] = *(const bf16x8*)(&Bh[row * GB_LD + kq]);
         bl[b] = *(const bf16x8*)(&Bl[row * GB_LD + kq]);
       }
+      // small products first, four independent accumulators between
+      // consecutive updates of the same accumulator
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
+        for (int b = 0; b < 2; ++b)
           acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
           acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
           acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
-        }
     }
   }
   // store: element e of lane l in a 32x32 tile is (row (e/4)*8 + (l/32)*4 + e%4, col l%32)
@@ -233,135 +258,180 @@ __global__ void __launch_bounds__(256) gemm_nt_f64_kernel(const double* a, int64
 // ---------------------------------------------------------------------------
 constexpr int CH_NB = 64;
 
-// Factor the diagonal block A[k0:k0+nb, k0:k0+nb] in LDS, write L11 back and
-// L11^{-1} to inv (nb x nb).  A non-positive or non-finite pivot sets *info.
-__global__ void __launch_bounds__(256) potf2_inv_kernel(double* __restrict__ A, int64_t lda, int64_t n,
-                                                        int64_t k0, double* __restrict__ inv,
-                                                        int32_t* __restrict__ info) {
+// Factor the diagonal block A[k0:k0+nb, k0:k0+nb] with one wavefront, write
+// L11 back and L11^{-1} to inv (64 x 64).  The block lives in LDS; lane i owns
+// row i, so the only synchronisation is within the wavefront.  Rows/columns
+// past nb are an identity padding.  A non-positive or non-finite pivot sets
+// *info (first failing column + 1) and is replaced by 1 so the kernel still
+// completes.
+__global__ void __launch_bounds__(64) potf2_inv_kernel(double* __restrict__ A, int64_t lda, int64_t n,
+                                                       int64_t k0, double* __restrict__ inv,
+                                                       int32_t* __restrict__ info) {
   if (*info != 0) return;
-  __shared__ double L[CH_NB][CH_NB + 1];
-  __shared__ double X[CH_NB][CH_NB + 1];
-  __shared__ int bad;
+  __shared__ double Ls[CH_NB][CH_NB + 1];
+  __shared__ double Xs[CH_NB][CH_NB + 1];
+  const int i = threadIdx.x;
   const int nb = (int)min<int64_t>(CH_NB, n - k0);
-  const int tid = threadIdx.x;
-  if (tid == 0) bad = 0;
-  for (int idx = tid; idx < CH_NB * CH_NB; idx += blockDim.x) {
-    const int r = idx / CH_NB, cc = idx % CH_NB;
+  for (int idx = i; idx < CH_NB * CH_NB; idx += 64) {
+    const int r = idx >> 6, c = idx & 63;
     double v = 0.0;
-    if (r < nb && cc < nb) v = (cc <= r) ? A[(k0 + r) * lda + k0 + cc] : 0.0;
-    else if (r == cc) v = 1.0;  // identity padding keeps the padded factor regular
-    L[r][cc] = v;
+    if (r < nb && c < nb) v = (c <= r) ? A[(k0 + r) * lda + k0 + c] : 0.0;
+    else if (r == c) v = 1.0;
+    Ls[r][c] = v;
   }
-  __syncthreads();
+  wave_lds_sync();
+  int bad = 0;
   for (int j = 0; j < CH_NB; ++j) {
-    if (tid == 0) {
-      const double d = L[j][j];
-      if (!(d > 0.0) || !isfinite(d)) {
-        if (!bad) bad = (int)(k0 + j + 1);
-        L[j][j] = 1.0;
-      } else {
-        L[j][j] = sqrt(d);
-      }
-    }
-    __syncthreads();
-    const double djj = L[j][j];
-    for (int i = j + 1 + tid; i < CH_NB; i += blockDim.x) L[i][j] /= djj;
-    __syncthreads();
-    const int rem = CH_NB - j - 1;
-    for (int idx = tid; idx < rem * rem; idx += blockDim.x) {
-      const int i = j + 1 + idx / rem, cc = j + 1 + idx % rem;
-      if (cc <= i) L[i][cc] -= L[i][j] * L[cc][j];
-    }
-    __syncthreads();
+    const double ajj = Ls[j][j];
+    const bool ok = (ajj > 0.0) && isfinite(ajj);
+    if (!ok && bad == 0) bad = (int)(k0 + j + 1);
+    const double d = ok ? sqrt(ajj) : 1.0;
+    wave_lds_sync();
+    if (i == j) Ls[j][j] = d;
+    if (i > j) Ls[i][j] = Ls[i][j] / d;
+    wave_lds_sync();
+    const double lij = (i > j) ? Ls[i][j] : 0.0;
+    for (int c = j + 1; c <= i; ++c) Ls[i][c] -= lij * Ls[c][j];
+    wave_lds_sync();
   }
-  // inverse of the lower-triangular L: column cc solved by forward substitution,
-  // four threads per column sharing the dot products
-  const int col = tid >> 2, part = tid & 3;
-  for (int i = 0; i < CH_NB; ++i) {
-    double s = 0.0;
-    if (i >= col) {
-      for (int k = col + part; k < i; k += 4) s += L[i][k] * X[k][col];
-    }
-    s += __shfl_xor(s, 1, 64);
-    s += __shfl_xor(s, 2, 64);
-    if (part == 0) {
-      if (i < col) X[i][col] = 0.0;
-      else X[i][col] = ((i == col ? 1.0 : 0.0) - s) / L[i][i];
-    }
-    __syncthreads();
+  for (int idx = i; idx < CH_NB * CH_NB; idx += 64) {
+    const int r = idx >> 6, c = idx & 63;
+    if (r < nb && c < nb && c <= r) A[(k0 + r) * lda + k0 + c] = Ls[r][c];
   }
-  for (int idx = tid; idx < CH_NB * CH_NB; idx += blockDim.x) {
-    const int r = idx / CH_NB, cc = idx % CH_NB;
-    inv[idx] = X[r][cc];
-    if (r < nb && cc <= r) A[(k0 + r) * lda + k0 + cc] = L[r][cc];
+  // column i of X = L^{-1} by forward substitution; X[k][i] = 0 for k < i
+  for (int r = 0; r < CH_NB; ++r) {
+    double sum = 0.0;
+    for (int k = 0; k < r; ++k) sum += Ls[r][k] * Xs[k][i];
+    Xs[r][i] = (r < i) ? 0.0 : (((r == i) ? 1.0 : 0.0) - sum) / Ls[r][r];
   }
-  if (tid == 0 && bad) atomicCAS(info, 0, bad);
+  wave_lds_sync();
+  for (int idx = i; idx < CH_NB * CH_NB; idx += 64) inv[idx] = Xs[idx >> 6][idx & 63];
+  if (i == 0 && bad) atomicCAS(info, 0, bad);
+}
+
+// Zero n 32-bit words (a fill kernel rather than a memset node, so that a
+// captured graph replays it correctly).
+__global__ void zero_u32_kernel(uint32_t* __restrict__ p, int64_t n) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * blockDim.x)
+    p[k] = 0u;
 }
 
 // ---------------------------------------------------------------------------
-// Triangular solves with the blocked factor.  One launch per block column:
-// every workgroup recomputes y = inv_kk b_k (64x64) into LDS, workgroup 0
-// stores it, and each workgroup applies y to its slice of the remaining rows.
+// Triangular solves with the blocked factor, one launch each.  Workgroup k
+// owns the 64-row block k; it waits (bounded spin on a flag in global memory)
+// for every block it depends on, and only ever on blocks with a smaller
+// workgroup id, which the hardware dispatched earlier, so the wait always
+// ends.  A wait that exceeds the spin bound sets *err and the workgroup
+// proceeds (the result is then invalid and reported as such by the caller).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) trsv_fwd_kernel(const double* __restrict__ A, int64_t lda, int64_t n,
-                                                       int64_t k0, const double* __restrict__ inv,
-                                                       double* __restrict__ b, double* __restrict__ y_out,
-                                                       const int32_t* __restrict__ info) {
-  if (*info != 0) return;
-  __shared__ double y[CH_NB];
-  __shared__ double bk[CH_NB];
-  const int nb = (int)min<int64_t>(CH_NB, n - k0);
-  const int tid = threadIdx.x;
-  if (tid < CH_NB) bk[tid] = tid < nb ? b[k0 + tid] : 0.0;
-  __syncthreads();
-  if (tid < CH_NB) {
-    double s = 0.0;
-    for (int c = 0; c <= tid; ++c) s += inv[tid * CH_NB + c] * bk[c];
-    y[tid] = s;
-    if (blockIdx.x == 0 && tid < nb) y_out[k0 + tid] = s;
-  }
-  __syncthreads();
-  // rows below the block: b[r] -= sum_c A[r, k0+c] y[c]; one wave per row
-  const int wid = tid / 64, lane = lane_id();
-  const int64_t r0 = k0 + nb;
-  for (int64_t r = r0 + (int64_t)blockIdx.x * 4 + wid; r < n; r += (int64_t)gridDim.x * 4) {
-    double s = lane < nb ? A[r * lda + k0 + lane] * y[lane] : 0.0;
-    s = wave_sum(s);
-    if (lane == 0) b[r] -= s;
+constexpr int64_t kSpinLimit = 1LL << 26;
+
+__device__ inline void wait_flag(const uint32_t* flag, uint32_t want, int32_t* err) {
+  int64_t spins = 0;
+  while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != want) {
+    __builtin_amdgcn_s_sleep(1);
+    if (++spins > kSpinLimit) {
+      atomicOr(err, 1);
+      break;
+    }
   }
 }
 
-__global__ void __launch_bounds__(256) trsv_bwd_kernel(const double* __restrict__ A, int64_t lda, int64_t n,
-                                                       int64_t k0, const double* __restrict__ inv,
-                                                       double* __restrict__ b, double* __restrict__ x_out,
-                                                       const int32_t* __restrict__ info) {
+__device__ inline void set_flag(uint32_t* flag, uint32_t v) {
+  __hip_atomic_store(flag, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// forward: y_k = inv_kk (b_k - sum_{j<k} L_kj y_j); block kb = blockIdx.x
+__global__ void __launch_bounds__(256) trsv_fwd_lookback(const double* __restrict__ A, int64_t lda, int64_t n,
+                                                         const double* __restrict__ inv,
+                                                         const double* __restrict__ b, double* y,
+                                                         uint32_t* flags, const int32_t* __restrict__ info,
+                                                         int32_t* err) {
   if (*info != 0) return;
-  __shared__ double y[CH_NB];
-  __shared__ double bk[CH_NB];
+  __shared__ double part[4][CH_NB];
+  __shared__ double t[CH_NB];
+  const int kb = blockIdx.x;
+  const int64_t k0 = (int64_t)kb * CH_NB;
   const int nb = (int)min<int64_t>(CH_NB, n - k0);
-  const int tid = threadIdx.x;
-  if (tid < CH_NB) bk[tid] = tid < nb ? b[k0 + tid] : 0.0;
-  __syncthreads();
-  if (tid < CH_NB) {
-    // x_k = inv^T b_k
-    double s = 0.0;
-    for (int c = tid; c < CH_NB; ++c) s += inv[c * CH_NB + tid] * bk[c];
-    y[tid] = tid < nb ? s : 0.0;
-    if (blockIdx.x == 0 && tid < nb) x_out[k0 + tid] = s;
+  const int wid = threadIdx.x / 64, r = lane_id();
+  double acc = 0.0;
+  for (int jb = wid; jb < kb; jb += 4) {
+    const int64_t j0 = (int64_t)jb * CH_NB;
+    double Lrow[CH_NB];
+#pragma unroll
+    for (int c = 0; c < CH_NB; ++c) Lrow[c] = r < nb ? A[(k0 + r) * lda + j0 + c] : 0.0;
+    if (r == 0) wait_flag(flags + jb, 1u, err);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+#pragma unroll
+    for (int c = 0; c < CH_NB; ++c) acc += Lrow[c] * y[j0 + c];
   }
+  part[wid][r] = acc;
   __syncthreads();
-  // rows above: b[r] -= sum_c A[k0+c, r] y[c]; thread per row (coalesced along r)
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + tid; r < k0; r += (int64_t)gridDim.x * blockDim.x) {
+  if (wid == 0) {
+    const double bk = r < nb ? b[k0 + r] : 0.0;
+    t[r] = bk - (part[0][r] + part[1][r] + part[2][r] + part[3][r]);
+    wave_lds_sync();
     double s = 0.0;
-    for (int c = 0; c < nb; ++c) s += A[(k0 + c) * lda + r] * y[c];
-    b[r] -= s;
+    for (int c = 0; c <= r; ++c) s += inv[r * CH_NB + c] * t[c];
+    if (r < nb) y[k0 + r] = s;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_wave_barrier();
+    if (r == 0) set_flag(flags + kb, 1u);
+  }
+}
+
+// backward: x_k = inv_kk^T (y_k - sum_{j>k} L_jk^T x_j); workgroup w owns block
+// kb = nblk - 1 - w so that dependencies point to smaller workgroup ids
+__global__ void __launch_bounds__(256) trsv_bwd_lookback(const double* __restrict__ A, int64_t lda, int64_t n,
+                                                         const double* __restrict__ inv,
+                                                         const double* __restrict__ y, double* x,
+                                                         uint32_t* flags, const int32_t* __restrict__ info,
+                                                         int32_t* err) {
+  if (*info != 0) return;
+  __shared__ double part[4][CH_NB];
+  __shared__ double t[CH_NB];
+  const int nblk = gridDim.x;
+  const int kb = nblk - 1 - blockIdx.x;
+  const int64_t k0 = (int64_t)kb * CH_NB;
+  const int nb = (int)min<int64_t>(CH_NB, n - k0);
+  const int wid = threadIdx.x / 64, r = lane_id();
+  double acc = 0.0;
+  for (int jb = kb + 1 + wid; jb < nblk; jb += 4) {
+    const int64_t j0 = (int64_t)jb * CH_NB;
+    const int mb = (int)min<int64_t>(CH_NB, n - j0);
+    // column r of the tile L[j0:j0+mb, k0:k0+nb] (coalesced over r)
+    double Lcol[CH_NB];
+#pragma unroll
+    for (int c = 0; c < CH_NB; ++c) Lcol[c] = (c < mb && r < nb) ? A[(j0 + c) * lda + k0 + r] : 0.0;
+    if (r == 0) wait_flag(flags + (nblk - 1 - jb), 1u, err);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+#pragma unroll
+    for (int c = 0; c < CH_NB; ++c)
+      if (c < mb) acc += Lcol[c] * x[j0 + c];
+  }
+  part[wid][r] = acc;
+  __syncthreads();
+  if (wid == 0) {
+    const double yk = r < nb ? y[k0 + r] : 0.0;
+    t[r] = yk - (part[0][r] + part[1][r] + part[2][r] + part[3][r]);
+    wave_lds_sync();
+    double s = 0.0;
+    for (int c = r; c < CH_NB; ++c) s += inv[c * CH_NB + r] * t[c];
+    if (r < nb) x[k0 + r] = s;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_wave_barrier();
+    if (r == 0) set_flag(flags + blockIdx.x, 1u);
   }
 }
 
 // A failed factorisation leaves NaN in the solution, as LAPACK-based
 // cho_solve does on a non-positive-definite matrix.
-__global__ void nan_on_info_kernel(double* __restrict__ x, int64_t n, const int32_t* __restrict__ info) {
-  if (*info == 0) return;
+__global__ void nan_on_info_kernel(double* __restrict__ x, int64_t n, const int32_t* __restrict__ info,
+                                   const int32_t* __restrict__ err) {
+  if (*info == 0 && *err == 0) return;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     x[i] = __longlong_as_double(0x7FF8000000000000LL);
@@ -590,30 +660,57 @@ int nka_gemm_nt_f64(const double* a, int64_t lda, const double* b, int64_t ldb, 
   return NKA_OK;
 }
 
+// Workspace: 64x64 inverses of the diagonal blocks, the intermediate vector
+// of the solves, one flag per block and one error word.
+constexpr int CH_NBO = 256;  // outer panel width of the two-level blocking
+
 int64_t nka_potrf_workspace(int64_t n) {
   if (n <= 0) return 0;
   const int64_t nblk = (n + CH_NB - 1) / CH_NB;
-  return nblk * CH_NB * CH_NB * (int64_t)sizeof(double) + (n + 64) * (int64_t)sizeof(double);
+  return nblk * CH_NB * CH_NB * (int64_t)sizeof(double) + (n + 64) * (int64_t)sizeof(double) +
+         (nblk + 64) * (int64_t)sizeof(uint32_t);
 }
 
+static inline double* ws_inv(void* ws) { return (double*)ws; }
+static inline double* ws_vec(void* ws, int64_t nblk) {
+  return (double*)ws + nblk * CH_NB * CH_NB;
+}
+static inline uint32_t* ws_flags(void* ws, int64_t n, int64_t nblk) {
+  return (uint32_t*)(ws_vec(ws, nblk) + n + 64);
+}
+
+// Right-looking Cholesky with two levels of blocking: 256-column panels are
+// factored by 64-column steps (diagonal block in one wavefront, panel solve
+// and panel update as FP64 MFMA products restricted to the panel columns),
+// then the trailing matrix receives one rank-256 update.
 int nka_potrf_f64(double* a, int64_t n, int64_t lda, void* workspace, int32_t* info,
                   void* stream) {
   NKA_CHECK_ARG(n >= 0 && lda >= n && a && workspace && info, "nka_potrf_f64: bad arguments");
   hipStream_t s = (hipStream_t)stream;
-  double* inv = (double*)workspace;
-  for (int64_t k0 = 0; k0 < n; k0 += CH_NB) {
-    const int64_t nb = std::min<int64_t>(CH_NB, n - k0);
-    hipLaunchKernelGGL(potf2_inv_kernel, dim3(1), dim3(256), 0, s, a, lda, n, k0,
-                       inv + (k0 / CH_NB) * CH_NB * CH_NB, info);
-    const int64_t rest = n - k0 - nb;
-    if (rest > 0) {
-      // panel: L21 = A21 L11^{-T}  (in place, single column tile of width nb)
+  double* inv = ws_inv(workspace);
+  for (int64_t K0 = 0; K0 < n; K0 += CH_NBO) {
+    const int64_t nbo = std::min<int64_t>(CH_NBO, n - K0);
+    for (int64_t k0 = K0; k0 < K0 + nbo; k0 += CH_NB) {
+      const int64_t nb = std::min<int64_t>(CH_NB, n - k0);
+      double* invk = inv + (k0 / CH_NB) * CH_NB * CH_NB;
+      hipLaunchKernelGGL(potf2_inv_kernel, dim3(1), dim3(64), 0, s, a, lda, n, k0, invk, info);
+      const int64_t rest = n - k0 - nb;
+      if (rest <= 0) continue;
+      // L21 = A21 L11^{-T} for every row below (in place, one column tile)
       double* a21 = a + (k0 + nb) * lda + k0;
-      gemm_launch(a21, lda, inv + (k0 / CH_NB) * CH_NB * CH_NB, CH_NB, a21, lda, rest, nb, nb, 1.0,
-                  0.0, 0.0, 0, info, s);
-      // trailing update: A22 -= L21 L21^T (lower)
-      double* a22 = a + (k0 + nb) * lda + (k0 + nb);
-      gemm_launch(a21, lda, a21, lda, a22, lda, rest, rest, nb, -1.0, 1.0, 0.0, 1, info, s);
+      gemm_launch(a21, lda, invk, CH_NB, a21, lda, rest, nb, nb, 1.0, 0.0, 0.0, 0, info, s);
+      // update of the remaining columns of this panel
+      const int64_t cols = K0 + nbo - (k0 + nb);
+      if (cols > 0) {
+        double* c = a + (k0 + nb) * lda + (k0 + nb);
+        gemm_launch(a21, lda, a21, lda, c, lda, rest, cols, nb, -1.0, 1.0, 0.0, 0, info, s);
+      }
+    }
+    const int64_t rest2 = n - K0 - nbo;
+    if (rest2 > 0) {
+      const double* l21 = a + (K0 + nbo) * lda + K0;
+      double* a22 = a + (K0 + nbo) * lda + (K0 + nbo);
+      gemm_launch(l21, lda, l21, lda, a22, lda, rest2, rest2, nbo, -1.0, 1.0, 0.0, 1, info, s);
     }
   }
   NKA_CHECK_LAUNCH("nka_potrf_f64");
@@ -623,27 +720,24 @@ int nka_potrf_f64(double* a, int64_t n, int64_t lda, void* workspace, int32_t* i
 int nka_potrs_f64(const double* a, int64_t n, int64_t lda, const void* workspace, double* b,
                   const int32_t* info, void* stream) {
   NKA_CHECK_ARG(n >= 0 && lda >= n && a && workspace && b && info, "nka_potrs_f64: bad arguments");
+  if (n == 0) return NKA_OK;
   hipStream_t s = (hipStream_t)stream;
-  const double* inv = (const double*)workspace;
+  void* ws = (void*)workspace;
   const int64_t nblk = (n + CH_NB - 1) / CH_NB;
-  double* y = (double*)((char*)workspace + nblk * CH_NB * CH_NB * sizeof(double));
-  // forward: L y = b  (b is consumed; y collects the solution blocks)
-  for (int64_t kb = 0; kb < nblk; ++kb) {
-    const int64_t k0 = kb * CH_NB;
-    const int64_t rows = n - k0 - CH_NB;
-    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((rows + 3) / 4, 1024));
-    hipLaunchKernelGGL(trsv_fwd_kernel, dim3(g), dim3(256), 0, s, a, lda, n, k0,
-                       inv + kb * CH_NB * CH_NB, b, y, info);
-  }
-  // backward: L^T x = y  (y is consumed; x written into b)
-  for (int64_t kb = nblk - 1; kb >= 0; --kb) {
-    const int64_t k0 = kb * CH_NB;
-    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((k0 + 255) / 256, 1024));
-    hipLaunchKernelGGL(trsv_bwd_kernel, dim3(g), dim3(256), 0, s, a, lda, n, k0,
-                       inv + kb * CH_NB * CH_NB, y, b, info);
-  }
+  NKA_CHECK_ARG(nblk < (1LL << 31), "nka_potrs_f64: too large");
+  const double* inv = ws_inv(ws);
+  double* y = ws_vec(ws, nblk);
+  uint32_t* flags = ws_flags(ws, n, nblk);
+  int32_t* err = (int32_t*)(flags + nblk);
+  const unsigned gz = (unsigned)std::min<int64_t>((nblk + 1 + 255) / 256, 64);
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(gz), dim3(256), 0, s, flags, nblk + 1);
+  hipLaunchKernelGGL(trsv_fwd_lookback, dim3((unsigned)nblk), dim3(256), 0, s, a, lda, n, inv, b, y,
+                     flags, info, err);
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(gz), dim3(256), 0, s, flags, nblk);
+  hipLaunchKernelGGL(trsv_bwd_lookback, dim3((unsigned)nblk), dim3(256), 0, s, a, lda, n, inv, y, b,
+                     flags, info, err);
   hipLaunchKernelGGL(nan_on_info_kernel, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 1024)),
-                     dim3(256), 0, s, b, n, info);
+                     dim3(256), 0, s, b, n, info, err);
   NKA_CHECK_LAUNCH("nka_potrs_f64");
   return NKA_OK;
 }

@@ -114,10 +114,18 @@ __global__ void __launch_bounds__(256) rbm_metropolis_kernel(nka_rbm m, SweepArg
   uint64_t q = sa.counter;
   T nth[JL], nlc[JL];
   const int n_sweeps = sa.n_discard + sa.chain_length;
+  // the draw of the next proposal is computed one proposal ahead, so its
+  // latency overlaps the current proposal's reductions
+  u32x4 rnd_next = philox4x32_10({(uint32_t)q, (uint32_t)(q >> 32), (uint32_t)c, TAG_METROPOLIS},
+                                 sa.k0, sa.k1);
   for (int sw = 0; sw < n_sweeps; ++sw) {
     for (int p = 0; p < sa.sweep_size; ++p, ++q) {
-      const u32x4 rnd = philox4x32_10({(uint32_t)q, (uint32_t)(q >> 32), (uint32_t)c, TAG_METROPOLIS},
-                                      sa.k0, sa.k1);
+      const u32x4 rnd = rnd_next;
+      {
+        const uint64_t qn = q + 1;
+        rnd_next = philox4x32_10({(uint32_t)qn, (uint32_t)(qn >> 32), (uint32_t)c, TAG_METROPOLIS},
+                                 sa.k0, sa.k1);
+      }
       int sites[2];
       int nf;
       T corr = (T)0;
@@ -151,9 +159,7 @@ __global__ void __launch_bounds__(256) rbm_metropolis_kernel(nka_rbm m, SweepArg
             if (sel < 0 && r < RS) {
               const int cnt = __popcll(bal[r]);
               if (k < cnt) {
-                uint64_t mb = bal[r];
-                for (int z = 0; z < k; ++z) mb &= mb - 1;
-                sel = r * NKA_WAVE + (int)__builtin_ctzll(mb);
+                sel = r * NKA_WAVE + kth_set_bit(bal[r], k);
               } else {
                 k -= cnt;
               }

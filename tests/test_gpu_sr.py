@@ -97,7 +97,7 @@ def test_qgt_dense(gpu, dtype):
         assert np.all(err <= bound)
 
 
-@pytest.mark.parametrize("n", [1, 63, 64, 200, 333])
+@pytest.mark.parametrize("n", [1, 63, 64, 200, 333, 777, 1500])
 def test_cholesky_solve(gpu, n):
     print(f"case cholesky n={n}", flush=True)
     rng = np.random.default_rng(n)
