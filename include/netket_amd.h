@@ -143,13 +143,16 @@ int nka_random_states(int8_t* sigma, int n_chains, int n_sites, int n_down, uint
  * state is written to samples (n_chains, chain_length, n_sites).  sigma
  * (n_chains, n_sites) and log_prob (n_chains,, model dtype) are the persistent
  * chain state.  Proposal q of chain c draws Philox4x32-10 at counter
- * (counter + q, c, "METR") with key seed.  clusters: (n_clusters, 2) int32
- * (exchange only).  n_accepted (n_chains,) is incremented. */
+ * (counter + q, c, "METR") with key seed.  When counter_dev is not NULL the
+ * base counter is read from that device word instead and advanced there by
+ * (n_discard + chain_length) * sweep_size after the sweep, so a captured
+ * graph replays with fresh draws.  clusters: (n_clusters, 2) int32 (exchange
+ * only).  n_accepted (n_chains,) is incremented. */
 int nka_rbm_metropolis(const nka_rbm* m, int rule, const int32_t* clusters, int n_clusters,
                        int8_t* sigma, void* log_prob, int n_chains, int sweep_size,
                        int n_discard, int chain_length, double machine_pow, uint64_t seed,
-                       uint64_t counter, int8_t* samples, void* log_prob_samples,
-                       int64_t* n_accepted, void* stream);
+                       uint64_t counter, uint64_t* counter_dev, int8_t* samples,
+                       void* log_prob_samples, int64_t* n_accepted, void* stream);
 
 /* ------------------------------------------------------------------------
  * Statistics (netket/stats/mc_stats_old.py:87-196): out[0..7] =
@@ -211,6 +214,19 @@ int nka_rbm_jacobian(const nka_rbm* m, const int8_t* sigma, const void* t, const
 int nka_gram_bf16x3(const uint16_t* x_hi, const uint16_t* x_lo, int64_t n, int64_t k,
                     int64_t ldx, double shift, double offset, double* out, int64_t ldo,
                     void* stream);
+
+/* Centred NTK of the RBM without forming its Jacobian: out = scale * O_c O_c^T
+ * + shift I + offset 11^T (lower triangle incl. diagonal, FP64, row-major ldo),
+ * O_c[s,p] = O[s,p] - mean[p], O as for nka_rbm_contract and t from
+ * nka_rbm_prepare.  It uses (O O^T)_st = (hb + sigma_s.sigma_t)(t_s.t_t)
+ * + ha sigma_s.sigma_t (hb, ha = 1 with a hidden / visible bias), two FP64
+ * products with inner dimension n_hidden and n_visible instead of n_params.
+ * With scale = 1/ns this is T = O_L O_L^T of netket/_src/ngd/srt.py:55-66.
+ * workspace holds nka_rbm_ntk_workspace(...) bytes. */
+int64_t nka_rbm_ntk_workspace(const nka_rbm* m, int64_t ns);
+int nka_rbm_ntk(const nka_rbm* m, const int8_t* sigma, const void* t, int64_t ns,
+                const double* mean, double scale, double shift, double offset, double* out,
+                int64_t ldo, void* workspace, void* stream);
 
 /* C = beta C + alpha X X^T on the lower triangle (FP64 MFMA), X n x k row-major. */
 int nka_syrk_f64(const double* x, int64_t n, int64_t k, int64_t ldx, double alpha, double beta,

@@ -120,12 +120,41 @@ def gram(bufs: JacobianBuffers, shift: float, offset: float = 0.0,
     return out
 
 
+def rbm_ntk(model, flat, sigma2d: torch.Tensor, t: torch.Tensor, mean: torch.Tensor, scale: float,
+            shift: float, offset: float = 0.0, out: torch.Tensor | None = None) -> torch.Tensor:
+    """scale (O - mean)(O - mean)^T + shift I + offset 11^T on the lower triangle
+    (FP64) from the RBM's structure, without a Jacobian."""
+    n, N = sigma2d.shape
+    if out is None or out.shape != (n, n):
+        out = torch.empty((n, n), dtype=torch.float64, device=sigma2d.device)
+    m = model.native(flat, N)
+    ws = _ops.WORKSPACE.get("rbm_ntk", nat.lib().nka_rbm_ntk_workspace(C.byref(m), n))
+    with _ops.TIMER.region("ntk_structured"):
+        nat.call("nka_rbm_ntk", C.byref(m), _ops.ptr(sigma2d), _ops.ptr(t), n, _ops.ptr(mean),
+                 float(scale), float(shift), float(offset), _ops.ptr(out), out.stride(0),
+                 _ops.ptr(ws), _ops.stream())
+    return out
+
+
 def centre_scale(e: torch.Tensor, mean_dev: torch.Tensor, scale: float) -> torch.Tensor:
     out = torch.empty_like(e)
     with _ops.TIMER.region("centre"):
         nat.call("nka_centre_scale", _ops.ptr(e), e.numel(), _ops.ptr(mean_dev), float(scale),
                  _ops.ptr(out), _ops.stream())
     return out
+
+
+_ZERO = {}
+
+
+def scale(x: torch.Tensor, s: float) -> torch.Tensor:
+    """x * s as a new FP64 vector."""
+    dev = x.device
+    z = _ZERO.get(dev)
+    if z is None:
+        z = torch.zeros(1, dtype=torch.float64, device=dev)
+        _ZERO[dev] = z
+    return centre_scale(x, z, s)
 
 
 def sgd_update_(flat: torch.Tensor, dp: torch.Tensor, lr: float):

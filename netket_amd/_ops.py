@@ -240,17 +240,19 @@ def localop_conn(op, x, return_n_conn: bool = False):
 # linear algebra helpers
 # ---------------------------------------------------------------------------
 class Workspace:
-    """Grow-only device scratch buffers keyed by name (no per-call allocation)."""
+    """Grow-only device scratch buffers keyed by name and stream, so that work
+    enqueued on different streams never shares scratch memory."""
 
     def __init__(self):
         self._bufs = {}
 
     def get(self, name: str, nbytes: int) -> torch.Tensor:
         nbytes = max(int(nbytes), 16)
-        b = self._bufs.get(name)
+        key = (name, torch.cuda.current_stream().cuda_stream)
+        b = self._bufs.get(key)
         if b is None or b.numel() < nbytes:
             b = torch.empty(nbytes, dtype=torch.uint8, device=device())
-            self._bufs[name] = b
+            self._bufs[key] = b
         return b
 
     def clear(self):

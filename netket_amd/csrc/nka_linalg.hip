@@ -373,8 +373,9 @@ __global__ void __launch_bounds__(256) trsv_fwd_lookback(const double* __restric
     const double bk = r < nb ? b[k0 + r] : 0.0;
     t[r] = bk - (part[0][r] + part[1][r] + part[2][r] + part[3][r]);
     wave_lds_sync();
+    const double* invk = inv + (int64_t)kb * CH_NB * CH_NB;
     double s = 0.0;
-    for (int c = 0; c <= r; ++c) s += inv[r * CH_NB + c] * t[c];
+    for (int c = 0; c <= r; ++c) s += invk[r * CH_NB + c] * t[c];
     if (r < nb) y[k0 + r] = s;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __builtin_amdgcn_wave_barrier();
@@ -418,8 +419,9 @@ __global__ void __launch_bounds__(256) trsv_bwd_lookback(const double* __restric
     const double yk = r < nb ? y[k0 + r] : 0.0;
     t[r] = yk - (part[0][r] + part[1][r] + part[2][r] + part[3][r]);
     wave_lds_sync();
+    const double* invk = inv + (int64_t)kb * CH_NB * CH_NB;
     double s = 0.0;
-    for (int c = r; c < CH_NB; ++c) s += inv[c * CH_NB + r] * t[c];
+    for (int c = r; c < CH_NB; ++c) s += invk[c * CH_NB + r] * t[c];
     if (r < nb) x[k0 + r] = s;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __builtin_amdgcn_wave_barrier();

@@ -1,0 +1,206 @@
+// Structured neural tangent kernel of the RBM for minSR
+// (T = O_L O_L^T + eps I, netket/_src/ngd/srt.py:55-66).
+//
+// With O[s] = [b? t_s, sigma_s (x) t_s, a? sigma_s] (t = tanh theta) the
+// uncentred kernel is
+//   (O O^T)_st = (hb + sigma_s.sigma_t) (t_s.t_t) + ha sigma_s.sigma_t
+// (hb, ha = 1 when the hidden / visible bias exists), and centring with the
+// column means mu gives
+//   T_st = scale * [(O O^T)_st - v_s - v_t + c],  v = O mu,  c = |mu|^2,
+// scale = 1/Ns.  Both inner products are formed on FP64 matrix cores with
+// K = M and K = N, instead of the K = Np of the dense Jacobian product.
+#include "nka_common.hpp"
+#include "netket_amd.h"
+
+namespace nka {
+
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+// t64 = (double) t  (ns x M, model dtype -> FP64)
+template <class T>
+__global__ void to_f64_kernel(const T* __restrict__ t, int64_t total, double* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (double)t[i];
+}
+
+// v_s = hb t_s.mu_b + sum_i sigma_si U_si + ha sigma_s.mu_a, one wavefront per sample;
+// block 0 also reduces c = |mu|^2.
+__global__ void __launch_bounds__(256) ntk_vec_kernel(const double* __restrict__ t64, const int8_t* __restrict__ sigma,
+                                                      const double* __restrict__ U, int64_t ns, int N, int M,
+                                                      const double* __restrict__ mean, int64_t b_off,
+                                                      int64_t a_off, int64_t np, double* __restrict__ v,
+                                                      double* __restrict__ c) {
+  const int wid = threadIdx.x / 64, lane = lane_id();
+  for (int64_t s = (int64_t)blockIdx.x * 4 + wid; s < ns; s += (int64_t)gridDim.x * 4) {
+    double acc = 0.0;
+    for (int i = lane; i < N; i += 64) {
+      const double sg = sigma[s * N + i] > 0 ? 1.0 : -1.0;
+      acc += sg * U[s * N + i];
+      if (a_off >= 0) acc += sg * mean[a_off + i];
+    }
+    if (b_off >= 0)
+      for (int j = lane; j < M; j += 64) acc += t64[s * M + j] * mean[b_off + j];
+    acc = wave_sum(acc);
+    if (lane == 0) v[s] = acc;
+  }
+  if (blockIdx.x == 0) {
+    __shared__ double red[4];
+    double q = 0.0;
+    for (int64_t p = threadIdx.x; p < np; p += blockDim.x) q += mean[p] * mean[p];
+    q = wave_sum(q);
+    if (lane == 0) red[wid] = q;
+    __syncthreads();
+    if (threadIdx.x == 0) c[0] = red[0] + red[1] + red[2] + red[3];
+  }
+}
+
+__device__ inline void ntk_tri_tile(int64_t b, int& ti, int& tj) {
+  int64_t i = (int64_t)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
+  while ((i + 1) * (i + 2) / 2 <= b) ++i;
+  while (i * (i + 1) / 2 > b) --i;
+  ti = (int)i;
+  tj = (int)(b - i * (i + 1) / 2);
+}
+
+constexpr int NT_T = 64;
+constexpr int NT_K = 16;
+constexpr int NT_LD = 17;
+
+// Lower-triangular 64x64 tiles; 4 waves of 32x32 (2x2 v_mfma_f64_16x16x4).
+__global__ void __launch_bounds__(256) rbm_ntk_kernel(const double* __restrict__ t64, int M,
+                                                      const int8_t* __restrict__ sigma, int N, int64_t ns,
+                                                      const double* __restrict__ v,
+                                                      const double* __restrict__ c, double hb, double ha,
+                                                      double scale, double shift, double offset,
+                                                      double* __restrict__ out, int64_t ldo,
+                                                      int64_t n_tiles) {
+  __shared__ double sA[NT_T * NT_LD];
+  __shared__ double sB[NT_T * NT_LD];
+  const int64_t G = gridDim.x;
+  const int64_t bid = (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8;  // XCD-aware order
+  if (bid >= n_tiles) return;
+  int ti, tj;
+  ntk_tri_tile(bid, ti, tj);
+  const int64_t i0 = (int64_t)ti * NT_T, j0 = (int64_t)tj * NT_T;
+  const int tid = threadIdx.x, wid = tid / 64, lane = lane_id();
+  const int wm = (wid >> 1) * 32, wn = (wid & 1) * 32;
+  doublex4 th[2][2], sg[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) {
+      th[x][y] = (doublex4){0.0, 0.0, 0.0, 0.0};
+      sg[x][y] = (doublex4){0.0, 0.0, 0.0, 0.0};
+    }
+  // pass 0: Theta = t t^T (K = M); pass 1: Sigma = sigma sigma^T (K = N)
+  for (int pass = 0; pass < 2; ++pass) {
+    const int K = pass == 0 ? M : N;
+    for (int k0 = 0; k0 < K; k0 += NT_K) {
+      __syncthreads();
+#pragma unroll
+      for (int it = 0; it < 4; ++it) {
+        const int idx = tid + it * 256;
+        const int r = idx >> 4, kc = idx & 15;
+        const int kk = k0 + kc;
+        double va = 0.0, vb = 0.0;
+        if (kk < K) {
+          if (pass == 0) {
+            if (i0 + r < ns) va = t64[(i0 + r) * M + kk];
+            if (j0 + r < ns) vb = t64[(j0 + r) * M + kk];
+          } else {
+            if (i0 + r < ns) va = sigma[(i0 + r) * N + kk] > 0 ? 1.0 : -1.0;
+            if (j0 + r < ns) vb = sigma[(j0 + r) * N + kk] > 0 ? 1.0 : -1.0;
+          }
+        }
+        sA[r * NT_LD + kc] = va;
+        sB[r * NT_LD + kc] = vb;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int ks = 0; ks < NT_K; ks += 4) {
+        const int kq = ks + (lane >> 4);
+        double av[2], bv[2];
+#pragma unroll
+        for (int x = 0; x < 2; ++x) av[x] = sA[(wm + x * 16 + (lane & 15)) * NT_LD + kq];
+#pragma unroll
+        for (int y = 0; y < 2; ++y) bv[y] = sB[(wn + y * 16 + (lane & 15)) * NT_LD + kq];
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+          for (int y = 0; y < 2; ++y) {
+            if (pass == 0)
+              th[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x], bv[y], th[x][y], 0, 0, 0);
+            else
+              sg[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x], bv[y], sg[x][y], 0, 0, 0);
+          }
+      }
+    }
+  }
+  const double cc = *c;
+  // element e of lane l in a 16x16 tile is (row l/16 + 4e, col l%16)
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t gi = i0 + wm + x * 16 + (lane >> 4) + 4 * e;
+        const int64_t gj = j0 + wn + y * 16 + (lane & 15);
+        if (gi < ns && gj <= gi) {
+          const double S = sg[x][y][e], Th = th[x][y][e];
+          double val = scale * ((hb + S) * Th + ha * S - v[gi] - v[gj] + cc) + offset;
+          if (gi == gj) val += shift;
+          out[gi * ldo + gj] = val;
+        }
+      }
+}
+
+}  // namespace nka
+
+using namespace nka;
+
+extern "C" {
+
+int64_t nka_rbm_ntk_workspace(const nka_rbm* m, int64_t ns) {
+  if (!m || ns <= 0) return 0;
+  return (ns * (int64_t)m->n_hidden + ns * (int64_t)m->n_visible + ns + 8) * (int64_t)sizeof(double);
+}
+
+int nka_rbm_ntk(const nka_rbm* m, const int8_t* sigma, const void* t, int64_t ns,
+                const double* mean, double scale, double shift, double offset, double* out,
+                int64_t ldo, void* workspace, void* stream) {
+  NKA_CHECK_ARG(m && m->params && sigma && t && mean && out && workspace && ns > 0,
+                "nka_rbm_ntk: bad arguments");
+  NKA_CHECK_ARG(ldo >= ns, "nka_rbm_ntk: ldo < ns");
+  hipStream_t s = (hipStream_t)stream;
+  const int N = m->n_visible, M = m->n_hidden;
+  const int64_t np = (m->bias_off >= 0 ? M : 0) + (int64_t)N * M + (m->visible_off >= 0 ? N : 0);
+  double* t64 = (double*)workspace;
+  double* U = t64 + ns * M;
+  double* v = U + ns * N;
+  double* c = v + ns;
+  const int64_t total = ns * M;
+  const unsigned g1 = (unsigned)std::min<int64_t>((total + 255) / 256, 65536);
+  if (m->dtype == NKA_F32)
+    hipLaunchKernelGGL(to_f64_kernel<float>, dim3(g1), dim3(256), 0, s, (const float*)t, total, t64);
+  else
+    hipLaunchKernelGGL(to_f64_kernel<double>, dim3(g1), dim3(256), 0, s, (const double*)t, total, t64);
+  // U = t mu_W^T (ns x N): mu_W is the (N x M) block of the means
+  int st = nka_gemm_nt_f64(t64, M, mean + m->kernel_off, M, U, N, ns, N, M, 1.0, 0.0, stream);
+  if (st) return st;
+  const unsigned g2 = (unsigned)std::min<int64_t>((ns + 3) / 4, 4096);
+  hipLaunchKernelGGL(ntk_vec_kernel, dim3(g2), dim3(256), 0, s, t64, sigma, U, ns, N, M, mean,
+                     m->bias_off, m->visible_off, np, v, c);
+  const int64_t T = (ns + NT_T - 1) / NT_T;
+  const int64_t tiles = T * (T + 1) / 2;
+  NKA_CHECK_ARG(tiles < (1LL << 31), "nka_rbm_ntk: too large");
+  const unsigned grid = (unsigned)(((tiles + 7) / 8) * 8);
+  hipLaunchKernelGGL(rbm_ntk_kernel, dim3(grid), dim3(256), 0, s, t64, M, sigma, N, ns, v, c,
+                     m->bias_off >= 0 ? 1.0 : 0.0, m->visible_off >= 0 ? 1.0 : 0.0, scale, shift,
+                     offset, out, ldo, tiles);
+  NKA_CHECK_LAUNCH("nka_rbm_ntk");
+  return NKA_OK;
+}
+
+}  // extern "C"

@@ -65,6 +65,7 @@ struct SweepArgs {
   double machine_pow;
   uint32_t k0, k1;
   uint64_t counter;
+  const uint64_t* counter_dev;  // when set, the counter is read from device memory
 };
 
 template <class T, int JL, bool STAGE>
@@ -112,6 +113,11 @@ __global__ void __launch_bounds__(256) rbm_metropolis_kernel(nka_rbm m, SweepArg
 
   int64_t nacc = 0;
   uint64_t q = sa.counter;
+  if (sa.counter_dev) {
+    const uint64_t cd = *sa.counter_dev;
+    q = ((uint64_t)uniform_value<uint32_t>((uint32_t)(cd >> 32)) << 32) |
+        (uint64_t)uniform_value<uint32_t>((uint32_t)cd);
+  }
   T nth[JL], nlc[JL];
   const int n_sweeps = sa.n_discard + sa.chain_length;
   // the draw of the next proposal is computed one proposal ahead, so its
@@ -224,6 +230,11 @@ __global__ void __launch_bounds__(256) rbm_metropolis_kernel(nka_rbm m, SweepArg
   }
 }
 
+// counter += inc (one thread), after the sweep that consumed the counter
+__global__ void advance_counter_kernel(uint64_t* __restrict__ c, uint64_t inc) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) c[0] = c[0] + inc;
+}
+
 template <class T>
 static bool stage_fits_s(const nka_rbm& m, int waves) {
   return pad16((size_t)m.n_visible * m.n_hidden * sizeof(T)) + waves * pad16(m.n_visible) <=
@@ -278,8 +289,8 @@ int nka_random_states(int8_t* sigma, int n_chains, int n_sites, int n_down, uint
 int nka_rbm_metropolis(const nka_rbm* m, int rule, const int32_t* clusters, int n_clusters,
                        int8_t* sigma, void* log_prob, int n_chains, int sweep_size,
                        int n_discard, int chain_length, double machine_pow, uint64_t seed,
-                       uint64_t counter, int8_t* samples, void* log_prob_samples,
-                       int64_t* n_accepted, void* stream) {
+                       uint64_t counter, uint64_t* counter_dev, int8_t* samples,
+                       void* log_prob_samples, int64_t* n_accepted, void* stream) {
   NKA_CHECK_ARG(m && m->params, "nka_rbm_metropolis: null model");
   NKA_CHECK_ARG(m->n_visible > 0 && m->n_hidden > 0 && m->n_hidden <= kMaxJL * NKA_WAVE,
                 "nka_rbm_metropolis: unsupported sizes");
@@ -308,6 +319,7 @@ int nka_rbm_metropolis(const nka_rbm* m, int rule, const int32_t* clusters, int 
   sa.k0 = (uint32_t)seed;
   sa.k1 = (uint32_t)(seed >> 32);
   sa.counter = counter;
+  sa.counter_dev = counter_dev;
   const int jl = pick_jl(m->n_hidden);
   hipStream_t s = (hipStream_t)stream;
   if (m->dtype == NKA_F32) {
@@ -316,6 +328,10 @@ int nka_rbm_metropolis(const nka_rbm* m, int rule, const int32_t* clusters, int 
   } else {
     NKA_DISPATCH_JL(jl, launch_sweep_f64, *m, sa, sigma, log_prob, samples, log_prob_samples,
                     n_accepted, s);
+  }
+  if (counter_dev) {
+    const uint64_t inc = (uint64_t)(n_discard + chain_length) * (uint64_t)sweep_size;
+    hipLaunchKernelGGL(advance_counter_kernel, dim3(1), dim3(64), 0, s, counter_dev, inc);
   }
   NKA_CHECK_LAUNCH("nka_rbm_metropolis");
   return NKA_OK;

@@ -24,7 +24,12 @@ from . import _ops
 from .logging import RuntimeLog
 from .optimizer import solver as solvers
 from .optimizer.sr import Sgd, identity_preconditioner
-from .stats import statistics
+from .stats import Stats, statistics
+
+
+def statistics_from_moments(st: Stats) -> Stats:
+    """A Stats object owning a copy of ``st``'s device moments."""
+    return Stats(_moments=st._moments.clone(), _shape=st._shape)
 
 
 class AbstractVariationalDriver:
@@ -101,6 +106,45 @@ class AbstractVariationalDriver:
     def energy(self):
         return self._loss_stats
 
+    # -- captured steps --------------------------------------------------------
+    def _eager_step(self):
+        self.reset_step()
+        st, dp = self.compute_loss_and_update()
+        self.update_parameters(dp)
+        return st
+
+    def capture_step(self, warmup: int = 2):
+        """Capture one optimisation step into a HIP graph.
+
+        ``warmup`` real steps run first on the capture stream (so every
+        workspace exists), then one step is recorded; each ``replay_step`` then
+        performs a full step (sampling with fresh Philox draws from the
+        device-side counter, local energies, solve, update) with a single
+        launch from the host.  Learning rate and diag_shift are fixed at
+        capture time.
+        """
+        self._graph_stream = torch.cuda.Stream()
+        self._graph_stream.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(self._graph_stream):
+            for _ in range(warmup):
+                self._eager_step()
+                self._step_count += 1
+        torch.cuda.current_stream().wait_stream(self._graph_stream)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=self._graph_stream):
+            st = self._eager_step()
+        self._graph = g
+        self._graph_stats = st
+        return g
+
+    def replay_step(self):
+        """One captured step; returns its Stats (resolved lazily)."""
+        self._graph.replay()
+        self._step_count += 1
+        st = self._graph_stats
+        return statistics_from_moments(st)
+
 
 class VMC(AbstractVariationalDriver):
     """Energy minimisation by VMC with an optional preconditioner (e.g. SR)."""
@@ -127,8 +171,11 @@ class VMC_SR(AbstractVariationalDriver):
 
     def __init__(self, hamiltonian, optimizer, *, diag_shift, variational_state, proj_reg=None,
                  momentum=None, linear_solver=solvers.cholesky_with_fallback, linear_solver_fn=None,
-                 chunk_size_bwd=None, mode=None, use_ntk=None, on_the_fly=None):
+                 chunk_size_bwd=None, mode=None, use_ntk=None, on_the_fly=None,
+                 ntk_method="structured"):
         super().__init__(variational_state, optimizer)
+        if ntk_method not in ("structured", "jacobian"):
+            raise ValueError("ntk_method must be 'structured' or 'jacobian'")
         if linear_solver_fn is not None:
             linear_solver = linear_solver_fn
         if mode not in (None, "real"):
@@ -150,6 +197,9 @@ class VMC_SR(AbstractVariationalDriver):
         self.use_ntk = bool(use_ntk)
         self.on_the_fly = False
         self.linear_solver = linear_solver
+        # NTK T from the RBM's structure (FP64, inner sizes n_hidden and n_visible)
+        # or from the explicit Jacobian (bf16x3 / FP64 Gram over n_params)
+        self.ntk_method = ntk_method
         self._old_updates = None
         self._jac = None
         self.info = None
@@ -175,16 +225,20 @@ class VMC_SR(AbstractVariationalDriver):
         mean = eng.contract(model, flat, sig, t, ones)[0]
         # dv = 2 (E - <E>) / sqrt(Ns)   (sr_srt_common.py:75-77)
         dv = eng.centre_scale(e, st.mean_device, 2.0 / math.sqrt(ns))
-        kind = nat.OUT_BF16X2 if flat.dtype == torch.float32 else nat.OUT_F64
-        layout = nat.LAYOUT_SP if self.use_ntk else nat.LAYOUT_PS
-        self._jac = eng.jacobian(model, flat, sig, t, tT, mean, 1.0 / math.sqrt(ns), layout, kind,
-                                 self._jac)
+        # contraction weights dv / sqrt(Ns) = 2 (E - <E>) / Ns
+        w_dv = eng.centre_scale(e, st.mean_device, 2.0 / ns)
+        structured = self.use_ntk and self.ntk_method == "structured"
+        if not structured:
+            kind = nat.OUT_BF16X2 if flat.dtype == torch.float32 else nat.OUT_F64
+            layout = nat.LAYOUT_SP if self.use_ntk else nat.LAYOUT_PS
+            self._jac = eng.jacobian(model, flat, sig, t, tT, mean, 1.0 / math.sqrt(ns), layout,
+                                     kind, self._jac)
         X = self._jac
         if not self.use_ntk:
             if proj_reg is not None:
                 raise ValueError("proj_reg not implemented for SR")
             # F = O_L^T dv  (the contraction weights are dv / sqrt(Ns))
-            F = eng.contract(model, flat, sig, t, (dv / math.sqrt(ns)).reshape(1, -1), mean)[0]
+            F = eng.contract(model, flat, sig, t, w_dv.reshape(1, -1), mean)[0]
             if momentum is not None and self._old_updates is not None:
                 F = F + diag_shift * momentum * self._old_updates
             system = solvers.DenseSystem(builder=lambda: eng.gram(X, diag_shift))
@@ -196,10 +250,15 @@ class VMC_SR(AbstractVariationalDriver):
             if momentum is not None and self._old_updates is not None:
                 raise NotImplementedError("SPRING momentum with the NTK is not supported")
             offset = 0.0 if proj_reg is None else float(proj_reg) / ns
-            system = solvers.DenseSystem(builder=lambda: eng.gram(X, diag_shift, offset))
+            if structured:
+                system = solvers.DenseSystem(builder=lambda: eng.rbm_ntk(
+                    model, flat, sig, t, mean, 1.0 / ns, diag_shift, offset))
+            else:
+                system = solvers.DenseSystem(builder=lambda: eng.gram(X, diag_shift, offset))
             y, info = self._solve(system, dv)
             # updates = O_L^T y (srt.py:79), from the analytic Jacobian
-            dp = eng.contract(model, flat, sig, t, (y / math.sqrt(ns)).reshape(1, -1), mean)[0]
+            dp = eng.contract(model, flat, sig, t, eng.scale(y, 1.0 / math.sqrt(ns)).reshape(1, -1),
+                              mean)[0]
         self.info = info
         return st, dp
 

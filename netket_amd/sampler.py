@@ -91,6 +91,7 @@ class MetropolisSamplerState:
     counter: int = 0                     # proposals drawn so far per chain
     n_accepted_proc: torch.Tensor | None = None   # (n_chains,) int64
     n_steps_proc: int = 0
+    counter_dev: torch.Tensor | None = None  # the counter on the device (source of truth)
     extra: dict = field(default_factory=dict)
 
     @property
@@ -161,8 +162,10 @@ class MetropolisSampler:
         seed = int(seed)
         sigma = self._random_sigma(seed, 0)
         nacc = torch.zeros(self.n_chains, dtype=torch.int64, device=sigma.device)
-        return MetropolisSamplerState(sigma=sigma, log_prob=None, seed=seed, counter=1 << 40,
-                                      n_accepted_proc=nacc)
+        counter = 1 << 40
+        cdev = torch.tensor([counter], dtype=torch.int64, device=sigma.device)
+        return MetropolisSamplerState(sigma=sigma, log_prob=None, seed=seed, counter=counter,
+                                      n_accepted_proc=nacc, counter_dev=cdev)
 
     def reset(self, machine, parameters, state: MetropolisSamplerState) -> MetropolisSamplerState:
         """Zero the acceptance counters; redraw the chains when reset_chains is set."""
@@ -206,8 +209,9 @@ class MetropolisSampler:
             nat.call("nka_rbm_metropolis", C.byref(m), self.rule.kind, cl_ptr, n_cl,
                      _ops.ptr(state.sigma), _ops.ptr(state.log_prob), self.n_chains,
                      self.sweep_size, int(n_discard), int(chain_length), float(self.machine_pow),
-                     state.seed & (2**64 - 1), state.counter, _ops.ptr(samples), _ops.ptr(lps),
-                     _ops.ptr(state.n_accepted_proc), _ops.stream())
+                     state.seed & (2**64 - 1), state.counter, _ops.ptr(state.counter_dev),
+                     _ops.ptr(samples), _ops.ptr(lps), _ops.ptr(state.n_accepted_proc),
+                     _ops.stream())
         n_prop = (int(n_discard) + int(chain_length)) * self.sweep_size
         state.counter += n_prop
         state.n_steps_proc += n_prop * self.n_chains

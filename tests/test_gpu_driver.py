@@ -54,3 +54,27 @@ def test_vmc_sr_and_minsr_runs_agree(gpu):
         energies[(ntk, "p")] = params
     np.testing.assert_allclose(energies[False], energies[True], rtol=0, atol=1e-10)
     np.testing.assert_allclose(energies[(False, "p")], energies[(True, "p")], rtol=1e-8, atol=1e-12)
+
+
+@pytest.mark.parametrize("ntk", [False, True])
+def test_graph_replay_matches_eager(gpu, ntk):
+    # a captured step replays sampling (device-side Philox counter), E_loc,
+    # Jacobian, Gram, Cholesky and update exactly as the eager path does
+    import torch
+
+    def make():
+        H, vs = _tfim_state(10, 16, 256, 4, seed=11, dtype=np.float32)
+        drv = nk.VMC_SR(H, nk.optimizer.Sgd(0.05), diag_shift=0.01, variational_state=vs,
+                        use_ntk=ntk, linear_solver=gsolver.cholesky)
+        return drv, vs
+
+    d1, v1 = make()
+    e1 = [d1._eager_step().mean for _ in range(5)]
+    d2, v2 = make()
+    d2.capture_step(warmup=2)
+    e2 = []
+    for _ in range(3):
+        e2.append(d2.replay_step().mean)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(v1.parameters_flat.cpu().numpy(), v2.parameters_flat.cpu().numpy())
+    np.testing.assert_array_equal(np.array(e1[2:]), np.array(e2))

@@ -97,6 +97,48 @@ def test_qgt_dense(gpu, dtype):
         assert np.all(err <= bound)
 
 
+@pytest.mark.parametrize("dtype,ns,biases", [(np.float64, 300, (True, True)),
+                                             (np.float64, 64, (False, True)),
+                                             (np.float64, 129, (True, False)),
+                                             (np.float32, 300, (True, True)),
+                                             (np.float64, 1, (True, True))])
+def test_rbm_ntk_structured(gpu, dtype, ns, biases):
+    # T = (O - mean)(O - mean)^T / ns + shift I + offset 11^T from the RBM
+    # structure equals the explicit product of the oracle's FP64 Jacobian
+    # (netket/_src/ngd/srt.py:55-66 with proj_reg giving the offset)
+    print(f"case rbm_ntk {np.dtype(dtype)} ns={ns} biases={biases}", flush=True)
+    N, alpha = 14, 3
+    M = alpha * N
+    model = nk.models.RBM(alpha=alpha, param_dtype=dtype, use_hidden_bias=biases[0],
+                          use_visible_bias=biases[1])
+    rng = np.random.default_rng(ns + 7)
+    n_par = model.layout(N)["n_params"]
+    flat = rng.normal(scale=0.3, size=n_par).astype(dtype)
+    x = rng.choice(np.array([-1, 1], dtype=np.int8), size=(ns, N))
+    sig = torch.as_tensor(x, device="cuda")
+    fl = torch.as_tensor(flat, device="cuda")
+    t = eng.prepare(model, fl, sig)[0]
+    ones = torch.full((1, ns), 1.0 / ns, dtype=torch.float64, device="cuda")
+    mean = eng.contract(model, fl, sig, t, ones)[0]
+    shift, offset = 1e-3, 0.25 / ns
+    T = eng.rbm_ntk(model, fl, sig, t, mean, 1.0 / ns, shift, offset).cpu().numpy()
+    p = orbm.RBMParams(flat.astype(np.float64), N, M, hidden_bias=biases[0],
+                       visible_bias=biases[1])
+    Ou = orbm.jacobian(p, x)
+    Oc = Ou - Ou.mean(axis=0)
+    ref = Oc @ Oc.T / ns + shift * np.eye(ns) + offset
+    lo = np.tril_indices(ns)
+    # the centring is applied to the uncentred products, so rounding scales
+    # with the uncentred norms |O_s| |O_t| / ns
+    dc = np.sqrt(np.sum(Oc * Oc, axis=1) / ns)
+    du = np.sqrt(np.sum(Ou * Ou, axis=1) / ns)
+    tol = 1e-12 if dtype == np.float64 else 2e-6
+    bound = (tol * np.outer(dc, dc) + 1e-14 * np.outer(du, du))[lo] + 1e-15
+    err = np.abs(T[lo] - ref[lo])
+    print(f"max |dT| / bound = {(err / bound).max():.2e}", flush=True)
+    assert np.all(err <= bound)
+
+
 @pytest.mark.parametrize("n", [1, 63, 64, 200, 333, 777, 1500])
 def test_cholesky_solve(gpu, n):
     print(f"case cholesky n={n}", flush=True)
@@ -143,10 +185,11 @@ def test_cg(gpu):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_vmc_sr_update_matches_oracle_and_minsr(gpu, dtype):
+@pytest.mark.parametrize("ntk_method", ["structured", "jacobian"])
+def test_vmc_sr_update_matches_oracle_and_minsr(gpu, dtype, ntk_method):
     # VMC_SR with use_ntk=False and use_ntk=True give the same update
     # (test/driver/test_srt.py:98-211), and both match the oracle's SR update
-    print(f"case sr/minsr {np.dtype(dtype)}", flush=True)
+    print(f"case sr/minsr {np.dtype(dtype)} {ntk_method}", flush=True)
     vs, H, p, conn = _state("ising", dtype, 16, 256, scale=0.1)
     samples = vs.samples.cpu().numpy()
     x = samples.reshape(-1, vs.hilbert.size)
@@ -158,7 +201,7 @@ def test_vmc_sr_update_matches_oracle_and_minsr(gpu, dtype):
     out = {}
     for ntk in (False, True):
         drv = nk.VMC_SR(H, nk.optimizer.Sgd(0.01), diag_shift=1e-3, variational_state=vs,
-                        use_ntk=ntk, linear_solver=gsolver.cholesky)
+                        use_ntk=ntk, linear_solver=gsolver.cholesky, ntk_method=ntk_method)
         st, dp = drv.compute_loss_and_update()
         out[ntk] = dp.cpu().numpy()
         assert vs.samples.cpu().numpy() is not None
