@@ -258,55 +258,124 @@ __global__ void __launch_bounds__(256) gemm_nt_f64_kernel(const double* a, int64
 // ---------------------------------------------------------------------------
 constexpr int CH_NB = 64;
 
-// Factor the diagonal block A[k0:k0+nb, k0:k0+nb] with one wavefront, write
-// L11 back and L11^{-1} to inv (64 x 64).  The block lives in LDS; lane i owns
-// row i, so the only synchronisation is within the wavefront.  Rows/columns
-// past nb are an identity padding.  A non-positive or non-finite pivot sets
-// *info (first failing column + 1) and is replaced by 1 so the kernel still
-// completes.
-__global__ void __launch_bounds__(64) potf2_inv_kernel(double* __restrict__ A, int64_t lda, int64_t n,
-                                                       int64_t k0, double* __restrict__ inv,
-                                                       int32_t* __restrict__ info) {
+constexpr int CH_LT = CH_NB + 2;  // LDS row stride of the transposed factor (16-byte rows)
+constexpr int TR_ROWS = 256;      // panel rows solved per workgroup (one per thread)
+
+// Factor the diagonal block A[k0:k0+nb, k0:k0+nb] in place (lower triangle)
+// with one wavefront: lane i keeps row i in registers, right-looking; the
+// unscaled column of step j is broadcast through LDS, its element j is the
+// pivot, and the update is a_c -= L_ij L_cj = a_ij a_cj / d^2.  A non-positive
+// or non-finite pivot sets *info (first failing column + 1) and is replaced
+// by 1 so the kernel completes.  Rows/columns past nb are an identity padding.
+__global__ void __launch_bounds__(64) potf2_kernel(double* __restrict__ A, int64_t lda, int64_t n,
+                                                   int64_t k0, int32_t* __restrict__ info) {
   if (*info != 0) return;
-  __shared__ double Ls[CH_NB][CH_NB + 1];
-  __shared__ double Xs[CH_NB][CH_NB + 1];
-  const int i = threadIdx.x;
+  __shared__ double colbuf[CH_NB];
+  const int lane = threadIdx.x;
   const int nb = (int)min<int64_t>(CH_NB, n - k0);
-  for (int idx = i; idx < CH_NB * CH_NB; idx += 64) {
-    const int r = idx >> 6, c = idx & 63;
-    double v = 0.0;
-    if (r < nb && c < nb) v = (c <= r) ? A[(k0 + r) * lda + k0 + c] : 0.0;
-    else if (r == c) v = 1.0;
-    Ls[r][c] = v;
+  const double* arow = A + (k0 + lane) * lda + k0;
+  double a[CH_NB];
+#pragma unroll
+  for (int c = 0; c < CH_NB; ++c) {
+    double v = (lane == c) ? 1.0 : 0.0;
+    if (lane < nb && c < nb && c <= lane) v = arow[c];
+    a[c] = v;
   }
-  wave_lds_sync();
   int bad = 0;
+#pragma unroll
   for (int j = 0; j < CH_NB; ++j) {
-    const double ajj = Ls[j][j];
+    colbuf[lane] = a[j];
+    wave_lds_sync();
+    const double ajj = colbuf[j];
     const bool ok = (ajj > 0.0) && isfinite(ajj);
     if (!ok && bad == 0) bad = (int)(k0 + j + 1);
     const double d = ok ? sqrt(ajj) : 1.0;
-    wave_lds_sync();
-    if (i == j) Ls[j][j] = d;
-    if (i > j) Ls[i][j] = Ls[i][j] / d;
-    wave_lds_sync();
-    const double lij = (i > j) ? Ls[i][j] : 0.0;
-    for (int c = j + 1; c <= i; ++c) Ls[i][c] -= lij * Ls[c][j];
+    const double rd = 1.0 / d;
+    const double lij = (lane > j) ? a[j] * rd : (lane == j ? d : 0.0);
+    const double sij = (lane > j) ? lij * rd : 0.0;
+    a[j] = lij;
+#pragma unroll
+    for (int c = j + 1; c < CH_NB; ++c) a[c] -= sij * colbuf[c];
     wave_lds_sync();
   }
-  for (int idx = i; idx < CH_NB * CH_NB; idx += 64) {
+  if (lane < nb) {
+    double* wrow = A + (k0 + lane) * lda + k0;
+#pragma unroll
+    for (int c = 0; c < CH_NB; ++c)
+      if (c <= lane && c < nb) wrow[c] = a[c];
+  }
+  if (lane == 0 && bad) atomicCAS(info, 0, bad);
+}
+
+// Panel solve L21 = A21 L11^{-T} in place for the rows below the diagonal
+// block k0 (row k0+64+256 b+t in thread t of workgroup b): L11^T is staged in
+// LDS and each thread substitutes its row column by column
+// (x_j = (a_j - sum_{c<j} x_c L_jc) / L_jj).  Rows exist below a block only
+// when the block is a full 64 columns wide, so the row loads are unmasked.
+__global__ void __launch_bounds__(256) trsm_panel_kernel(double* __restrict__ A, int64_t lda, int64_t n,
+                                                         int64_t k0, const int32_t* __restrict__ info) {
+  if (*info != 0) return;
+  __shared__ double LT[CH_NB * CH_LT];   // LT[c * CH_LT + r] = L11[r][c]
+  __shared__ double rdiag[CH_NB];
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < CH_NB * CH_NB; idx += 256) {
     const int r = idx >> 6, c = idx & 63;
-    if (r < nb && c < nb && c <= r) A[(k0 + r) * lda + k0 + c] = Ls[r][c];
+    LT[c * CH_LT + r] = (c <= r) ? A[(k0 + r) * lda + k0 + c] : 0.0;
   }
-  // column i of X = L^{-1} by forward substitution; X[k][i] = 0 for k < i
-  for (int r = 0; r < CH_NB; ++r) {
-    double sum = 0.0;
-    for (int k = 0; k < r; ++k) sum += Ls[r][k] * Xs[k][i];
-    Xs[r][i] = (r < i) ? 0.0 : (((r == i) ? 1.0 : 0.0) - sum) / Ls[r][r];
+  __syncthreads();
+  if (tid < CH_NB) rdiag[tid] = 1.0 / LT[tid * CH_LT + tid];
+  __syncthreads();
+  const int64_t row = k0 + CH_NB + (int64_t)blockIdx.x * TR_ROWS + tid;
+  if (row >= n) return;
+  double* ar = A + row * lda + k0;
+  double x[CH_NB];
+#pragma unroll
+  for (int c = 0; c < CH_NB; ++c) x[c] = ar[c];
+#pragma unroll
+  for (int j = 0; j < CH_NB; ++j) {
+    const double xj = x[j] * rdiag[j];
+    x[j] = xj;
+#pragma unroll
+    for (int c = j + 1; c < CH_NB; ++c) x[c] -= xj * LT[j * CH_LT + c];
+  }
+#pragma unroll
+  for (int c = 0; c < CH_NB; ++c) ar[c] = x[c];
+}
+
+// inv_b = L_bb^{-1} for every 64x64 diagonal block b of the factor (row-major
+// 64x64 each, identity padding); one wavefront per block, lane k computes
+// column k by substitution with the block's columns broadcast from LDS.
+__global__ void __launch_bounds__(64) trtri_blocks_kernel(const double* __restrict__ A, int64_t lda,
+                                                          int64_t n, double* __restrict__ inv,
+                                                          const int32_t* __restrict__ info) {
+  if (*info != 0) return;
+  __shared__ double LT[CH_NB * CH_LT];
+  __shared__ double rdiag[CH_NB];
+  const int k = threadIdx.x;
+  const int64_t k0 = (int64_t)blockIdx.x * CH_NB;
+  const int nb = (int)min<int64_t>(CH_NB, n - k0);
+  for (int idx = k; idx < CH_NB * CH_NB; idx += 64) {
+    const int r = idx >> 6, c = idx & 63;
+    double v = (r == c) ? 1.0 : 0.0;
+    if (r < nb && c < nb && c <= r) v = A[(k0 + r) * lda + k0 + c];
+    LT[c * CH_LT + r] = v;
   }
   wave_lds_sync();
-  for (int idx = i; idx < CH_NB * CH_NB; idx += 64) inv[idx] = Xs[idx >> 6][idx & 63];
-  if (i == 0 && bad) atomicCAS(info, 0, bad);
+  rdiag[k] = 1.0 / LT[k * CH_LT + k];
+  wave_lds_sync();
+  double x[CH_NB];
+#pragma unroll
+  for (int r = 0; r < CH_NB; ++r) x[r] = (r == k) ? 1.0 : 0.0;
+#pragma unroll
+  for (int m = 0; m < CH_NB; ++m) {
+    const double xm = x[m] * rdiag[m];
+    x[m] = xm;
+#pragma unroll
+    for (int r = m + 1; r < CH_NB; ++r) x[r] -= xm * LT[m * CH_LT + r];
+  }
+  double* out = inv + (int64_t)blockIdx.x * CH_NB * CH_NB;
+#pragma unroll
+  for (int r = 0; r < CH_NB; ++r) out[r * CH_NB + k] = x[r];
 }
 
 // Zero n 32-bit words (a fill kernel rather than a memset node, so that a
@@ -342,7 +411,10 @@ __device__ inline void set_flag(uint32_t* flag, uint32_t v) {
   __hip_atomic_store(flag, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// forward: y_k = inv_kk (b_k - sum_{j<k} L_kj y_j); block kb = blockIdx.x
+// forward: y_k = inv_kk (b_k - sum_{j<k} L_kj y_j); block kb = blockIdx.x.
+// Each wave walks its share of the blocks j < k from the oldest (finished
+// first) to the newest, so only the last tile waits on the chain; the tiles
+// of L and the inverse block are loaded before any wait.
 __global__ void __launch_bounds__(256) trsv_fwd_lookback(const double* __restrict__ A, int64_t lda, int64_t n,
                                                          const double* __restrict__ inv,
                                                          const double* __restrict__ b, double* y,
@@ -351,10 +423,15 @@ __global__ void __launch_bounds__(256) trsv_fwd_lookback(const double* __restric
   if (*info != 0) return;
   __shared__ double part[4][CH_NB];
   __shared__ double t[CH_NB];
+  __shared__ double Xs[CH_NB * (CH_NB + 1)];
   const int kb = blockIdx.x;
   const int64_t k0 = (int64_t)kb * CH_NB;
   const int nb = (int)min<int64_t>(CH_NB, n - k0);
   const int wid = threadIdx.x / 64, r = lane_id();
+  const double* invk = inv + (int64_t)kb * CH_NB * CH_NB;
+  for (int idx = threadIdx.x; idx < CH_NB * CH_NB; idx += 256)
+    Xs[(idx >> 6) * (CH_NB + 1) + (idx & 63)] = invk[idx];
+  const double bk = (wid == 0 && r < nb) ? b[k0 + r] : 0.0;
   double acc = 0.0;
   for (int jb = wid; jb < kb; jb += 4) {
     const int64_t j0 = (int64_t)jb * CH_NB;
@@ -370,12 +447,10 @@ __global__ void __launch_bounds__(256) trsv_fwd_lookback(const double* __restric
   part[wid][r] = acc;
   __syncthreads();
   if (wid == 0) {
-    const double bk = r < nb ? b[k0 + r] : 0.0;
     t[r] = bk - (part[0][r] + part[1][r] + part[2][r] + part[3][r]);
     wave_lds_sync();
-    const double* invk = inv + (int64_t)kb * CH_NB * CH_NB;
     double s = 0.0;
-    for (int c = 0; c <= r; ++c) s += invk[r * CH_NB + c] * t[c];
+    for (int c = 0; c <= r; ++c) s += Xs[r * (CH_NB + 1) + c] * t[c];
     if (r < nb) y[k0 + r] = s;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __builtin_amdgcn_wave_barrier();
@@ -384,7 +459,8 @@ __global__ void __launch_bounds__(256) trsv_fwd_lookback(const double* __restric
 }
 
 // backward: x_k = inv_kk^T (y_k - sum_{j>k} L_jk^T x_j); workgroup w owns block
-// kb = nblk - 1 - w so that dependencies point to smaller workgroup ids
+// kb = nblk - 1 - w so that dependencies point to smaller workgroup ids.  Each
+// wave walks its blocks j > k from the farthest (finished first) down to k+1.
 __global__ void __launch_bounds__(256) trsv_bwd_lookback(const double* __restrict__ A, int64_t lda, int64_t n,
                                                          const double* __restrict__ inv,
                                                          const double* __restrict__ y, double* x,
@@ -393,13 +469,18 @@ __global__ void __launch_bounds__(256) trsv_bwd_lookback(const double* __restric
   if (*info != 0) return;
   __shared__ double part[4][CH_NB];
   __shared__ double t[CH_NB];
+  __shared__ double Xs[CH_NB * (CH_NB + 1)];
   const int nblk = gridDim.x;
   const int kb = nblk - 1 - blockIdx.x;
   const int64_t k0 = (int64_t)kb * CH_NB;
   const int nb = (int)min<int64_t>(CH_NB, n - k0);
   const int wid = threadIdx.x / 64, r = lane_id();
+  const double* invk = inv + (int64_t)kb * CH_NB * CH_NB;
+  for (int idx = threadIdx.x; idx < CH_NB * CH_NB; idx += 256)
+    Xs[(idx >> 6) * (CH_NB + 1) + (idx & 63)] = invk[idx];
+  const double yk = (wid == 0 && r < nb) ? y[k0 + r] : 0.0;
   double acc = 0.0;
-  for (int jb = kb + 1 + wid; jb < nblk; jb += 4) {
+  for (int jb = nblk - 1 - wid; jb > kb; jb -= 4) {
     const int64_t j0 = (int64_t)jb * CH_NB;
     const int mb = (int)min<int64_t>(CH_NB, n - j0);
     // column r of the tile L[j0:j0+mb, k0:k0+nb] (coalesced over r)
@@ -416,12 +497,10 @@ __global__ void __launch_bounds__(256) trsv_bwd_lookback(const double* __restric
   part[wid][r] = acc;
   __syncthreads();
   if (wid == 0) {
-    const double yk = r < nb ? y[k0 + r] : 0.0;
     t[r] = yk - (part[0][r] + part[1][r] + part[2][r] + part[3][r]);
     wave_lds_sync();
-    const double* invk = inv + (int64_t)kb * CH_NB * CH_NB;
     double s = 0.0;
-    for (int c = r; c < CH_NB; ++c) s += invk[c * CH_NB + r] * t[c];
+    for (int c = r; c < CH_NB; ++c) s += Xs[c * (CH_NB + 1) + r] * t[c];
     if (r < nb) x[k0 + r] = s;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __builtin_amdgcn_wave_barrier();
@@ -682,28 +761,29 @@ static inline uint32_t* ws_flags(void* ws, int64_t n, int64_t nblk) {
 }
 
 // Right-looking Cholesky with two levels of blocking: 256-column panels are
-// factored by 64-column steps (diagonal block in one wavefront, panel solve
-// and panel update as FP64 MFMA products restricted to the panel columns),
-// then the trailing matrix receives one rank-256 update.
+// factored by 64-column steps (diagonal block, panel solve by substitution,
+// then the update of the panel's remaining columns as an FP64 MFMA product),
+// then the trailing matrix receives one rank-256 update.  The inverses of the
+// diagonal blocks, used by the triangular solves, are formed at the end.
 int nka_potrf_f64(double* a, int64_t n, int64_t lda, void* workspace, int32_t* info,
                   void* stream) {
   NKA_CHECK_ARG(n >= 0 && lda >= n && a && workspace && info, "nka_potrf_f64: bad arguments");
+  if (n == 0) return NKA_OK;
   hipStream_t s = (hipStream_t)stream;
   double* inv = ws_inv(workspace);
   for (int64_t K0 = 0; K0 < n; K0 += CH_NBO) {
     const int64_t nbo = std::min<int64_t>(CH_NBO, n - K0);
     for (int64_t k0 = K0; k0 < K0 + nbo; k0 += CH_NB) {
       const int64_t nb = std::min<int64_t>(CH_NB, n - k0);
-      double* invk = inv + (k0 / CH_NB) * CH_NB * CH_NB;
-      hipLaunchKernelGGL(potf2_inv_kernel, dim3(1), dim3(64), 0, s, a, lda, n, k0, invk, info);
       const int64_t rest = n - k0 - nb;
-      if (rest <= 0) continue;
-      // L21 = A21 L11^{-T} for every row below (in place, one column tile)
-      double* a21 = a + (k0 + nb) * lda + k0;
-      gemm_launch(a21, lda, invk, CH_NB, a21, lda, rest, nb, nb, 1.0, 0.0, 0.0, 0, info, s);
+      hipLaunchKernelGGL(potf2_kernel, dim3(1), dim3(64), 0, s, a, lda, n, k0, info);
+      if (rest <= 0) continue;  // rows below imply nb == 64
+      const unsigned grid = (unsigned)((rest + TR_ROWS - 1) / TR_ROWS);
+      hipLaunchKernelGGL(trsm_panel_kernel, dim3(grid), dim3(256), 0, s, a, lda, n, k0, info);
       // update of the remaining columns of this panel
       const int64_t cols = K0 + nbo - (k0 + nb);
       if (cols > 0) {
+        const double* a21 = a + (k0 + nb) * lda + k0;
         double* c = a + (k0 + nb) * lda + (k0 + nb);
         gemm_launch(a21, lda, a21, lda, c, lda, rest, cols, nb, -1.0, 1.0, 0.0, 0, info, s);
       }
@@ -715,6 +795,9 @@ int nka_potrf_f64(double* a, int64_t n, int64_t lda, void* workspace, int32_t* i
       gemm_launch(l21, lda, l21, lda, a22, lda, rest2, rest2, nbo, -1.0, 1.0, 0.0, 1, info, s);
     }
   }
+  const int64_t nblk = (n + CH_NB - 1) / CH_NB;
+  NKA_CHECK_ARG(nblk < (1LL << 31), "nka_potrf_f64: too large");
+  hipLaunchKernelGGL(trtri_blocks_kernel, dim3((unsigned)nblk), dim3(64), 0, s, a, lda, n, inv, info);
   NKA_CHECK_LAUNCH("nka_potrf_f64");
   return NKA_OK;
 }
