@@ -254,46 +254,179 @@ __global__ void __launch_bounds__(256) gemm_nt_f64_kernel(const double* a, int64
 }
 
 // ---------------------------------------------------------------------------
+// Large FP64 NT GEMM for the trailing updates: C = beta C + alpha A B^T on a
+// 128x128 tile per workgroup, 4 waves of 64x64 (4x4 v_mfma_f64_16x16x4
+// blocks, 16 accumulators per wave), K step 16.  The next K tile is loaded
+// into registers while the current one is multiplied out of LDS; the LDS
+// tiles are k-major so a fragment read is 16 consecutive rows.  lower_only
+// skips tiles above the diagonal and elements with j > i.  A, B and C must
+// not overlap.
+// ---------------------------------------------------------------------------
+constexpr int G2_T = 128;
+constexpr int G2_K = 16;
+constexpr int G2_LD = G2_T + 4;   // doubles per k row of an LDS tile
+
+template <bool VEC>
+__global__ void __launch_bounds__(256, 2) gemm_nt_f64_big_kernel(const double* __restrict__ a, int64_t lda,
+                                                              const double* __restrict__ b, int64_t ldb,
+                                                              double* __restrict__ c, int64_t ldc, int64_t m,
+                                                              int64_t n, int64_t K, double alpha, double beta,
+                                                              int lower_only, int64_t tiles_n,
+                                                              int64_t n_tiles_total, const int32_t* info) {
+  if (info && *info != 0) return;
+  __shared__ double sA[G2_K * G2_LD];
+  __shared__ double sB[G2_K * G2_LD];
+  const int64_t bid = xcd_remap(blockIdx.x, gridDim.x);
+  if (bid >= n_tiles_total) return;
+  int ti, tj;
+  if (lower_only) {
+    tri_tile(bid, ti, tj);
+  } else {
+    ti = (int)(bid / tiles_n);
+    tj = (int)(bid % tiles_n);
+  }
+  const int64_t i0 = (int64_t)ti * G2_T, j0 = (int64_t)tj * G2_T;
+  const int tid = threadIdx.x, wid = tid / 64, lane = lane_id();
+  const int wm = (wid >> 1) * 64, wn = (wid & 1) * 64;
+  // global -> register mapping: thread t loads row t/2, k = (t%2)*8 .. +8 of
+  // each operand.  Rows past the edge re-read the last row (K is a multiple
+  // of 16, so every load is in bounds); their products are never stored.
+  const int lr = tid >> 1, lk = (tid & 1) * 8;
+  const double* pa = a + min<int64_t>(i0 + lr, m - 1) * lda + lk;
+  const double* pb = b + min<int64_t>(j0 + lr, n - 1) * ldb + lk;
+  double ra[8], rb[8];
+  auto load = [&](int64_t k0) {
+    if (VEC) {
+#pragma unroll
+      for (int q = 0; q < 8; q += 2) {
+        const double2 va = *reinterpret_cast<const double2*>(pa + k0 + q);
+        const double2 vb = *reinterpret_cast<const double2*>(pb + k0 + q);
+        ra[q] = va.x; ra[q + 1] = va.y;
+        rb[q] = vb.x; rb[q + 1] = vb.y;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        ra[q] = pa[k0 + q];
+        rb[q] = pb[k0 + q];
+      }
+    }
+  };
+  doublex4 acc[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) acc[x][y] = (doublex4){0.0, 0.0, 0.0, 0.0};
+  load(0);
+  for (int64_t k0 = 0; k0 < K; k0 += G2_K) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      sA[(lk + q) * G2_LD + lr] = ra[q];
+      sB[(lk + q) * G2_LD + lr] = rb[q];
+    }
+    __syncthreads();
+    if (k0 + G2_K < K) load(k0 + G2_K);
+#pragma unroll
+    for (int ks = 0; ks < G2_K; ks += 4) {
+      const int kq = ks + (lane >> 4);
+      double av[4], bv[4];
+#pragma unroll
+      for (int x = 0; x < 4; ++x) av[x] = sA[kq * G2_LD + wm + x * 16 + (lane & 15)];
+#pragma unroll
+      for (int y = 0; y < 4; ++y) bv[y] = sB[kq * G2_LD + wn + y * 16 + (lane & 15)];
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+          acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x], bv[y], acc[x][y], 0, 0, 0);
+    }
+  }
+  // element e of lane l in a 16x16 block is (row l/16 + 4e, col l%16)
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t gi = i0 + wm + x * 16 + (lane >> 4) + 4 * e;
+        const int64_t gj = j0 + wn + y * 16 + (lane & 15);
+        if (gi < m && gj < n && (!lower_only || gj <= gi)) {
+          double v = alpha * acc[x][y][e];
+          if (beta != 0.0) v += beta * c[gi * ldc + gj];
+          c[gi * ldc + gj] = v;
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
 // Cholesky building blocks (block size 64).
 // ---------------------------------------------------------------------------
 constexpr int CH_NB = 64;
 
 constexpr int CH_LT = CH_NB + 2;  // LDS row stride of the transposed factor (16-byte rows)
-constexpr int TR_ROWS = 256;      // panel rows solved per workgroup (one per thread)
+
+// Lane `src`'s value of v in every lane (src a compile-time constant after
+// unrolling).
+__device__ inline double readlane_f64(double v, int src) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+  return __hiloint2double(hi, lo);
+}
+
+// 1/sqrt(a) from the hardware estimate (about 2^-23 relative error) refined by
+// two Newton-Raphson steps y <- y (3 - a y^2) / 2, each of which squares the
+// relative error.
+__device__ inline double rsqrt_refined(double a) {
+  double y = __builtin_amdgcn_rsq(a);
+  y = y * (1.5 - 0.5 * a * y * y);
+  y = y * (1.5 - 0.5 * a * y * y);
+  return y;
+}
 
 // Factor the diagonal block A[k0:k0+nb, k0:k0+nb] in place (lower triangle)
-// with one wavefront: lane i keeps row i in registers, right-looking; the
-// unscaled column of step j is broadcast through LDS, its element j is the
-// pivot, and the update is a_c -= L_ij L_cj = a_ij a_cj / d^2.  A non-positive
-// or non-finite pivot sets *info (first failing column + 1) and is replaced
-// by 1 so the kernel completes.  Rows/columns past nb are an identity padding.
+// with one wavefront: lane i keeps row i in registers, right-looking.  The
+// pivot of step j comes from lane j by readlane (the short dependency chain);
+// the unscaled column j is broadcast through LDS for the update
+// a_c -= L_ij L_cj = a_ij a_cj / a_jj.  The factor is also written
+// transposed and packed to lt (lt[c*64 + r] = L[r][c]) with the reciprocal
+// diagonal in rd, for the panel solve.  A non-positive or non-finite pivot
+// sets *info (first failing column + 1) and is replaced by 1 so the kernel
+// completes.  Rows/columns past nb are an identity padding.
 __global__ void __launch_bounds__(64) potf2_kernel(double* __restrict__ A, int64_t lda, int64_t n,
-                                                   int64_t k0, int32_t* __restrict__ info) {
+                                                   int64_t k0, double* __restrict__ lt,
+                                                   double* __restrict__ rd, int32_t* __restrict__ info) {
   if (*info != 0) return;
   __shared__ double colbuf[CH_NB];
+  __shared__ double blk[CH_NB * CH_LT];
   const int lane = threadIdx.x;
   const int nb = (int)min<int64_t>(CH_NB, n - k0);
-  const double* arow = A + (k0 + lane) * lda + k0;
+  // stage the (identity-padded) lower triangle through LDS with coalesced reads
+  for (int idx = lane; idx < CH_NB * CH_NB; idx += 64) {
+    const int r = idx >> 6, c = idx & 63;
+    double v = (r == c) ? 1.0 : 0.0;
+    if (r < nb && c < nb && c <= r) v = A[(k0 + r) * lda + k0 + c];
+    blk[r * CH_LT + c] = v;
+  }
+  wave_lds_sync();
   double a[CH_NB];
 #pragma unroll
-  for (int c = 0; c < CH_NB; ++c) {
-    double v = (lane == c) ? 1.0 : 0.0;
-    if (lane < nb && c < nb && c <= lane) v = arow[c];
-    a[c] = v;
-  }
+  for (int c = 0; c < CH_NB; ++c) a[c] = blk[lane * CH_LT + c];
   int bad = 0;
+  double myrd = 1.0;
 #pragma unroll
   for (int j = 0; j < CH_NB; ++j) {
     colbuf[lane] = a[j];
-    wave_lds_sync();
-    const double ajj = colbuf[j];
+    const double ajj = readlane_f64(a[j], j);
     const bool ok = (ajj > 0.0) && isfinite(ajj);
     if (!ok && bad == 0) bad = (int)(k0 + j + 1);
-    const double d = ok ? sqrt(ajj) : 1.0;
-    const double rd = 1.0 / d;
-    const double lij = (lane > j) ? a[j] * rd : (lane == j ? d : 0.0);
-    const double sij = (lane > j) ? lij * rd : 0.0;
+    const double piv = ok ? ajj : 1.0;
+    const double y = rsqrt_refined(piv);
+    const double lij = (lane > j) ? a[j] * y : (lane == j ? piv * y : 0.0);
+    const double sij = (lane > j) ? lij * y : 0.0;
+    if (lane == j) myrd = y;
     a[j] = lij;
+    wave_lds_sync();
 #pragma unroll
     for (int c = j + 1; c < CH_NB; ++c) a[c] -= sij * colbuf[c];
     wave_lds_sync();
@@ -304,42 +437,49 @@ __global__ void __launch_bounds__(64) potf2_kernel(double* __restrict__ A, int64
     for (int c = 0; c < CH_NB; ++c)
       if (c <= lane && c < nb) wrow[c] = a[c];
   }
+#pragma unroll
+  for (int c = 0; c < CH_NB; ++c) lt[c * CH_NB + lane] = (c <= lane) ? a[c] : 0.0;
+  rd[lane] = myrd;
   if (lane == 0 && bad) atomicCAS(info, 0, bad);
 }
 
 // Panel solve L21 = A21 L11^{-T} in place for the rows below the diagonal
-// block k0 (row k0+64+256 b+t in thread t of workgroup b): L11^T is staged in
-// LDS and each thread substitutes its row column by column
-// (x_j = (a_j - sum_{c<j} x_c L_jc) / L_jj).  Rows exist below a block only
-// when the block is a full 64 columns wide, so the row loads are unmasked.
-__global__ void __launch_bounds__(256) trsm_panel_kernel(double* __restrict__ A, int64_t lda, int64_t n,
-                                                         int64_t k0, const int32_t* __restrict__ info) {
+// block k0 (row k0+64+64 b+t in lane t of workgroup b), each lane
+// substituting its row column by column (x_j = (a_j - sum_{c<j} x_c L_jc) / L_jj).
+// L11^T (lt, packed by the factor kernel) is staged in LDS and read at
+// wave-uniform addresses; one wavefront per workgroup keeps the LDS read
+// bandwidth of a CU for that wave.  Rows exist below a block only when the
+// block is a full 64 columns wide, so the row loads are unmasked.
+__global__ void __launch_bounds__(64) trsm_panel_kernel(double* __restrict__ A, int64_t lda, int64_t n,
+                                                        int64_t k0, const double* __restrict__ lt,
+                                                        const double* __restrict__ rd,
+                                                        const int32_t* __restrict__ info) {
   if (*info != 0) return;
-  __shared__ double LT[CH_NB * CH_LT];   // LT[c * CH_LT + r] = L11[r][c]
+  __shared__ double LT[CH_NB * CH_NB];
   __shared__ double rdiag[CH_NB];
-  const int tid = threadIdx.x;
-  for (int idx = tid; idx < CH_NB * CH_NB; idx += 256) {
-    const int r = idx >> 6, c = idx & 63;
-    LT[c * CH_LT + r] = (c <= r) ? A[(k0 + r) * lda + k0 + c] : 0.0;
-  }
-  __syncthreads();
-  if (tid < CH_NB) rdiag[tid] = 1.0 / LT[tid * CH_LT + tid];
-  __syncthreads();
-  const int64_t row = k0 + CH_NB + (int64_t)blockIdx.x * TR_ROWS + tid;
-  if (row >= n) return;
-  double* ar = A + row * lda + k0;
+  const int lane = threadIdx.x;
+#pragma unroll 4
+  for (int q = lane; q < CH_NB * CH_NB / 2; q += 64)
+    reinterpret_cast<double2*>(LT)[q] = reinterpret_cast<const double2*>(lt)[q];
+  rdiag[lane] = rd[lane];
+  const int64_t row = k0 + CH_NB + (int64_t)blockIdx.x * CH_NB + lane;
+  const bool live = row < n;
+  double* ar = A + (live ? row : k0 + CH_NB) * lda + k0;
   double x[CH_NB];
 #pragma unroll
   for (int c = 0; c < CH_NB; ++c) x[c] = ar[c];
+  wave_lds_sync();
 #pragma unroll
   for (int j = 0; j < CH_NB; ++j) {
     const double xj = x[j] * rdiag[j];
     x[j] = xj;
 #pragma unroll
-    for (int c = j + 1; c < CH_NB; ++c) x[c] -= xj * LT[j * CH_LT + c];
+    for (int c = j + 1; c < CH_NB; ++c) x[c] -= xj * LT[j * CH_NB + c];
   }
+  if (live) {
 #pragma unroll
-  for (int c = 0; c < CH_NB; ++c) ar[c] = x[c];
+    for (int c = 0; c < CH_NB; ++c) ar[c] = x[c];
+  }
 }
 
 // inv_b = L_bb^{-1} for every 64x64 diagonal block b of the factor (row-major
@@ -723,6 +863,27 @@ static int gemm_launch(const double* a, int64_t lda, const double* b, int64_t ld
   return NKA_OK;
 }
 
+// The 128-tile kernel needs K to be a multiple of its K step; other shapes
+// take the 64-tile kernel.
+static void gemm_big_launch(const double* a, int64_t lda, const double* b, int64_t ldb, double* c,
+                            int64_t ldc, int64_t m, int64_t n, int64_t k, double alpha, double beta,
+                            int lower, const int32_t* info, hipStream_t s) {
+  if (m <= 0 || n <= 0) return;
+  if (k % G2_K != 0) {
+    gemm_launch(a, lda, b, ldb, c, ldc, m, n, k, alpha, beta, 0.0, lower, info, s);
+    return;
+  }
+  const int64_t tm = (m + G2_T - 1) / G2_T, tn = (n + G2_T - 1) / G2_T;
+  const int64_t tiles = lower ? tm * (tm + 1) / 2 : tm * tn;
+  const bool vec = ((uintptr_t)a % 16 == 0) && ((uintptr_t)b % 16 == 0) && lda % 2 == 0 && ldb % 2 == 0;
+  if (vec)
+    hipLaunchKernelGGL(gemm_nt_f64_big_kernel<true>, dim3(grid_mult8(tiles)), dim3(256), 0, s, a, lda, b,
+                       ldb, c, ldc, m, n, k, alpha, beta, lower, tn, tiles, info);
+  else
+    hipLaunchKernelGGL(gemm_nt_f64_big_kernel<false>, dim3(grid_mult8(tiles)), dim3(256), 0, s, a, lda, b,
+                       ldb, c, ldc, m, n, k, alpha, beta, lower, tn, tiles, info);
+}
+
 int nka_syrk_f64(const double* x, int64_t n, int64_t k, int64_t ldx, double alpha, double beta,
                  double shift, double* c, int64_t ldc, void* stream) {
   NKA_CHECK_ARG(n >= 0 && k >= 0 && ldx >= k && ldc >= n, "nka_syrk_f64: bad sizes");
@@ -771,15 +932,21 @@ int nka_potrf_f64(double* a, int64_t n, int64_t lda, void* workspace, int32_t* i
   if (n == 0) return NKA_OK;
   hipStream_t s = (hipStream_t)stream;
   double* inv = ws_inv(workspace);
+  const int64_t nblk = (n + CH_NB - 1) / CH_NB;
+  NKA_CHECK_ARG(nblk < (1LL << 31), "nka_potrf_f64: too large");
   for (int64_t K0 = 0; K0 < n; K0 += CH_NBO) {
     const int64_t nbo = std::min<int64_t>(CH_NBO, n - K0);
     for (int64_t k0 = K0; k0 < K0 + nbo; k0 += CH_NB) {
       const int64_t nb = std::min<int64_t>(CH_NB, n - k0);
       const int64_t rest = n - k0 - nb;
-      hipLaunchKernelGGL(potf2_kernel, dim3(1), dim3(64), 0, s, a, lda, n, k0, info);
+      // the block's inverse slot and the vector area serve as scratch for
+      // L11^T and 1/diag until the inverses are formed at the end
+      double* lt = inv + (k0 / CH_NB) * CH_NB * CH_NB;
+      double* rd = ws_vec(workspace, nblk);
+      hipLaunchKernelGGL(potf2_kernel, dim3(1), dim3(64), 0, s, a, lda, n, k0, lt, rd, info);
       if (rest <= 0) continue;  // rows below imply nb == 64
-      const unsigned grid = (unsigned)((rest + TR_ROWS - 1) / TR_ROWS);
-      hipLaunchKernelGGL(trsm_panel_kernel, dim3(grid), dim3(256), 0, s, a, lda, n, k0, info);
+      const unsigned grid = (unsigned)((rest + CH_NB - 1) / CH_NB);
+      hipLaunchKernelGGL(trsm_panel_kernel, dim3(grid), dim3(64), 0, s, a, lda, n, k0, lt, rd, info);
       // update of the remaining columns of this panel
       const int64_t cols = K0 + nbo - (k0 + nb);
       if (cols > 0) {
@@ -792,11 +959,9 @@ int nka_potrf_f64(double* a, int64_t n, int64_t lda, void* workspace, int32_t* i
     if (rest2 > 0) {
       const double* l21 = a + (K0 + nbo) * lda + K0;
       double* a22 = a + (K0 + nbo) * lda + (K0 + nbo);
-      gemm_launch(l21, lda, l21, lda, a22, lda, rest2, rest2, nbo, -1.0, 1.0, 0.0, 1, info, s);
+      gemm_big_launch(l21, lda, l21, lda, a22, lda, rest2, rest2, nbo, -1.0, 1.0, 1, info, s);
     }
   }
-  const int64_t nblk = (n + CH_NB - 1) / CH_NB;
-  NKA_CHECK_ARG(nblk < (1LL << 31), "nka_potrf_f64: too large");
   hipLaunchKernelGGL(trtri_blocks_kernel, dim3((unsigned)nblk), dim3(64), 0, s, a, lda, n, inv, info);
   NKA_CHECK_LAUNCH("nka_potrf_f64");
   return NKA_OK;
