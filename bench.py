@@ -163,6 +163,32 @@ def roofline_for(stage: str, mean_ms: float):
     return None
 
 
+def copy_calibration(nbytes: int = 1 << 30, reps: int = 3):
+    """A plain device-to-device copy of nbytes (reads nbytes, writes nbytes),
+    the reference for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    src = torch.empty(nbytes, dtype=torch.uint8, device="cuda").fill_(1)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps}))
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of ``kernel`` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy calibration), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        return data["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +197,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=25.0)
+    ap.add_argument("--no-graph", action="store_true", help="skip the captured-graph replays")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -182,6 +211,9 @@ def main():
 
     from netket_amd import _ops
 
+    if args.copy_calibration:
+        copy_calibration()
+        return
     drv, H, vs = build_cfg2()
     for _ in range(args.warmup):
         step(drv)
@@ -197,19 +229,26 @@ def main():
     stages = _ops.TIMER.summary()
     dt_eager = t1 - t0
     # the same step captured once into a HIP graph and replayed
-    drv.capture_step(warmup=1)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    stats_g = [drv.replay_step() for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    dt_graph = time.perf_counter() - t0
-    stats = stats + stats_g
+    dt_graph = float("inf")
+    if not args.no_graph:
+        drv.capture_step(warmup=1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stats_g = [drv.replay_step() for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        dt_graph = time.perf_counter() - t0
+        stats = stats + stats_g
     dt = min(dt_eager, dt_graph)
     sps = args.steps / dt
     gpu_ms_total = sum(v["total_ms"] for v in stages.values()) / args.steps
     energies = [s.mean for s in stats]
     dom = max(stages.items(), key=lambda kv: kv[1]["total_ms"])
-    roof = roofline_for(dom[0], dom[1]["mean_ms"])
+    # roofline of the largest single kernel (the S Gram, one launch per step);
+    # the other stages are listed beside it
+    roof = roofline_for("gram_bf16x3", stages["gram_bf16x3"]["mean_ms"])
+    roof["traffic"] = measured_traffic("gram_bf16x3_kernel")
+    others = {k: roofline_for(k, stages[k]["mean_ms"]) for k in ("potrf", "metropolis", "local_energy")
+              if k in stages}
     acc = None if args.no_accuracy else accuracy_run()
     cpu = None
     if not args.no_cpu_baseline:
@@ -229,7 +268,7 @@ def main():
         "warmup": args.warmup,
         "ms_per_step": dt / args.steps * 1e3,
         "ms_per_step_eager": dt_eager / args.steps * 1e3,
-        "ms_per_step_graph": dt_graph / args.steps * 1e3,
+        "ms_per_step_graph": (dt_graph / args.steps * 1e3) if math.isfinite(dt_graph) else None,
         "samples_per_s": sps * 16384,
         "higher_is_better": True,
         "scaling": "weak",
@@ -245,6 +284,7 @@ def main():
         "gpu_busy_ms_per_step": gpu_ms_total,
         "dominant_stage": dom[0],
         "roofline": roof,
+        "roofline_other_stages": others,
         "cpu_baseline": cpu,
         "accuracy": acc,
         "energy_first_last": [energies[0], energies[-1]],

@@ -159,6 +159,42 @@ template <> __device__ inline double uniform_value<double>(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// Lower-triangular tile pair (ti >= tj) of linear id b in an order made of
+// SxS super-blocks of tiles (row-major over the super-blocks of the lower
+// triangle, then row-major inside each; diagonal super-blocks are
+// triangular).  T is the number of tile rows.  Tiles that run at the same time
+// on one XCD then share a few operand panels (about S rows + S columns)
+// instead of one row panel and a different column panel each.
+__device__ inline void tri_supertile(int64_t b, int64_t T, int& ti, int& tj) {
+  constexpr int64_t S = 8;
+  const int64_t SB = (T + S - 1) / S;
+  int64_t rem = b;
+  for (int64_t I = 0; I < SB; ++I) {
+    const int64_t rows = min<int64_t>(S, T - I * S);
+    for (int64_t J = 0; J <= I; ++J) {
+      const int64_t cols = min<int64_t>(S, T - J * S);
+      const int64_t cnt = (I == J) ? rows * (rows + 1) / 2 : rows * cols;
+      if (rem < cnt) {
+        int64_t r, c;
+        if (I == J) {
+          r = (int64_t)((sqrt(8.0 * (double)rem + 1.0) - 1.0) * 0.5);
+          while ((r + 1) * (r + 2) / 2 <= rem) ++r;
+          while (r * (r + 1) / 2 > rem) --r;
+          c = rem - r * (r + 1) / 2;
+        } else {
+          r = rem / cols;
+          c = rem % cols;
+        }
+        ti = (int)(I * S + r);
+        tj = (int)(J * S + c);
+        return;
+      }
+      rem -= cnt;
+    }
+  }
+  ti = tj = 0;  // not reached for b < T (T + 1) / 2
+}
+
 }  // namespace nka
 
 // ---------------------------------------------------------------------------

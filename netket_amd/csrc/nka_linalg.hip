@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(256) gram_bf16x3_kernel(const uint16_t* __rest
   const int64_t bid = xcd_remap(blockIdx.x, gridDim.x);
   if (bid >= n_tiles_total) return;
   int ti, tj;
-  tri_tile(bid, ti, tj);
+  tri_supertile(bid, (n + GB_T - 1) / GB_T, ti, tj);
   const int64_t i0 = (int64_t)ti * GB_T, j0 = (int64_t)tj * GB_T;
   const int tid = threadIdx.x, wid = tid / 64, lane = lane_id();
   const int wm = (wid >> 1) * 64, wn = (wid & 1) * 64;  // wave tile origin in block tile
@@ -280,7 +280,7 @@ __global__ void __launch_bounds__(256, 2) gemm_nt_f64_big_kernel(const double* _
   if (bid >= n_tiles_total) return;
   int ti, tj;
   if (lower_only) {
-    tri_tile(bid, ti, tj);
+    tri_supertile(bid, tiles_n, ti, tj);
   } else {
     ti = (int)(bid / tiles_n);
     tj = (int)(bid % tiles_n);

@@ -55,14 +55,6 @@ __global__ void __launch_bounds__(256) ntk_vec_kernel(const double* __restrict__
   }
 }
 
-__device__ inline void ntk_tri_tile(int64_t b, int& ti, int& tj) {
-  int64_t i = (int64_t)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
-  while ((i + 1) * (i + 2) / 2 <= b) ++i;
-  while (i * (i + 1) / 2 > b) --i;
-  ti = (int)i;
-  tj = (int)(b - i * (i + 1) / 2);
-}
-
 constexpr int NT_T = 64;
 constexpr int NT_K = 16;
 constexpr int NT_LD = 17;
@@ -81,7 +73,7 @@ __global__ void __launch_bounds__(256) rbm_ntk_kernel(const double* __restrict__
   const int64_t bid = (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8;  // XCD-aware order
   if (bid >= n_tiles) return;
   int ti, tj;
-  ntk_tri_tile(bid, ti, tj);
+  tri_supertile(bid, (ns + NT_T - 1) / NT_T, ti, tj);
   const int64_t i0 = (int64_t)ti * NT_T, j0 = (int64_t)tj * NT_T;
   const int tid = threadIdx.x, wid = tid / 64, lane = lane_id();
   const int wm = (wid >> 1) * 32, wn = (wid & 1) * 32;
