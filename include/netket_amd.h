@@ -122,6 +122,80 @@ int nka_rbm_local_energy(const nka_rbm* m, int op_kind, const void* op, const in
                          int64_t n, double* eloc, void* log_psi, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Two-layer dense network: log psi(x) = sum_k log_cosh(z2_k) with
+ * z2 = log_cosh(x W1 + b1) W2 + b2 (Dense -> log_cosh -> Dense -> log_cosh ->
+ * sum: nn.Dense of netket/nn/blocks/mlp.py:105-115, log_cosh of
+ * netket/nn/activation.py:78-84; the user module of the J1-J2 workload).
+ * Flat parameters in ravel order Dense_0/bias (H1), Dense_0/kernel (N, H1),
+ * Dense_1/bias (H2), Dense_1/kernel (H1, H2); a bias offset is -1 when the
+ * layer has no bias.  n_hidden1, n_hidden2 <= 1024.
+ * ---------------------------------------------------------------------- */
+typedef struct nka_ffnn {
+  int n_visible;
+  int n_hidden1;
+  int n_hidden2;
+  int dtype;
+  const void* params;
+  int64_t b1_off;
+  int64_t w1_off;
+  int64_t b2_off;
+  int64_t w2_off;
+} nka_ffnn;
+
+/* log psi of n configurations (model dtype). */
+int nka_ffnn_log_psi(const nka_ffnn* m, const int8_t* sigma, int64_t n, void* log_psi, void* stream);
+
+/* local_value_kernel_jax (netket/vqs/mc/kernels.py:62-71) with
+ * get_conn_padded of an Ising or LocalOperator (op_kind as for
+ * nka_rbm_local_energy): eloc[s] = sum_k mel_k exp(log psi(x'_k) - log psi(x_s))
+ * (FP64).  The connected configurations are formed in chunks in the
+ * workspace (nka_ffnn_local_energy_workspace bytes suffice; a smaller
+ * workspace gives smaller chunks); the first layer of x'_k is updated from
+ * x_s by the sites where they differ.  log_psi (model dtype) may be NULL. */
+int64_t nka_ffnn_local_energy_workspace(const nka_ffnn* m, int op_kind, const void* op, int64_t n);
+int nka_ffnn_local_energy(const nka_ffnn* m, int op_kind, const void* op, const int8_t* sigma, int64_t n,
+                          double* eloc, void* log_psi, void* workspace, int64_t workspace_bytes,
+                          void* stream);
+
+/* Metropolis sweeps for the network, with the arguments and semantics of
+ * nka_rbm_metropolis (netket/sampler/metropolis.py:382-505). */
+int nka_ffnn_metropolis(const nka_ffnn* m, int rule, const int32_t* clusters, int n_clusters,
+                        int8_t* sigma, void* log_prob, int n_chains, int sweep_size, int n_discard,
+                        int chain_length, double machine_pow, uint64_t seed, uint64_t counter,
+                        uint64_t* counter_dev, int8_t* samples, void* log_prob_samples,
+                        int64_t* n_accepted, void* stream);
+
+/* Per-sample factors of the log-derivatives (model dtype): h1 = log_cosh(theta1)
+ * (ns, H1), d1 = d log psi / d theta1 = (W2 tanh(z2)) * tanh(theta1) (ns, H1),
+ * g2 = tanh(z2) (ns, H2); the Jacobian is O = [d1 | x (x) d1 | g2 | h1 (x) g2]
+ * in ravel order (nkjax.jacobian, netket/jax/_jacobian/logic.py:308-387).
+ * workspace holds nka_ffnn_prepare_workspace(...) bytes. */
+int64_t nka_ffnn_prepare_workspace(const nka_ffnn* m, int64_t ns);
+int nka_ffnn_prepare(const nka_ffnn* m, const int8_t* sigma, int64_t ns, void* h1, void* d1, void* g2,
+                     void* workspace, void* stream);
+
+/* out[v, p] = sum_s (O[s, p] - mean[p]) w[v, s] (FP64, n_w <= 2), as
+ * nka_rbm_contract, from the factors of nka_ffnn_prepare. */
+int64_t nka_ffnn_contract_workspace(const nka_ffnn* m, int64_t ns, int n_w);
+int nka_ffnn_contract(const nka_ffnn* m, const int8_t* sigma, const void* h1, const void* d1,
+                      const void* g2, int64_t ns, const double* w, int n_w, const double* mean,
+                      double* out, void* workspace, void* stream);
+
+/* Centred NTK without the Jacobian, as nka_rbm_ntk:
+ * (O O^T)_st = (hb2 + h1_s.h1_t)(g2_s.g2_t) + (hb1 + x_s.x_t)(d1_s.d1_t), centred
+ * with v = O mean and c = |mean|^2; out = scale * O_c O_c^T + shift I + offset 11^T
+ * (lower triangle, FP64; netket/_src/ngd/srt.py:55-66 with scale = 1/ns). */
+int64_t nka_ffnn_ntk_workspace(const nka_ffnn* m, int64_t ns);
+int nka_ffnn_ntk(const nka_ffnn* m, const int8_t* sigma, const void* h1, const void* d1, const void* g2,
+                 int64_t ns, const double* mean, double scale, double shift, double offset, double* out,
+                 int64_t ldo, void* workspace, void* stream);
+
+/* Centred, rescaled Jacobian (layouts and output kinds of nka_rbm_jacobian). */
+int nka_ffnn_jacobian(const nka_ffnn* m, const int8_t* sigma, const void* h1, const void* d1,
+                      const void* g2, int64_t ns, const double* mean, double scale, int layout,
+                      int out_kind, void* out_a, void* out_b, int64_t ldo, void* stream);
+
+/* ------------------------------------------------------------------------
  * Sampler.
  * ---------------------------------------------------------------------- */
 

@@ -22,15 +22,42 @@ def local_energies(model, flat, op, sigma2d: torch.Tensor, log_psi: bool = False
     kind, op_ptr, keep = _ops.op_struct(op)
     m = model.native(flat, N)
     with _ops.TIMER.region("local_energy"):
-        nat.call("nka_rbm_local_energy", C.byref(m), kind, op_ptr, _ops.ptr(sigma2d), n,
-                 _ops.ptr(eloc), _ops.ptr(lp), _ops.stream())
+        if model.kind == "ffnn":
+            wsb = nat.lib().nka_ffnn_local_energy_workspace(C.byref(m), kind, op_ptr, n)
+            ws = _ops.WORKSPACE.get("ffnn_eloc", wsb)
+            nat.call("nka_ffnn_local_energy", C.byref(m), kind, op_ptr, _ops.ptr(sigma2d), n,
+                     _ops.ptr(eloc), _ops.ptr(lp), _ops.ptr(ws), ws.numel(), _ops.stream())
+        else:
+            nat.call("nka_rbm_local_energy", C.byref(m), kind, op_ptr, _ops.ptr(sigma2d), n,
+                     _ops.ptr(eloc), _ops.ptr(lp), _ops.stream())
     del keep
     return (eloc, lp) if log_psi else eloc
 
 
+class FfnnFactors:
+    """Per-sample factors of the network's log-derivatives (model dtype):
+    h1 = log_cosh(theta1), d1 = d log psi / d theta1, g2 = tanh(z2)."""
+
+    def __init__(self, h1, d1, g2):
+        self.h1, self.d1, self.g2 = h1, d1, g2
+
+
 def prepare(model, flat, sigma2d: torch.Tensor, row_major: bool = True, transposed: bool = False):
-    """tanh(theta) per sample: (ns, M) and/or (M, ns) in the model dtype."""
+    """Per-sample factors of the log-derivatives.  RBM: tanh(theta) as (ns, M)
+    and/or (M, ns) in the model dtype; FFNN: (FfnnFactors, None)."""
     n, N = sigma2d.shape
+    if model.kind == "ffnn":
+        H1, H2 = model.hidden_dims
+        dev = sigma2d.device
+        h1 = torch.empty((n, H1), dtype=flat.dtype, device=dev)
+        d1 = torch.empty((n, H1), dtype=flat.dtype, device=dev)
+        g2 = torch.empty((n, H2), dtype=flat.dtype, device=dev)
+        m = model.native(flat, N)
+        ws = _ops.WORKSPACE.get("ffnn_prepare", nat.lib().nka_ffnn_prepare_workspace(C.byref(m), n))
+        with _ops.TIMER.region("prepare"):
+            nat.call("nka_ffnn_prepare", C.byref(m), _ops.ptr(sigma2d), n, _ops.ptr(h1), _ops.ptr(d1),
+                     _ops.ptr(g2), _ops.ptr(ws), _ops.stream())
+        return FfnnFactors(h1, d1, g2), None
     M = model.n_hidden(N)
     t = torch.empty((n, M), dtype=flat.dtype, device=sigma2d.device) if row_major else None
     tT = torch.empty((M, n), dtype=flat.dtype, device=sigma2d.device) if transposed else None
@@ -50,6 +77,14 @@ def contract(model, flat, sigma2d: torch.Tensor, t: torch.Tensor, w: torch.Tenso
     m = model.native(flat, N)
     np_ = model.layout(N)["n_params"]
     out = torch.empty((n_w, np_), dtype=torch.float64, device=sigma2d.device)
+    if model.kind == "ffnn":
+        f = t
+        ws = _ops.WORKSPACE.get("ffnn_contract", nat.lib().nka_ffnn_contract_workspace(C.byref(m), n, n_w))
+        with _ops.TIMER.region("contract"):
+            nat.call("nka_ffnn_contract", C.byref(m), _ops.ptr(sigma2d), _ops.ptr(f.h1), _ops.ptr(f.d1),
+                     _ops.ptr(f.g2), n, _ops.ptr(w), n_w, _ops.ptr(mean), _ops.ptr(out), _ops.ptr(ws),
+                     _ops.stream())
+        return out
     ws_bytes = nat.lib().nka_rbm_contract_workspace(C.byref(m), n, n_w)
     ws = _ops.WORKSPACE.get("contract", ws_bytes)
     with _ops.TIMER.region("contract"):
@@ -92,6 +127,13 @@ def jacobian(model, flat, sigma2d, t, tT, mean, scale: float, layout: int, kind:
     if bufs is None or not bufs.matches(rows, cols, kind):
         bufs = JacobianBuffers(rows, cols, kind, sigma2d.device)
     m = model.native(flat, N)
+    if model.kind == "ffnn":
+        f = t
+        with _ops.TIMER.region("jacobian"):
+            nat.call("nka_ffnn_jacobian", C.byref(m), _ops.ptr(sigma2d), _ops.ptr(f.h1), _ops.ptr(f.d1),
+                     _ops.ptr(f.g2), n, _ops.ptr(mean), float(scale), layout, kind, _ops.ptr(bufs.hi),
+                     _ops.ptr(bufs.lo), bufs.ld, _ops.stream())
+        return bufs
     with _ops.TIMER.region("jacobian"):
         nat.call("nka_rbm_jacobian", C.byref(m), _ops.ptr(sigma2d), _ops.ptr(t), _ops.ptr(tT), n,
                  _ops.ptr(mean), float(scale), layout, kind, _ops.ptr(bufs.hi),
@@ -120,20 +162,31 @@ def gram(bufs: JacobianBuffers, shift: float, offset: float = 0.0,
     return out
 
 
-def rbm_ntk(model, flat, sigma2d: torch.Tensor, t: torch.Tensor, mean: torch.Tensor, scale: float,
-            shift: float, offset: float = 0.0, out: torch.Tensor | None = None) -> torch.Tensor:
+def ntk_structured(model, flat, sigma2d: torch.Tensor, t, mean: torch.Tensor, scale: float,
+                   shift: float, offset: float = 0.0, out: torch.Tensor | None = None) -> torch.Tensor:
     """scale (O - mean)(O - mean)^T + shift I + offset 11^T on the lower triangle
-    (FP64) from the RBM's structure, without a Jacobian."""
+    (FP64) from the model's layer structure, without a Jacobian."""
     n, N = sigma2d.shape
     if out is None or out.shape != (n, n):
         out = torch.empty((n, n), dtype=torch.float64, device=sigma2d.device)
     m = model.native(flat, N)
+    if model.kind == "ffnn":
+        f = t
+        ws = _ops.WORKSPACE.get("ffnn_ntk", nat.lib().nka_ffnn_ntk_workspace(C.byref(m), n))
+        with _ops.TIMER.region("ntk_structured"):
+            nat.call("nka_ffnn_ntk", C.byref(m), _ops.ptr(sigma2d), _ops.ptr(f.h1), _ops.ptr(f.d1),
+                     _ops.ptr(f.g2), n, _ops.ptr(mean), float(scale), float(shift), float(offset),
+                     _ops.ptr(out), out.stride(0), _ops.ptr(ws), _ops.stream())
+        return out
     ws = _ops.WORKSPACE.get("rbm_ntk", nat.lib().nka_rbm_ntk_workspace(C.byref(m), n))
     with _ops.TIMER.region("ntk_structured"):
         nat.call("nka_rbm_ntk", C.byref(m), _ops.ptr(sigma2d), _ops.ptr(t), n, _ops.ptr(mean),
                  float(scale), float(shift), float(offset), _ops.ptr(out), out.stride(0),
                  _ops.ptr(ws), _ops.stream())
     return out
+
+
+rbm_ntk = ntk_structured
 
 
 def centre_scale(e: torch.Tensor, mean_dev: torch.Tensor, scale: float) -> torch.Tensor:

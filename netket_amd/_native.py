@@ -58,6 +58,20 @@ class NkaRbm(C.Structure):
     ]
 
 
+class NkaFfnn(C.Structure):
+    _fields_ = [
+        ("n_visible", C.c_int),
+        ("n_hidden1", C.c_int),
+        ("n_hidden2", C.c_int),
+        ("dtype", C.c_int),
+        ("params", C.c_void_p),
+        ("b1_off", C.c_int64),
+        ("w1_off", C.c_int64),
+        ("b2_off", C.c_int64),
+        ("w2_off", C.c_int64),
+    ]
+
+
 class NkaIsing(C.Structure):
     _fields_ = [
         ("edges", C.c_void_p),
@@ -81,6 +95,19 @@ SIGNATURES = {
     "nka_ising_conn": (i32, [vp, i64, i32, vp, i32, f64, f64, i32, i32, vp, vp, vp]),
     "nka_localop_conn": (i32, [vp, i64, i32, C.POINTER(NkaLocalOp), vp, vp, vp, vp]),
     "nka_rbm_log_psi": (i32, [C.POINTER(NkaRbm), vp, i64, vp, vp]),
+    "nka_ffnn_log_psi": (i32, [C.POINTER(NkaFfnn), vp, i64, vp, vp]),
+    "nka_ffnn_local_energy_workspace": (i64, [C.POINTER(NkaFfnn), i32, vp, i64]),
+    "nka_ffnn_local_energy": (i32, [C.POINTER(NkaFfnn), i32, vp, vp, i64, vp, vp, vp, i64, vp]),
+    "nka_ffnn_metropolis": (i32, [C.POINTER(NkaFfnn), i32, vp, i32, vp, vp, i32, i32, i32, i32, f64,
+                                  C.c_uint64, C.c_uint64, vp, vp, vp, vp, vp]),
+    "nka_ffnn_prepare_workspace": (i64, [C.POINTER(NkaFfnn), i64]),
+    "nka_ffnn_prepare": (i32, [C.POINTER(NkaFfnn), vp, i64, vp, vp, vp, vp, vp]),
+    "nka_ffnn_contract_workspace": (i64, [C.POINTER(NkaFfnn), i64, i32]),
+    "nka_ffnn_contract": (i32, [C.POINTER(NkaFfnn), vp, vp, vp, vp, i64, vp, i32, vp, vp, vp, vp]),
+    "nka_ffnn_ntk_workspace": (i64, [C.POINTER(NkaFfnn), i64]),
+    "nka_ffnn_ntk": (i32, [C.POINTER(NkaFfnn), vp, vp, vp, vp, i64, vp, f64, f64, f64, vp, i64, vp, vp]),
+    "nka_ffnn_jacobian": (i32, [C.POINTER(NkaFfnn), vp, vp, vp, vp, i64, vp, f64, i32, i32, vp, vp, i64,
+                                vp]),
     "nka_rbm_local_energy": (i32, [C.POINTER(NkaRbm), i32, vp, vp, i64, vp, vp, vp]),
     "nka_random_states": (i32, [vp, i32, i32, i32, u64, u64, vp]),
     "nka_rbm_metropolis": (

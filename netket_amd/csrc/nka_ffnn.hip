@@ -1,0 +1,1159 @@
+// Two-layer dense network ansatz on gfx950:
+//   log psi(x) = sum_k log_cosh(z2_k),  z2 = log_cosh(x W1 + b1) W2 + b2
+// (Dense -> log_cosh -> Dense -> log_cosh -> sum; nn.Dense of
+// netket/nn/blocks/mlp.py:105-115, log_cosh of netket/nn/activation.py:78-84).
+//
+// Every evaluation works on tiles of 32 configurations per workgroup: the
+// hidden layer h1 of the tile sits in LDS as the A operand of the second
+// layer, whose W2 operand streams from L2 (each element read once per
+// tile).  FP32 models use v_mfma_f32_32x32x2f32 for that product; FP64
+// models use plain FMAs.  Row sums are reduced in a fixed order (butterfly
+// shuffles, then per-block partials summed in order), so results do not
+// depend on scheduling.
+//
+// Callers of the tile kernel: log psi, the local energies (connected
+// configurations from the conn kernels, first layer updated from the
+// sample's theta1 by the differing sites), the per-sample quantities of the
+// Jacobian (h1, tanh z2, back-propagated d1) and the Metropolis sampler
+// (32 chains per workgroup stepping in lock step).
+#include "nka_common.hpp"
+#include "netket_amd.h"
+
+extern "C" int nka_ising_conn(const int8_t* x, int64_t n_batch, int n_sites, const int32_t* edges,
+                              int n_edges, double h, double J, int mel_dtype, int n_rows, int8_t* xp,
+                              void* mels, void* stream);
+extern "C" int nka_localop_conn(const int8_t* x, int64_t n_batch, int n_sites, const nka_localop* op,
+                                int8_t* xp, void* mels, int32_t* n_conn, void* stream);
+extern "C" int nka_gemm_nt_f64(const double* a, int64_t lda, const double* b, int64_t ldb, double* c,
+                               int64_t ldc, int64_t m, int64_t n, int64_t k, double alpha, double beta,
+                               void* stream);
+
+namespace nka {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+constexpr int FF_ROWS = 32;      // configurations (or chains) per workgroup
+constexpr int FF_THREADS = 256;
+constexpr int FF_GROUPS = FF_THREADS / FF_ROWS;  // threads per row in the first layer
+constexpr int FF_HPT = 128;      // hidden units per thread: H1 <= FF_GROUPS * FF_HPT
+constexpr int FF_MAXH = FF_GROUPS * FF_HPT;      // 1024
+constexpr int FF_MAXBLK = FF_MAXH / 32;          // column blocks of the second layer
+
+template <class T>
+struct FfnnView {
+  int N, H1, H2;
+  const T* b1;
+  const T* W1;
+  const T* b2;
+  const T* W2;
+};
+
+template <class T>
+__host__ __device__ inline FfnnView<T> ffnn_view(const nka_ffnn& m) {
+  const T* p = (const T*)m.params;
+  FfnnView<T> v;
+  v.N = m.n_visible;
+  v.H1 = m.n_hidden1;
+  v.H2 = m.n_hidden2;
+  v.b1 = m.b1_off >= 0 ? p + m.b1_off : nullptr;
+  v.W1 = p + m.w1_off;
+  v.b2 = m.b2_off >= 0 ? p + m.b2_off : nullptr;
+  v.W2 = p + m.w2_off;
+  return v;
+}
+
+__host__ __device__ inline int lda_of(int H1) { return H1 + ((H1 & 1) ? 2 : 1); }  // odd, >= H1 + 1
+__host__ __device__ inline int64_t ffpad(int64_t b) { return (b + 15) & ~(int64_t)15; }
+constexpr int64_t FF_LDS_MAX = 160 * 1024;
+
+// ---------------------------------------------------------------------------
+// Second layer over a 32-row tile: z[r][c] = sum_k A[r][k] B(k, c) for the
+// rows of A (LDS, row stride lda, K columns) and ncols columns; epi(r, c, z)
+// is applied to every element and its values are summed per row into
+// rowsum[r] (0 <= r < 32) in a fixed order.  part is LDS scratch of
+// 32 * FF_MAXBLK values.  All threads of the workgroup must call it.
+// ---------------------------------------------------------------------------
+template <class BLoad, class Epi>
+__device__ void rows_gemm(const float* As, int lda, int K, int ncols, BLoad bload, Epi epi,
+                          float* part, float* rowsum) {
+  const int wid = threadIdx.x >> 6, lane = lane_id();
+  const int nblk = (ncols + 31) >> 5;
+  const int kh = lane >> 5, lc = lane & 31;
+  for (int cb = wid; cb < nblk; cb += 4) {
+    floatx16 acc;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+    const int col = cb * 32 + lc;
+    const bool cok = col < ncols;
+    int k = 0;
+    for (; k + 8 <= K; k += 8) {
+      float a[4], b[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int kk = k + 2 * q + kh;
+        a[q] = As[lc * lda + kk];
+        b[q] = cok ? bload(kk, col) : 0.0f;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q], b[q], acc, 0, 0, 0);
+    }
+    for (; k < K; k += 2) {
+      const int kk = k + kh;
+      const float a = kk < K ? As[lc * lda + kk] : 0.0f;
+      const float b = (kk < K && cok) ? bload(kk, col) : 0.0f;
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+    }
+    // element e of lane l is (row (e/4)*8 + (l/32)*4 + e%4, col l%32)
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int row = (e >> 2) * 8 + kh * 4 + (e & 3);
+      float v = cok ? epi(row, col, acc[e]) : 0.0f;
+#pragma unroll
+      for (int off = 16; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+      if (lc == 0) part[row * FF_MAXBLK + cb] = v;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < FF_ROWS) {
+    float s = 0.0f;
+    for (int cb = 0; cb < nblk; ++cb) s += part[threadIdx.x * FF_MAXBLK + cb];
+    rowsum[threadIdx.x] = s;
+  }
+  __syncthreads();
+}
+
+template <class BLoad, class Epi>
+__device__ void rows_gemm(const double* As, int lda, int K, int ncols, BLoad bload, Epi epi,
+                          double* part, double* rowsum) {
+  const int r = threadIdx.x & 31, g = threadIdx.x >> 5;
+  double s = 0.0;
+  for (int c = g; c < ncols; c += FF_GROUPS) {
+    double z = 0.0;
+    for (int k = 0; k < K; ++k) z += As[r * lda + k] * bload(k, c);
+    s += epi(r, c, z);
+  }
+  part[r * FF_MAXBLK + g] = s;
+  __syncthreads();
+  if (threadIdx.x < FF_ROWS) {
+    double t = 0.0;
+    for (int q = 0; q < FF_GROUPS; ++q) t += part[threadIdx.x * FF_MAXBLK + q];
+    rowsum[threadIdx.x] = t;
+  }
+  __syncthreads();
+}
+
+// theta1 of unit a for a configuration given as int8 spins in LDS
+template <class T>
+__device__ inline T theta1_full(const FfnnView<T>& v, const int8_t* x, int a) {
+  T t = v.b1 ? v.b1[a] : (T)0;
+  for (int i = 0; i < v.N; ++i) t += (x[i] > 0 ? (T)1 : (T)-1) * v.W1[(int64_t)i * v.H1 + a];
+  return t;
+}
+
+// ---------------------------------------------------------------------------
+// Batched forward pass.  MODE 0: rows are configurations x[r]; theta1 from
+// scratch (optionally stored to th_out).  MODE 1: rows r = (s, k) of
+// connected configurations xp[r] of sample s = r / K; theta1 is the sample's
+// theta1 (th_base[s]) plus 2 x'_i W1[i] for every site where x' differs from
+// x[s].  Outputs log psi per row; g2_out (rows x H2), when set, receives
+// tanh(z2).
+// ---------------------------------------------------------------------------
+template <class T, int MODE>
+__global__ void __launch_bounds__(FF_THREADS) ffnn_forward_kernel(nka_ffnn m, const int8_t* __restrict__ xr,
+                                                                 const int8_t* __restrict__ xs, int Kc,
+                                                                 const T* __restrict__ th_base, int64_t rows,
+                                                                 T* __restrict__ logpsi, T* __restrict__ th_out,
+                                                                 T* __restrict__ g2_out) {
+  extern __shared__ unsigned char smem[];
+  const FfnnView<T> v = ffnn_view<T>(m);
+  const int N = v.N, H1 = v.H1, H2 = v.H2;
+  const int lda = lda_of(H1);
+  T* hA = (T*)smem;                                   // 32 x lda
+  T* part = hA + FF_ROWS * lda;                        // 32 x FF_MAXBLK
+  T* rowsum = part + FF_ROWS * FF_MAXBLK;              // 32
+  int8_t* xl = (int8_t*)(rowsum + FF_ROWS);            // 32 x N (+ 32 x N base rows)
+  int8_t* xb = xl + FF_ROWS * N;
+  const int64_t r0 = (int64_t)blockIdx.x * FF_ROWS;
+  for (int idx = threadIdx.x; idx < FF_ROWS * N; idx += FF_THREADS) {
+    const int r = idx / N, i = idx % N;
+    const int64_t gr = r0 + r;
+    xl[idx] = gr < rows ? xr[gr * N + i] : (int8_t)1;
+    if (MODE == 1) xb[idx] = gr < rows ? xs[(gr / Kc) * N + i] : (int8_t)1;
+  }
+  __syncthreads();
+  const int r = threadIdx.x / FF_GROUPS, g = threadIdx.x % FF_GROUPS;
+  const int64_t gr = r0 + r;
+  for (int a = g; a < H1; a += FF_GROUPS) {
+    T t;
+    if (MODE == 0) {
+      t = theta1_full<T>(v, xl + r * N, a);
+      if (th_out && gr < rows) th_out[gr * H1 + a] = t;
+    } else {
+      t = gr < rows ? th_base[(gr / Kc) * H1 + a] : (T)0;
+      const int8_t* x1 = xl + r * N;
+      const int8_t* x0 = xb + r * N;
+      for (int i = 0; i < N; ++i)
+        if (x1[i] != x0[i]) t += (x1[i] > 0 ? (T)2 : (T)-2) * v.W1[(int64_t)i * H1 + a];
+    }
+    hA[r * lda + a] = log_cosh<T>(t);
+  }
+  __syncthreads();
+  const T* W2 = v.W2;
+  const T* b2 = v.b2;
+  auto bload = [=](int k, int c) { return W2[(int64_t)k * H2 + c]; };
+  auto epi = [=](int rr, int c, T z) {
+    const T zz = z + (b2 ? b2[c] : (T)0);
+    if (g2_out && r0 + rr < rows) g2_out[(r0 + rr) * H2 + c] = tanh_t<T>(zz);
+    return log_cosh<T>(zz);
+  };
+  rows_gemm(hA, lda, H1, H2, bload, epi, part, rowsum);
+  if (threadIdx.x < FF_ROWS && r0 + threadIdx.x < rows) logpsi[r0 + threadIdx.x] = rowsum[threadIdx.x];
+}
+
+template <class T>
+inline size_t forward_lds(const nka_ffnn& m) {
+  return ffpad((size_t)FF_ROWS * lda_of(m.n_hidden1) * sizeof(T)) +
+         ffpad((size_t)FF_ROWS * FF_MAXBLK * sizeof(T)) + ffpad(FF_ROWS * sizeof(T)) +
+         ffpad((size_t)2 * FF_ROWS * m.n_visible);
+}
+
+// eloc[s] = sum_k mel[s,k] exp(lp[s,k] - lp0[s]) (FP64); one wavefront per sample
+template <class T, class M>
+__global__ void ffnn_eloc_reduce_kernel(const M* __restrict__ mels, const T* __restrict__ lp,
+                                        const T* __restrict__ lp0, int64_t ns, int K, double* __restrict__ eloc) {
+  const int wid = threadIdx.x >> 6, lane = lane_id();
+  const int64_t s = (int64_t)blockIdx.x * (blockDim.x >> 6) + wid;
+  if (s >= ns) return;
+  const double l0 = (double)lp0[s];
+  double acc = 0.0;
+  for (int k = lane; k < K; k += 64) {
+    const double mel = (double)mels[s * K + k];
+    if (mel != 0.0) acc += mel * exp((double)lp[s * K + k] - l0);
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) eloc[s] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Per-sample quantities of the log-derivatives: h1 = log_cosh(theta1),
+// g2 = tanh(z2) (from the forward pass) and d1 = (W2 g2) * tanh(theta1), the
+// gradient with respect to theta1 (a second tile product with B = W2^T).
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(FF_THREADS) ffnn_backprop_kernel(nka_ffnn m, const T* __restrict__ th1,
+                                                                  const T* __restrict__ g2, int64_t rows,
+                                                                  T* __restrict__ h1_out, T* __restrict__ d1_out) {
+  extern __shared__ unsigned char smem[];
+  const FfnnView<T> v = ffnn_view<T>(m);
+  const int H1 = v.H1, H2 = v.H2;
+  const int ldg = lda_of(H2);
+  T* gA = (T*)smem;                           // 32 x ldg
+  T* part = gA + FF_ROWS * ldg;
+  T* rowsum = part + FF_ROWS * FF_MAXBLK;
+  const int64_t r0 = (int64_t)blockIdx.x * FF_ROWS;
+  for (int idx = threadIdx.x; idx < FF_ROWS * H2; idx += FF_THREADS) {
+    const int r = idx / H2, c = idx % H2;
+    gA[r * ldg + c] = r0 + r < rows ? g2[(r0 + r) * H2 + c] : (T)0;
+  }
+  for (int idx = threadIdx.x; idx < FF_ROWS * H1; idx += FF_THREADS) {
+    const int r = idx / H1, a = idx % H1;
+    if (r0 + r < rows) h1_out[(r0 + r) * H1 + a] = log_cosh<T>(th1[(r0 + r) * H1 + a]);
+  }
+  __syncthreads();
+  const T* W2 = v.W2;
+  auto bload = [=](int k, int a) { return W2[(int64_t)a * H2 + k]; };
+  auto epi = [=](int rr, int a, T u) {
+    if (r0 + rr < rows) d1_out[(r0 + rr) * H1 + a] = u * tanh_t<T>(th1[(r0 + rr) * H1 + a]);
+    return (T)0;
+  };
+  rows_gemm(gA, ldg, H2, H1, bload, epi, part, rowsum);
+}
+
+template <class T>
+inline size_t backprop_lds(const nka_ffnn& m) {
+  return ffpad((size_t)FF_ROWS * lda_of(m.n_hidden2) * sizeof(T)) +
+         ffpad((size_t)FF_ROWS * FF_MAXBLK * sizeof(T)) + ffpad(FF_ROWS * sizeof(T));
+}
+
+// ---------------------------------------------------------------------------
+// Weighted sums over samples of the log-derivatives (FP64):
+//   out[v, p] = sum_s (O[s, p] - mean[p]) w[v, s]
+// with O = [d1 | x (x) d1 | g2 | h1 (x) g2] in ravel order.  The outer-product
+// blocks are out[i, a] = sum_s w_s L[s, i] R[s, a] with (L, R) = (x, d1) and
+// (h1, g2); a bias block takes L = 1.  Each workgroup sums a chunk of
+// samples for a 16 x 64 tile of (i, a) into its own partial buffer, and the
+// chunks are added in order afterwards.
+// ---------------------------------------------------------------------------
+constexpr int CT_I = 16, CT_A = 64, CT_S = 256;
+
+template <class T>
+__global__ void __launch_bounds__(256) ffnn_outer_partial(const int8_t* __restrict__ x, const T* __restrict__ L,
+                                                          int nL, const T* __restrict__ R, int nR, int64_t ns,
+                                                          const double* __restrict__ w, double* __restrict__ part,
+                                                          int64_t plane) {
+  __shared__ double sl[CT_S][CT_I + 1];
+  __shared__ double sw[CT_S];
+  const int ti = blockIdx.x, ta = blockIdx.y, chunk = blockIdx.z;
+  const int i0 = ti * CT_I, a0 = ta * CT_A;
+  const int64_t s0 = (int64_t)chunk * CT_S;
+  const int64_t s1 = min<int64_t>(ns, s0 + CT_S);
+  const int nsl = (int)(s1 - s0);
+  for (int idx = threadIdx.x; idx < nsl * CT_I; idx += 256) {
+    const int s = idx / CT_I, i = idx % CT_I;
+    double val = 0.0;
+    if (i0 + i < nL) {
+      if (L == nullptr && x == nullptr) val = 1.0;
+      else if (x) val = x[(s0 + s) * nL + i0 + i] > 0 ? 1.0 : -1.0;
+      else val = (double)L[(s0 + s) * nL + i0 + i];
+    }
+    sl[s][i] = val;
+  }
+  for (int s = threadIdx.x; s < nsl; s += 256) sw[s] = w[s0 + s];
+  __syncthreads();
+  const int a = a0 + (threadIdx.x & 63);
+  const int ig = threadIdx.x >> 6;  // 4 groups of 4 i each
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (a < nR) {
+    for (int s = 0; s < nsl; ++s) {
+      const double rv = (double)R[(s0 + s) * nR + a] * sw[s];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q] += sl[s][ig * 4 + q] * rv;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = i0 + ig * 4 + q;
+      if (i < nL) part[(int64_t)chunk * plane + (int64_t)i * nR + a] = acc[q];
+    }
+  }
+}
+
+// out[off + p] = sum_chunks part[chunk][p] - mean[off + p] * wsum, p < count
+__global__ void ffnn_outer_reduce(const double* __restrict__ part, int nchunks, int64_t plane, int64_t count,
+                                  const double* __restrict__ mean, int64_t off, const double* __restrict__ wsum,
+                                  double* __restrict__ out) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= count) return;
+  double s = 0.0;
+  for (int c = 0; c < nchunks; ++c) s += part[(int64_t)c * plane + p];
+  if (mean) s -= mean[off + p] * wsum[0];
+  out[off + p] = s;
+}
+
+// wsum[0] = sum_s w[s] (one workgroup, fixed order per thread then tree)
+__global__ void ffnn_wsum_kernel(const double* __restrict__ w, int64_t ns, double* __restrict__ wsum) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < ns; i += 256) s += w[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) wsum[0] = red[0];
+}
+
+// ---------------------------------------------------------------------------
+// Structured centred NTK of the network:
+//   (O O^T)_st = (hb2 + h1_s.h1_t)(g2_s.g2_t) + (hb1 + x_s.x_t)(d1_s.d1_t)
+//   T_st = scale * [(O O^T)_st - v_s - v_t + c] + shift delta_st + offset
+// v = O mean, c = |mean|^2.  Four FP64 Gram products with inner sizes H1,
+// H2, N, H1 per 64x64 tile (4 waves of 32x32, v_mfma_f64_16x16x4).
+// ---------------------------------------------------------------------------
+constexpr int NF_T = 64, NF_K = 16, NF_LD = 17;
+
+__global__ void __launch_bounds__(256) ffnn_ntk_kernel(const double* __restrict__ h1, const double* __restrict__ g2,
+                                                       const int8_t* __restrict__ x, const double* __restrict__ d1,
+                                                       int N, int H1, int H2, int64_t ns,
+                                                       const double* __restrict__ vv, const double* __restrict__ cc,
+                                                       double hb1, double hb2, double scale, double shift,
+                                                       double offset, double* __restrict__ out, int64_t ldo,
+                                                       int64_t n_tiles) {
+  __shared__ double sA[NF_T * NF_LD];
+  __shared__ double sB[NF_T * NF_LD];
+  const int64_t G = gridDim.x;
+  const int64_t bid = (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8;
+  if (bid >= n_tiles) return;
+  int ti, tj;
+  tri_supertile(bid, (ns + NF_T - 1) / NF_T, ti, tj);
+  const int64_t i0 = (int64_t)ti * NF_T, j0 = (int64_t)tj * NF_T;
+  const int tid = threadIdx.x, wid = tid / 64, lane = lane_id();
+  const int wm = (wid >> 1) * 32, wn = (wid & 1) * 32;
+  doublex4 acc[4][2][2];
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc[p][a][b] = (doublex4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int pass = 0; pass < 4; ++pass) {
+    const int K = pass == 0 ? H1 : pass == 1 ? H2 : pass == 2 ? N : H1;
+    const double* src = pass == 0 ? h1 : pass == 1 ? g2 : pass == 2 ? nullptr : d1;
+    for (int k0 = 0; k0 < K; k0 += NF_K) {
+      __syncthreads();
+#pragma unroll
+      for (int it = 0; it < 4; ++it) {
+        const int idx = tid + it * 256;
+        const int r = idx >> 4, kc = idx & 15;
+        const int kk = k0 + kc;
+        double va = 0.0, vb = 0.0;
+        if (kk < K) {
+          if (pass == 2) {
+            if (i0 + r < ns) va = x[(i0 + r) * N + kk] > 0 ? 1.0 : -1.0;
+            if (j0 + r < ns) vb = x[(j0 + r) * N + kk] > 0 ? 1.0 : -1.0;
+          } else {
+            if (i0 + r < ns) va = src[(i0 + r) * K + kk];
+            if (j0 + r < ns) vb = src[(j0 + r) * K + kk];
+          }
+        }
+        sA[r * NF_LD + kc] = va;
+        sB[r * NF_LD + kc] = vb;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int ks = 0; ks < NF_K; ks += 4) {
+        const int kq = ks + (lane >> 4);
+        double av[2], bv[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) av[a] = sA[(wm + a * 16 + (lane & 15)) * NF_LD + kq];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) bv[b] = sB[(wn + b * 16 + (lane & 15)) * NF_LD + kq];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b)
+            acc[pass][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[pass][a][b], 0, 0, 0);
+      }
+    }
+  }
+  const double c = *cc;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t gi = i0 + wm + a * 16 + (lane >> 4) + 4 * e;
+        const int64_t gj = j0 + wn + b * 16 + (lane & 15);
+        if (gi < ns && gj <= gi) {
+          const double oo = (hb2 + acc[0][a][b][e]) * acc[1][a][b][e] + (hb1 + acc[2][a][b][e]) * acc[3][a][b][e];
+          double val = scale * (oo - vv[gi] - vv[gj] + c) + offset;
+          if (gi == gj) val += shift;
+          out[gi * ldo + gj] = val;
+        }
+      }
+}
+
+template <class T>
+__global__ void ffnn_to_f64_kernel(const T* __restrict__ a, int64_t n, double* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (double)a[i];
+}
+
+// v_s = d1_s.mu_b1 + x_s.U1_s + g2_s.mu_b2 + h1_s.U2_s (one wave per sample); block 0 also writes c = |mu|^2
+__global__ void __launch_bounds__(256) ffnn_ntk_vec_kernel(const double* __restrict__ h1, const double* __restrict__ g2,
+                                                           const int8_t* __restrict__ x, const double* __restrict__ d1,
+                                                           const double* __restrict__ U1, const double* __restrict__ U2,
+                                                           int N, int H1, int H2, int64_t ns,
+                                                           const double* __restrict__ mean, int64_t b1_off,
+                                                           int64_t b2_off, int64_t np, double* __restrict__ vv,
+                                                           double* __restrict__ cc) {
+  const int wid = threadIdx.x >> 6, lane = lane_id();
+  for (int64_t s = (int64_t)blockIdx.x * 4 + wid; s < ns; s += (int64_t)gridDim.x * 4) {
+    double acc = 0.0;
+    for (int a = lane; a < H1; a += 64) {
+      acc += h1[s * H1 + a] * U2[s * H1 + a];
+      if (b1_off >= 0) acc += d1[s * H1 + a] * mean[b1_off + a];
+    }
+    for (int i = lane; i < N; i += 64) acc += (x[s * N + i] > 0 ? 1.0 : -1.0) * U1[s * N + i];
+    if (b2_off >= 0)
+      for (int k = lane; k < H2; k += 64) acc += g2[s * H2 + k] * mean[b2_off + k];
+    acc = wave_sum(acc);
+    if (lane == 0) vv[s] = acc;
+  }
+  if (blockIdx.x == 0) {
+    __shared__ double red[4];
+    double q = 0.0;
+    for (int64_t p = threadIdx.x; p < np; p += 256) q += mean[p] * mean[p];
+    q = wave_sum(q);
+    if (lane == 0) red[wid] = q;
+    __syncthreads();
+    if (threadIdx.x == 0) cc[0] = red[0] + red[1] + red[2] + red[3];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Centred, rescaled Jacobian (same layouts and output kinds as the RBM's).
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ inline double ffnn_o(const nka_ffnn& m, const int8_t* x, const T* h1, const T* d1, const T* g2,
+                                int64_t s, int64_t p) {
+  const int N = m.n_visible, H1 = m.n_hidden1, H2 = m.n_hidden2;
+  if (m.b1_off >= 0 && p >= m.b1_off && p < m.b1_off + H1) return (double)d1[s * H1 + (p - m.b1_off)];
+  if (p >= m.w1_off && p < m.w1_off + (int64_t)N * H1) {
+    const int64_t q = p - m.w1_off;
+    const int i = (int)(q / H1), a = (int)(q % H1);
+    return (x[s * N + i] > 0 ? 1.0 : -1.0) * (double)d1[s * H1 + a];
+  }
+  if (m.b2_off >= 0 && p >= m.b2_off && p < m.b2_off + H2) return (double)g2[s * H2 + (p - m.b2_off)];
+  const int64_t q = p - m.w2_off;
+  const int a = (int)(q / H2), k = (int)(q % H2);
+  return (double)h1[s * H1 + a] * (double)g2[s * H2 + k];
+}
+
+__device__ inline uint16_t bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+template <class T>
+__global__ void ffnn_jacobian_kernel(nka_ffnn m, const int8_t* __restrict__ x, const T* __restrict__ h1,
+                                     const T* __restrict__ d1, const T* __restrict__ g2, int64_t ns, int64_t np,
+                                     const double* __restrict__ mean, double scale, int layout, int kind,
+                                     void* __restrict__ oa, void* __restrict__ ob, int64_t ldo) {
+  const int64_t total = ns * np;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int64_t s, p;
+    if (layout == NKA_LAYOUT_PS) {
+      p = idx / ns;
+      s = idx % ns;
+    } else {
+      s = idx / np;
+      p = idx % np;
+    }
+    double o = ffnn_o<T>(m, x, h1, d1, g2, s, p);
+    if (mean) o -= mean[p];
+    o *= scale;
+    const int64_t off = layout == NKA_LAYOUT_PS ? p * ldo + s : s * ldo + p;
+    if (kind == NKA_OUT_F64) {
+      ((double*)oa)[off] = o;
+    } else if (kind == NKA_OUT_F32) {
+      ((float*)oa)[off] = (float)o;
+    } else {
+      const float f = (float)o;
+      const uint16_t hi = bf16_rne(f);
+      const float rem = f - __uint_as_float((uint32_t)hi << 16);
+      ((uint16_t*)oa)[off] = hi;
+      ((uint16_t*)ob)[off] = bf16_rne(rem);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Metropolis sampler: 32 chains per workgroup in lock step.  Thread t holds
+// theta1 of chain t / 8 for the units a = t % 8 + 8 j in registers; the chain
+// spins live in LDS.  Each proposal: wave w draws the proposals of chains
+// 8w .. 8w+7 (same rules and Philox stream layout as the RBM sampler:
+// counter q, chain id, TAG_METROPOLIS; x word picks the site / cluster, the
+// uniform of the acceptance comes from the same draw); theta1' and
+// h1' = log_cosh(theta1') are formed, the second layer gives log psi'; a
+// chain accepts when u < exp(machine_pow (log psi' - log psi) + corr).
+// ---------------------------------------------------------------------------
+struct FfSweepArgs {
+  int rule;
+  const int32_t* clusters;
+  int n_clusters;
+  int C, sweep_size, n_discard, chain_length;
+  double machine_pow;
+  uint32_t k0, k1;
+  uint64_t counter;
+  const uint64_t* counter_dev;
+};
+
+constexpr int FF_MAXCL = 16;  // cluster slots per lane: n_clusters <= 1024
+
+template <class T, int HPT>
+__global__ void __launch_bounds__(FF_THREADS) ffnn_metropolis_kernel(nka_ffnn m, FfSweepArgs sa,
+                                                                    int8_t* __restrict__ sigma, T* __restrict__ log_prob,
+                                                                    int8_t* __restrict__ samples,
+                                                                    T* __restrict__ lp_samples,
+                                                                    int64_t* __restrict__ n_accepted) {
+  extern __shared__ unsigned char smem[];
+  const FfnnView<T> v = ffnn_view<T>(m);
+  const int N = v.N, H1 = v.H1, H2 = v.H2;
+  const int lda = lda_of(H1);
+  T* hA = (T*)smem;
+  T* part = hA + FF_ROWS * lda;
+  T* rowsum = part + FF_ROWS * FF_MAXBLK;
+  T* lp = rowsum + FF_ROWS;                       // current log prob per chain
+  T* pcorr = lp + FF_ROWS;                        // proposal correction
+  T* pu = pcorr + FF_ROWS;                        // acceptance uniform
+  int* psite = (int*)(pu + FF_ROWS);              // 2 sites per chain
+  int* pnf = psite + 2 * FF_ROWS;                 // flipped sites (0: no move)
+  int* pacc = pnf + FF_ROWS;                      // accepted flag
+  int* nacc = pacc + FF_ROWS;
+  int8_t* xs = (int8_t*)(nacc + FF_ROWS);         // 32 x N spins
+  const int tid = threadIdx.x, wid = tid >> 6, lane = lane_id();
+  const int c0 = blockIdx.x * FF_ROWS;
+  const int cr = tid / FF_GROUPS, g = tid % FF_GROUPS;   // chain row and unit group of this thread
+  const int chain = c0 + cr;
+
+  for (int idx = tid; idx < FF_ROWS * N; idx += FF_THREADS) {
+    const int r = idx / N, i = idx % N;
+    xs[idx] = (c0 + r < sa.C) ? sigma[(int64_t)(c0 + r) * N + i] : (int8_t)1;
+  }
+  if (tid < FF_ROWS) nacc[tid] = 0;
+  __syncthreads();
+  T th[HPT];
+#pragma unroll
+  for (int j = 0; j < HPT; ++j) {
+    const int a = g + FF_GROUPS * j;
+    th[j] = a < H1 ? theta1_full<T>(v, xs + cr * N, a) : (T)0;
+    if (a < H1) hA[cr * lda + a] = log_cosh<T>(th[j]);
+  }
+  __syncthreads();
+  const T* W2 = v.W2;
+  const T* b2 = v.b2;
+  auto bload = [=](int k, int c) { return W2[(int64_t)k * H2 + c]; };
+  auto epi = [=](int, int c, T z) { return log_cosh<T>(z + (b2 ? b2[c] : (T)0)); };
+  rows_gemm(hA, lda, H1, H2, bload, epi, part, rowsum);
+  const T pw = (T)sa.machine_pow;
+  if (tid < FF_ROWS) lp[tid] = pw * rowsum[tid];
+
+  // cluster slots of this lane (exchange rule): cl = lane + 64 r
+  const bool exch = sa.rule == NKA_RULE_EXCHANGE;
+  const int RS = exch ? (sa.n_clusters + 63) / 64 : 0;
+  int cu[FF_MAXCL], cv[FF_MAXCL];
+#pragma unroll
+  for (int r = 0; r < FF_MAXCL; ++r) {
+    const int cl = lane + 64 * r;
+    const bool ok = r < RS && cl < sa.n_clusters;
+    cu[r] = ok ? sa.clusters[2 * cl] : -1;
+    cv[r] = ok ? sa.clusters[2 * cl + 1] : -1;
+  }
+  uint64_t q = sa.counter;
+  if (sa.counter_dev) {
+    const uint64_t cd = *sa.counter_dev;
+    q = ((uint64_t)uniform_value<uint32_t>((uint32_t)(cd >> 32)) << 32) |
+        (uint64_t)uniform_value<uint32_t>((uint32_t)cd);
+  }
+  __syncthreads();
+  const int n_sweeps = sa.n_discard + sa.chain_length;
+  for (int sw = 0; sw < n_sweeps; ++sw) {
+    for (int p = 0; p < sa.sweep_size; ++p, ++q) {
+      // proposals of chains 8w .. 8w+7, one at a time per wavefront
+      for (int cc = 0; cc < FF_ROWS / 4; ++cc) {
+        const int r = wid * (FF_ROWS / 4) + cc;
+        const int ch = c0 + r;
+        const u32x4 rnd = philox4x32_10({(uint32_t)q, (uint32_t)(q >> 32), (uint32_t)ch, TAG_METROPOLIS},
+                                        sa.k0, sa.k1);
+        const int8_t* x = xs + r * N;
+        int s0 = 0, s1 = 0, nf = 1;
+        T corr = (T)0;
+        if (!exch) {
+          s0 = (int)bounded(uniform_value<uint32_t>(rnd.x), (uint32_t)N);
+        } else {
+          uint64_t bal[FF_MAXCL];
+          int n_diff = 0;
+#pragma unroll
+          for (int rr = 0; rr < FF_MAXCL; ++rr) {
+            const bool d = rr < RS && cu[rr] >= 0 && x[cu[rr]] != x[cv[rr]];
+            bal[rr] = __ballot(d);
+            n_diff += __popcll(bal[rr]);
+          }
+          n_diff = uniform_value<int>(n_diff);
+          if (n_diff == 0) {
+            nf = 0;
+          } else {
+            int k = (int)bounded(uniform_value<uint32_t>(rnd.x), (uint32_t)n_diff);
+            int sel = -1;
+#pragma unroll
+            for (int rr = 0; rr < FF_MAXCL; ++rr) {
+              if (sel < 0 && rr < RS) {
+                const int cnt = __popcll(bal[rr]);
+                if (k < cnt) sel = rr * 64 + kth_set_bit(bal[rr], k);
+                else k -= cnt;
+              }
+            }
+            sel = uniform_value<int>(sel);
+            if (sel < 0 || sel >= sa.n_clusters) sel = 0;  // unreachable: k < n_diff
+            s0 = sa.clusters[2 * sel];
+            s1 = sa.clusters[2 * sel + 1];
+            nf = 2;
+            int n_new = 0;
+#pragma unroll
+            for (int rr = 0; rr < FF_MAXCL; ++rr) {
+              bool d = false;
+              if (rr < RS && cu[rr] >= 0) {
+                const int ua = cu[rr], ub = cv[rr];
+                const int8_t xa = ua == s0 ? x[s1] : (ua == s1 ? x[s0] : x[ua]);
+                const int8_t xb = ub == s0 ? x[s1] : (ub == s1 ? x[s0] : x[ub]);
+                d = xa != xb;
+              }
+              n_new += __popcll(__ballot(d));
+            }
+            n_new = uniform_value<int>(n_new);
+            corr = uniform_value<T>(log_t<T>((T)n_diff) - log_t<T>((T)n_new));
+          }
+        }
+        if (lane == 0) {
+          psite[2 * r] = s0;
+          psite[2 * r + 1] = s1;
+          pnf[r] = nf;
+          pcorr[r] = corr;
+          pu[r] = uniform01<T>(rnd);
+        }
+      }
+      __syncthreads();
+      // theta1' and h1' of every chain
+      T thn[HPT];
+      {
+        const int nf = pnf[cr];
+        const int8_t* x = xs + cr * N;
+#pragma unroll
+        for (int j = 0; j < HPT; ++j) {
+          const int a = g + FF_GROUPS * j;
+          T t = th[j];
+          if (a < H1) {
+            for (int f = 0; f < nf; ++f) {
+              const int site = psite[2 * cr + f];
+              // x'_site = -x_site: theta1 gains (x' - x) W1 = -2 x W1
+              t += (x[site] > 0 ? (T)-2 : (T)2) * v.W1[(int64_t)site * H1 + a];
+            }
+            hA[cr * lda + a] = log_cosh<T>(t);
+          }
+          thn[j] = t;
+        }
+      }
+      __syncthreads();
+      rows_gemm(hA, lda, H1, H2, bload, epi, part, rowsum);
+      if (tid < FF_ROWS) {
+        const int r = tid;
+        int acc = 0;
+        if (pnf[r] > 0 && c0 + r < sa.C) {
+          const T lpn = pw * rowsum[r];
+          acc = pu[r] < exp_t<T>(lpn - lp[r] + pcorr[r]);
+          if (acc) {
+            lp[r] = lpn;
+            for (int f = 0; f < pnf[r]; ++f) {
+              const int site = psite[2 * r + f];
+              xs[r * N + site] = (int8_t)(-xs[r * N + site]);
+            }
+            nacc[r] += 1;
+          }
+        }
+        pacc[r] = acc;
+      }
+      __syncthreads();
+      if (pacc[cr]) {
+#pragma unroll
+        for (int j = 0; j < HPT; ++j) th[j] = thn[j];
+      }
+    }
+    if (sw >= sa.n_discard) {
+      const int t = sw - sa.n_discard;
+      for (int idx = tid; idx < FF_ROWS * N; idx += FF_THREADS) {
+        const int r = idx / N, i = idx % N;
+        if (c0 + r < sa.C) samples[((int64_t)(c0 + r) * sa.chain_length + t) * N + i] = xs[idx];
+      }
+      if (lp_samples && tid < FF_ROWS && c0 + tid < sa.C)
+        lp_samples[(int64_t)(c0 + tid) * sa.chain_length + t] = lp[tid];
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < FF_ROWS * N; idx += FF_THREADS) {
+    const int r = idx / N, i = idx % N;
+    if (c0 + r < sa.C) sigma[(int64_t)(c0 + r) * N + i] = xs[idx];
+  }
+  if (tid < FF_ROWS && c0 + tid < sa.C) {
+    log_prob[c0 + tid] = lp[tid];
+    n_accepted[c0 + tid] += nacc[tid];
+  }
+  (void)chain;
+}
+
+template <class T>
+inline size_t metropolis_lds(const nka_ffnn& m) {
+  return ffpad((size_t)FF_ROWS * lda_of(m.n_hidden1) * sizeof(T)) +
+         ffpad((size_t)FF_ROWS * FF_MAXBLK * sizeof(T)) + ffpad((size_t)4 * FF_ROWS * sizeof(T)) +
+         ffpad((size_t)5 * FF_ROWS * sizeof(int)) + ffpad((size_t)FF_ROWS * m.n_visible);
+}
+
+// Dynamic LDS above the default 64 KB needs the function attribute.
+template <class K>
+static void allow_lds(K kernel, int64_t bytes) {
+  if (bytes > 64 * 1024)
+    (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+template <class T, int HPT>
+static void launch_ffnn_sweep(const nka_ffnn& m, const FfSweepArgs& sa, int8_t* sigma, void* lp, int8_t* smp,
+                              void* lps, int64_t* nacc, hipStream_t s) {
+  const unsigned g = (unsigned)((sa.C + FF_ROWS - 1) / FF_ROWS);
+  allow_lds(ffnn_metropolis_kernel<T, HPT>, (int64_t)metropolis_lds<T>(m));
+  hipLaunchKernelGGL((ffnn_metropolis_kernel<T, HPT>), dim3(g), dim3(FF_THREADS), metropolis_lds<T>(m), s, m,
+                     sa, sigma, (T*)lp, smp, (T*)lps, nacc);
+}
+
+template <class T>
+static void dispatch_ffnn_sweep(const nka_ffnn& m, const FfSweepArgs& sa, int8_t* sigma, void* lp, int8_t* smp,
+                                void* lps, int64_t* nacc, hipStream_t s) {
+  const int need = (m.n_hidden1 + FF_GROUPS - 1) / FF_GROUPS;
+  if (need <= 8) launch_ffnn_sweep<T, 8>(m, sa, sigma, lp, smp, lps, nacc, s);
+  else if (need <= 16) launch_ffnn_sweep<T, 16>(m, sa, sigma, lp, smp, lps, nacc, s);
+  else if (need <= 32) launch_ffnn_sweep<T, 32>(m, sa, sigma, lp, smp, lps, nacc, s);
+  else if (need <= 64) launch_ffnn_sweep<T, 64>(m, sa, sigma, lp, smp, lps, nacc, s);
+  else launch_ffnn_sweep<T, 128>(m, sa, sigma, lp, smp, lps, nacc, s);
+}
+
+static int check_model(const nka_ffnn* m, const char* who) {
+  NKA_CHECK_ARG(m && m->params, "%s: null model", who);
+  NKA_CHECK_ARG(m->n_visible > 0 && m->n_hidden1 > 0 && m->n_hidden2 > 0, "%s: bad sizes", who);
+  NKA_CHECK_ARG(m->n_hidden1 <= FF_MAXH && m->n_hidden2 <= FF_MAXH, "%s: hidden layers larger than %d", who,
+                FF_MAXH);
+  NKA_CHECK_ARG(m->dtype == NKA_F32 || m->dtype == NKA_F64, "%s: bad dtype", who);
+  const int64_t lds = m->dtype == NKA_F32
+                          ? std::max<int64_t>(std::max<int64_t>(forward_lds<float>(*m), backprop_lds<float>(*m)),
+                                              metropolis_lds<float>(*m))
+                          : std::max<int64_t>(std::max<int64_t>(forward_lds<double>(*m), backprop_lds<double>(*m)),
+                                              metropolis_lds<double>(*m));
+  NKA_CHECK_ARG(lds <= FF_LDS_MAX, "%s: layer sizes need %lld bytes of LDS per workgroup (max %lld)", who,
+                (long long)lds, (long long)FF_LDS_MAX);
+  return NKA_OK;
+}
+
+inline int64_t ffnn_np(const nka_ffnn& m) {
+  return (m.b1_off >= 0 ? m.n_hidden1 : 0) + (int64_t)m.n_visible * m.n_hidden1 +
+         (m.b2_off >= 0 ? m.n_hidden2 : 0) + (int64_t)m.n_hidden1 * m.n_hidden2;
+}
+
+template <class T>
+static void launch_forward0(const nka_ffnn& m, const int8_t* x, int64_t rows, T* lp, T* th, T* g2, hipStream_t s) {
+  if (rows <= 0) return;
+  const unsigned g = (unsigned)((rows + FF_ROWS - 1) / FF_ROWS);
+  allow_lds(ffnn_forward_kernel<T, 0>, (int64_t)forward_lds<T>(m));
+  hipLaunchKernelGGL((ffnn_forward_kernel<T, 0>), dim3(g), dim3(FF_THREADS), forward_lds<T>(m), s, m, x,
+                     (const int8_t*)nullptr, 1, (const T*)nullptr, rows, lp, th, g2);
+}
+
+static int op_rows(int op_kind, const void* op, int N, int* K, int* mel_dtype) {
+  if (op_kind == NKA_OP_ISING) {
+    const nka_ising* o = (const nka_ising*)op;
+    *K = o->h == 0.0 ? 1 : N + 1;
+    *mel_dtype = o->mel_dtype;
+  } else if (op_kind == NKA_OP_LOCAL) {
+    const nka_localop* o = (const nka_localop*)op;
+    *K = o->max_conn_size;
+    *mel_dtype = o->mel_dtype;
+  } else {
+    return NKA_ERR_ARG;
+  }
+  return NKA_OK;
+}
+
+// bytes per sample of the local-energy workspace
+static int64_t eloc_bytes_per_sample(const nka_ffnn& m, int K) {
+  const int64_t ts = m.dtype == NKA_F32 ? 4 : 8;
+  return ffpad((int64_t)K * m.n_visible) + (int64_t)K * 8 + (int64_t)K * ts + (int64_t)m.n_hidden1 * ts + ts + 4;
+}
+
+template <class T>
+static int ffnn_eloc_impl(const nka_ffnn& m, int op_kind, const void* op, const int8_t* sigma, int64_t n,
+                          double* eloc, T* log_psi, unsigned char* ws, int64_t ws_bytes, hipStream_t s) {
+  int K = 0, md = 0;
+  if (op_rows(op_kind, op, m.n_visible, &K, &md) != NKA_OK) {
+    set_error("nka_ffnn_local_energy: unknown operator kind %d", op_kind);
+    return NKA_ERR_ARG;
+  }
+  const int N = m.n_visible, H1 = m.n_hidden1;
+  const int64_t per = eloc_bytes_per_sample(m, K);
+  const int64_t chunk = std::min<int64_t>(n, (ws_bytes - 4096) / per);
+  NKA_CHECK_ARG(chunk >= 1, "nka_ffnn_local_energy: workspace too small");
+  unsigned char* p = ws;
+  auto take = [&](int64_t bytes) {
+    unsigned char* q = p;
+    p += ffpad(bytes);
+    return q;
+  };
+  int8_t* xp = (int8_t*)take(chunk * K * N);
+  void* mels = take(chunk * K * 8);
+  T* lp = (T*)take(chunk * K * sizeof(T));
+  T* th = (T*)take(chunk * H1 * sizeof(T));
+  T* lp0own = (T*)take(chunk * sizeof(T));
+  for (int64_t s0 = 0; s0 < n; s0 += chunk) {
+    const int64_t cs = std::min<int64_t>(chunk, n - s0);
+    const int8_t* x = sigma + s0 * N;
+    T* lp0 = log_psi ? log_psi + s0 : lp0own;
+    int st;
+    if (op_kind == NKA_OP_ISING) {
+      const nka_ising* o = (const nka_ising*)op;
+      st = nka_ising_conn(x, cs, N, o->edges, o->n_edges, o->h, o->J, o->mel_dtype, K, xp, mels, s);
+    } else {
+      st = nka_localop_conn(x, cs, N, (const nka_localop*)op, xp, mels, nullptr, s);
+    }
+    if (st) return st;
+    launch_forward0<T>(m, x, cs, lp0, th, nullptr, s);
+    const int64_t rows = cs * K;
+    const unsigned g = (unsigned)((rows + FF_ROWS - 1) / FF_ROWS);
+    allow_lds(ffnn_forward_kernel<T, 1>, (int64_t)forward_lds<T>(m));
+    hipLaunchKernelGGL((ffnn_forward_kernel<T, 1>), dim3(g), dim3(FF_THREADS), forward_lds<T>(m), s, m, xp, x, K,
+                       (const T*)th, rows, lp, (T*)nullptr, (T*)nullptr);
+    const unsigned gr = (unsigned)((cs + 3) / 4);
+    if (md == NKA_F32)
+      hipLaunchKernelGGL((ffnn_eloc_reduce_kernel<T, float>), dim3(gr), dim3(256), 0, s, (const float*)mels,
+                         (const T*)lp, (const T*)lp0, cs, K, eloc + s0);
+    else
+      hipLaunchKernelGGL((ffnn_eloc_reduce_kernel<T, double>), dim3(gr), dim3(256), 0, s, (const double*)mels,
+                         (const T*)lp, (const T*)lp0, cs, K, eloc + s0);
+  }
+  return NKA_OK;
+}
+
+template <class T>
+static void contract_one(const nka_ffnn& m, const int8_t* sigma, const T* h1, const T* d1, const T* g2, int64_t ns,
+                         const double* w, const double* mean, double* out, double* part, double* wsum,
+                         hipStream_t s) {
+  const int N = m.n_visible, H1 = m.n_hidden1, H2 = m.n_hidden2;
+  const int nch = (int)((ns + CT_S - 1) / CT_S);
+  hipLaunchKernelGGL(ffnn_wsum_kernel, dim3(1), dim3(256), 0, s, w, ns, wsum);
+  struct Blk {
+    int64_t off;
+    const int8_t* x;
+    const T* L;
+    int nL;
+    const T* R;
+    int nR;
+  };
+  Blk blks[4];
+  int nb = 0;
+  if (m.b1_off >= 0) blks[nb++] = {m.b1_off, nullptr, nullptr, 1, d1, H1};
+  blks[nb++] = {m.w1_off, sigma, nullptr, N, d1, H1};
+  if (m.b2_off >= 0) blks[nb++] = {m.b2_off, nullptr, nullptr, 1, g2, H2};
+  blks[nb++] = {m.w2_off, nullptr, h1, H1, g2, H2};
+  for (int b = 0; b < nb; ++b) {
+    const Blk& k = blks[b];
+    const int64_t plane = (int64_t)k.nL * k.nR;
+    dim3 grid((unsigned)((k.nL + CT_I - 1) / CT_I), (unsigned)((k.nR + CT_A - 1) / CT_A), (unsigned)nch);
+    hipLaunchKernelGGL((ffnn_outer_partial<T>), grid, dim3(256), 0, s, k.x, k.L, k.nL, k.R, k.nR, ns, w, part,
+                       plane);
+    hipLaunchKernelGGL(ffnn_outer_reduce, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s,
+                       (const double*)part, nch, plane, plane, mean, k.off, (const double*)wsum, out);
+  }
+}
+
+}  // namespace nka
+
+using namespace nka;
+
+extern "C" {
+
+int nka_ffnn_log_psi(const nka_ffnn* m, const int8_t* sigma, int64_t n, void* log_psi, void* stream) {
+  if (int st = check_model(m, "nka_ffnn_log_psi")) return st;
+  NKA_CHECK_ARG(n >= 0 && (n == 0 || (sigma && log_psi)), "nka_ffnn_log_psi: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  if (m->dtype == NKA_F32)
+    launch_forward0<float>(*m, sigma, n, (float*)log_psi, nullptr, nullptr, s);
+  else
+    launch_forward0<double>(*m, sigma, n, (double*)log_psi, nullptr, nullptr, s);
+  NKA_CHECK_LAUNCH("nka_ffnn_log_psi");
+  return NKA_OK;
+}
+
+int64_t nka_ffnn_local_energy_workspace(const nka_ffnn* m, int op_kind, const void* op, int64_t n) {
+  if (!m || !op || n <= 0) return 0;
+  int K = 0, md = 0;
+  if (op_rows(op_kind, op, m->n_visible, &K, &md) != NKA_OK || K <= 0) return 0;
+  const int64_t per = eloc_bytes_per_sample(*m, K);
+  const int64_t cap = (int64_t)512 << 20;
+  const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(n, cap / per));
+  return chunk * per + 4096;
+}
+
+int nka_ffnn_local_energy(const nka_ffnn* m, int op_kind, const void* op, const int8_t* sigma, int64_t n,
+                          double* eloc, void* log_psi, void* workspace, int64_t workspace_bytes, void* stream) {
+  if (int st = check_model(m, "nka_ffnn_local_energy")) return st;
+  NKA_CHECK_ARG(op && n >= 0 && (n == 0 || (sigma && eloc && workspace)), "nka_ffnn_local_energy: bad arguments");
+  if (n == 0) return NKA_OK;
+  hipStream_t s = (hipStream_t)stream;
+  int st;
+  if (m->dtype == NKA_F32)
+    st = ffnn_eloc_impl<float>(*m, op_kind, op, sigma, n, eloc, (float*)log_psi, (unsigned char*)workspace,
+                               workspace_bytes, s);
+  else
+    st = ffnn_eloc_impl<double>(*m, op_kind, op, sigma, n, eloc, (double*)log_psi, (unsigned char*)workspace,
+                                workspace_bytes, s);
+  if (st) return st;
+  NKA_CHECK_LAUNCH("nka_ffnn_local_energy");
+  return NKA_OK;
+}
+
+int nka_ffnn_metropolis(const nka_ffnn* m, int rule, const int32_t* clusters, int n_clusters, int8_t* sigma,
+                        void* log_prob, int n_chains, int sweep_size, int n_discard, int chain_length,
+                        double machine_pow, uint64_t seed, uint64_t counter, uint64_t* counter_dev,
+                        int8_t* samples, void* log_prob_samples, int64_t* n_accepted, void* stream) {
+  if (int st = check_model(m, "nka_ffnn_metropolis")) return st;
+  NKA_CHECK_ARG(rule == NKA_RULE_LOCAL || rule == NKA_RULE_EXCHANGE, "nka_ffnn_metropolis: unknown rule %d",
+                rule);
+  NKA_CHECK_ARG(rule == NKA_RULE_LOCAL || (clusters && n_clusters > 0 && n_clusters <= FF_MAXCL * 64),
+                "nka_ffnn_metropolis: exchange rule needs 1..%d clusters", FF_MAXCL * 64);
+  NKA_CHECK_ARG(n_chains >= 0 && sweep_size >= 0 && n_discard >= 0 && chain_length >= 0,
+                "nka_ffnn_metropolis: bad sizes");
+  NKA_CHECK_ARG(n_chains == 0 || (sigma && log_prob && n_accepted && (chain_length == 0 || samples)),
+                "nka_ffnn_metropolis: null buffer");
+  if (n_chains == 0) return NKA_OK;
+  FfSweepArgs sa;
+  sa.rule = rule;
+  sa.clusters = clusters;
+  sa.n_clusters = rule == NKA_RULE_EXCHANGE ? n_clusters : 0;
+  sa.C = n_chains;
+  sa.sweep_size = sweep_size;
+  sa.n_discard = n_discard;
+  sa.chain_length = chain_length;
+  sa.machine_pow = machine_pow;
+  sa.k0 = (uint32_t)seed;
+  sa.k1 = (uint32_t)(seed >> 32);
+  sa.counter = counter;
+  sa.counter_dev = counter_dev;
+  hipStream_t s = (hipStream_t)stream;
+  if (m->dtype == NKA_F32)
+    dispatch_ffnn_sweep<float>(*m, sa, sigma, log_prob, samples, log_prob_samples, n_accepted, s);
+  else
+    dispatch_ffnn_sweep<double>(*m, sa, sigma, log_prob, samples, log_prob_samples, n_accepted, s);
+  NKA_CHECK_LAUNCH("nka_ffnn_metropolis");
+  return NKA_OK;
+}
+
+int64_t nka_ffnn_prepare_workspace(const nka_ffnn* m, int64_t ns) {
+  if (!m || ns <= 0) return 0;
+  const int64_t ts = m->dtype == NKA_F32 ? 4 : 8;
+  return ffpad(ns * m->n_hidden1 * ts) + ffpad(ns * ts) + 64;
+}
+
+int nka_ffnn_prepare(const nka_ffnn* m, const int8_t* sigma, int64_t ns, void* h1, void* d1, void* g2,
+                     void* workspace, void* stream) {
+  if (int st = check_model(m, "nka_ffnn_prepare")) return st;
+  NKA_CHECK_ARG(ns >= 0 && (ns == 0 || (sigma && h1 && d1 && g2 && workspace)), "nka_ffnn_prepare: bad arguments");
+  if (ns == 0) return NKA_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t ts = m->dtype == NKA_F32 ? 4 : 8;
+  unsigned char* w = (unsigned char*)workspace;
+  void* th = w;
+  void* lp = w + ffpad(ns * m->n_hidden1 * ts);
+  const unsigned g = (unsigned)((ns + FF_ROWS - 1) / FF_ROWS);
+  if (m->dtype == NKA_F32) {
+    launch_forward0<float>(*m, sigma, ns, (float*)lp, (float*)th, (float*)g2, s);
+    allow_lds(ffnn_backprop_kernel<float>, (int64_t)backprop_lds<float>(*m));
+    hipLaunchKernelGGL((ffnn_backprop_kernel<float>), dim3(g), dim3(FF_THREADS), backprop_lds<float>(*m), s, *m,
+                       (const float*)th, (const float*)g2, ns, (float*)h1, (float*)d1);
+  } else {
+    launch_forward0<double>(*m, sigma, ns, (double*)lp, (double*)th, (double*)g2, s);
+    allow_lds(ffnn_backprop_kernel<double>, (int64_t)backprop_lds<double>(*m));
+    hipLaunchKernelGGL((ffnn_backprop_kernel<double>), dim3(g), dim3(FF_THREADS), backprop_lds<double>(*m), s, *m,
+                       (const double*)th, (const double*)g2, ns, (double*)h1, (double*)d1);
+  }
+  NKA_CHECK_LAUNCH("nka_ffnn_prepare");
+  return NKA_OK;
+}
+
+int64_t nka_ffnn_contract_workspace(const nka_ffnn* m, int64_t ns, int n_w) {
+  if (!m || ns <= 0) return 0;
+  const int64_t nch = (ns + CT_S - 1) / CT_S;
+  const int64_t biggest = std::max<int64_t>((int64_t)m->n_visible * m->n_hidden1,
+                                            (int64_t)m->n_hidden1 * m->n_hidden2);
+  return nch * biggest * 8 + 64 + (int64_t)n_w * 0;
+}
+
+int nka_ffnn_contract(const nka_ffnn* m, const int8_t* sigma, const void* h1, const void* d1, const void* g2,
+                      int64_t ns, const double* w, int n_w, const double* mean, double* out, void* workspace,
+                      void* stream) {
+  if (int st = check_model(m, "nka_ffnn_contract")) return st;
+  NKA_CHECK_ARG(ns > 0 && n_w >= 1 && n_w <= 2 && sigma && h1 && d1 && g2 && w && out && workspace,
+                "nka_ffnn_contract: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t np = ffnn_np(*m);
+  const int64_t nch = (ns + CT_S - 1) / CT_S;
+  const int64_t biggest = std::max<int64_t>((int64_t)m->n_visible * m->n_hidden1,
+                                            (int64_t)m->n_hidden1 * m->n_hidden2);
+  double* part = (double*)workspace;
+  double* wsum = part + nch * biggest;
+  for (int vi = 0; vi < n_w; ++vi) {
+    if (m->dtype == NKA_F32)
+      contract_one<float>(*m, sigma, (const float*)h1, (const float*)d1, (const float*)g2, ns, w + vi * ns, mean,
+                          out + vi * np, part, wsum, s);
+    else
+      contract_one<double>(*m, sigma, (const double*)h1, (const double*)d1, (const double*)g2, ns, w + vi * ns,
+                           mean, out + vi * np, part, wsum, s);
+  }
+  NKA_CHECK_LAUNCH("nka_ffnn_contract");
+  return NKA_OK;
+}
+
+int64_t nka_ffnn_ntk_workspace(const nka_ffnn* m, int64_t ns) {
+  if (!m || ns <= 0) return 0;
+  const int64_t H1 = m->n_hidden1, H2 = m->n_hidden2, N = m->n_visible;
+  return (ns * (H1 + H2 + H1) + ns * N + ns * H1 + ns + 8) * 8 + 256;
+}
+
+int nka_ffnn_ntk(const nka_ffnn* m, const int8_t* sigma, const void* h1, const void* d1, const void* g2, int64_t ns,
+                 const double* mean, double scale, double shift, double offset, double* out, int64_t ldo,
+                 void* workspace, void* stream) {
+  if (int st = check_model(m, "nka_ffnn_ntk")) return st;
+  NKA_CHECK_ARG(ns > 0 && sigma && h1 && d1 && g2 && mean && out && workspace && ldo >= ns,
+                "nka_ffnn_ntk: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  const int N = m->n_visible, H1 = m->n_hidden1, H2 = m->n_hidden2;
+  double* h1d = (double*)workspace;
+  double* g2d = h1d + ns * H1;
+  double* d1d = g2d + ns * H2;
+  double* U1 = d1d + ns * H1;
+  double* U2 = U1 + ns * N;
+  double* vv = U2 + ns * H1;
+  double* cc = vv + ns;
+  const unsigned gc = 4096;
+  if (m->dtype == NKA_F32) {
+    hipLaunchKernelGGL((ffnn_to_f64_kernel<float>), dim3(gc), dim3(256), 0, s, (const float*)h1, ns * H1, h1d);
+    hipLaunchKernelGGL((ffnn_to_f64_kernel<float>), dim3(gc), dim3(256), 0, s, (const float*)g2, ns * H2, g2d);
+    hipLaunchKernelGGL((ffnn_to_f64_kernel<float>), dim3(gc), dim3(256), 0, s, (const float*)d1, ns * H1, d1d);
+  } else {
+    hipLaunchKernelGGL((ffnn_to_f64_kernel<double>), dim3(gc), dim3(256), 0, s, (const double*)h1, ns * H1, h1d);
+    hipLaunchKernelGGL((ffnn_to_f64_kernel<double>), dim3(gc), dim3(256), 0, s, (const double*)g2, ns * H2, g2d);
+    hipLaunchKernelGGL((ffnn_to_f64_kernel<double>), dim3(gc), dim3(256), 0, s, (const double*)d1, ns * H1, d1d);
+  }
+  // U1 = D1 M_W1^T (ns x N), U2 = G2 M_W2^T (ns x H1)
+  int st = nka_gemm_nt_f64(d1d, H1, mean + m->w1_off, H1, U1, N, ns, N, H1, 1.0, 0.0, stream);
+  if (st) return st;
+  st = nka_gemm_nt_f64(g2d, H2, mean + m->w2_off, H2, U2, H1, ns, H1, H2, 1.0, 0.0, stream);
+  if (st) return st;
+  const int64_t np = ffnn_np(*m);
+  hipLaunchKernelGGL(ffnn_ntk_vec_kernel, dim3((unsigned)std::min<int64_t>((ns + 3) / 4, 4096)), dim3(256), 0, s,
+                     h1d, g2d, sigma, d1d, U1, U2, N, H1, H2, ns, mean, m->b1_off, m->b2_off, np, vv, cc);
+  const int64_t T = (ns + NF_T - 1) / NF_T;
+  const int64_t tiles = T * (T + 1) / 2;
+  NKA_CHECK_ARG(tiles < (1LL << 31), "nka_ffnn_ntk: too large");
+  const unsigned grid = (unsigned)(((tiles + 7) / 8) * 8);
+  hipLaunchKernelGGL(ffnn_ntk_kernel, dim3(grid), dim3(256), 0, s, h1d, g2d, sigma, d1d, N, H1, H2, ns, vv, cc,
+                     m->b1_off >= 0 ? 1.0 : 0.0, m->b2_off >= 0 ? 1.0 : 0.0, scale, shift, offset, out, ldo, tiles);
+  NKA_CHECK_LAUNCH("nka_ffnn_ntk");
+  return NKA_OK;
+}
+
+int nka_ffnn_jacobian(const nka_ffnn* m, const int8_t* sigma, const void* h1, const void* d1, const void* g2,
+                      int64_t ns, const double* mean, double scale, int layout, int out_kind, void* out_a,
+                      void* out_b, int64_t ldo, void* stream) {
+  if (int st = check_model(m, "nka_ffnn_jacobian")) return st;
+  NKA_CHECK_ARG(ns > 0 && sigma && h1 && d1 && g2 && out_a, "nka_ffnn_jacobian: bad arguments");
+  NKA_CHECK_ARG(layout == NKA_LAYOUT_PS || layout == NKA_LAYOUT_SP, "nka_ffnn_jacobian: bad layout");
+  NKA_CHECK_ARG(out_kind == NKA_OUT_F64 || out_kind == NKA_OUT_F32 || (out_kind == NKA_OUT_BF16X2 && out_b),
+                "nka_ffnn_jacobian: bad output kind");
+  const int64_t np = ffnn_np(*m);
+  NKA_CHECK_ARG(ldo >= (layout == NKA_LAYOUT_PS ? ns : np), "nka_ffnn_jacobian: ldo too small");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t total = ns * np;
+  const unsigned g = (unsigned)std::min<int64_t>((total + 255) / 256, 1 << 16);
+  if (m->dtype == NKA_F32)
+    hipLaunchKernelGGL((ffnn_jacobian_kernel<float>), dim3(g), dim3(256), 0, s, *m, sigma, (const float*)h1,
+                       (const float*)d1, (const float*)g2, ns, np, mean, scale, layout, out_kind, out_a, out_b, ldo);
+  else
+    hipLaunchKernelGGL((ffnn_jacobian_kernel<double>), dim3(g), dim3(256), 0, s, *m, sigma, (const double*)h1,
+                       (const double*)d1, (const double*)g2, ns, np, mean, scale, layout, out_kind, out_a, out_b,
+                       ldo);
+  NKA_CHECK_LAUNCH("nka_ffnn_jacobian");
+  return NKA_OK;
+}
+
+}  // extern "C"

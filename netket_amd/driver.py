@@ -251,7 +251,7 @@ class VMC_SR(AbstractVariationalDriver):
                 raise NotImplementedError("SPRING momentum with the NTK is not supported")
             offset = 0.0 if proj_reg is None else float(proj_reg) / ns
             if structured:
-                system = solvers.DenseSystem(builder=lambda: eng.rbm_ntk(
+                system = solvers.DenseSystem(builder=lambda: eng.ntk_structured(
                     model, flat, sig, t, mean, 1.0 / ns, diag_shift, offset))
             else:
                 system = solvers.DenseSystem(builder=lambda: eng.gram(X, diag_shift, offset))

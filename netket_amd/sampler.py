@@ -177,11 +177,11 @@ class MetropolisSampler:
         return state
 
     def _check_machine(self, machine):
-        from .models import RBM
+        from .models import FFNN, RBM
 
-        if not isinstance(machine, RBM):
+        if not isinstance(machine, (RBM, FFNN)):
             raise NotImplementedError(
-                f"the GPU Metropolis kernels support the RBM ansatz, not {type(machine).__name__}")
+                f"the GPU Metropolis kernels support the RBM and FFNN ansatze, not {type(machine).__name__}")
 
     def sample_fused(self, machine, parameters: torch.Tensor, state: MetropolisSamplerState,
                      n_discard: int, chain_length: int, return_log_probabilities: bool = False):
@@ -206,7 +206,7 @@ class MetropolisSampler:
         else:
             cl_ptr, n_cl = C.c_void_p(0), 0
         with _ops.TIMER.region("metropolis"):
-            nat.call("nka_rbm_metropolis", C.byref(m), self.rule.kind, cl_ptr, n_cl,
+            nat.call(machine.metropolis_symbol, C.byref(m), self.rule.kind, cl_ptr, n_cl,
                      _ops.ptr(state.sigma), _ops.ptr(state.log_prob), self.n_chains,
                      self.sweep_size, int(n_discard), int(chain_length), float(self.machine_pow),
                      state.seed & (2**64 - 1), state.counter, _ops.ptr(state.counter_dev),

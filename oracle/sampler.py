@@ -72,8 +72,10 @@ def local_moves(x: np.ndarray):
 
 def metropolis(params, sigma0: np.ndarray, *, rule: str, clusters=None, n_discard: int,
                chain_length: int, sweep_size: int, seed: int, counter: int,
-               machine_pow: float = 2.0):
-    """Returns (samples (C, L, N), final sigma, n_accepted per chain)."""
+               machine_pow: float = 2.0, log_psi_fn=None):
+    """Returns (samples (C, L, N), final sigma, n_accepted per chain).  The
+    ansatz is the RBM of ``params`` unless ``log_psi_fn(params, x)`` is given."""
+    log_psi_ = log_psi if log_psi_fn is None else log_psi_fn
     sigma = np.array(sigma0, dtype=np.int8, copy=True)
     C, N = sigma.shape
     samples = np.empty((C, chain_length, N), dtype=np.int8)
@@ -82,7 +84,7 @@ def metropolis(params, sigma0: np.ndarray, *, rule: str, clusters=None, n_discar
         clusters = np.asarray(clusters, dtype=np.int64)
     for c in range(C):
         x = sigma[c].astype(np.int8).copy()
-        lp = machine_pow * log_psi(params, x[None])[0]
+        lp = machine_pow * log_psi_(params, x[None])[0]
         q = counter
         for sw in range(n_discard + chain_length):
             for _ in range(sweep_size):
@@ -93,7 +95,7 @@ def metropolis(params, sigma0: np.ndarray, *, rule: str, clusters=None, n_discar
                     continue
                 k = philox.bounded(r[0], len(moves))
                 xp, corr = moves[k], corrs[k]
-                lpp = machine_pow * log_psi(params, xp[None])[0]
+                lpp = machine_pow * log_psi_(params, xp[None])[0]
                 u = philox.uniform01_f64(r)
                 if u < np.exp(lpp - lp + corr):
                     x = xp
