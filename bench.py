@@ -49,6 +49,129 @@ def build_cfg2(seed=0):
     return drv, H, vs
 
 
+def build_j1j2_ffnn(seed=0, n_chains=8192):
+    """J1-J2 Heisenberg on the 10x10 periodic square lattice (J1=1, J2=0.5,
+    Sz=0, no sign rule), FP32 FFNN 100->400->400 (log_cosh, sum pooled,
+    N(0, 0.01^2) init), MetropolisExchange over the 200 NN + 200 NNN bonds,
+    16384 samples from 8192 chains (2 per chain, 5 discarded sweeps of 100),
+    minSR with the structured NTK (16384^2, FP64), diag_shift 1e-4, Cholesky."""
+    import numpy as np
+
+    import netket_amd as nk
+    from netket_amd.optimizer import solver
+
+    g = nk.graph.Square(10, max_neighbor_order=2)
+    hi = nk.hilbert.Spin(0.5, N=g.n_nodes, total_sz=0)
+    H = nk.operator.Heisenberg(hi, g, J=[1.0, 0.5], sign_rule=[False, False])
+    sa = nk.sampler.MetropolisExchange(hi, graph=g, n_chains=n_chains)
+    model = nk.models.FFNN(hidden_dims=(400, 400), param_dtype=np.float32)
+    vs = nk.vqs.MCState(sa, model, n_samples=16384, n_discard_per_chain=5, seed=seed)
+    drv = nk.VMC_SR(H, nk.optimizer.Sgd(0.01), diag_shift=1e-4, variational_state=vs, use_ntk=True,
+                    linear_solver=solver.cholesky)
+    return drv, H, vs
+
+
+def build_tfim2d_rbm(seed=0):
+    """2-D TFIM on the 20x20 periodic square lattice, J=1, h=3.04, FP32 RBM
+    alpha=8 (3200 hidden units, 1,283,600 parameters), MetropolisLocal 8192
+    chains, 32768 samples (4 per chain, 5 discarded sweeps of 400), minSR with
+    the structured NTK (32768^2, FP64), diag_shift 1e-4, Cholesky."""
+    import numpy as np
+
+    import netket_amd as nk
+    from netket_amd.optimizer import solver
+
+    g = nk.graph.Square(20)
+    hi = nk.hilbert.Spin(0.5, N=g.n_nodes)
+    H = nk.operator.Ising(hi, g, h=3.04, J=1.0)
+    sa = nk.sampler.MetropolisLocal(hi, n_chains=8192)
+    model = nk.models.RBM(alpha=8, param_dtype=np.float32)
+    vs = nk.vqs.MCState(sa, model, n_samples=32768, n_discard_per_chain=5, seed=seed)
+    drv = nk.VMC_SR(H, nk.optimizer.Sgd(0.01), diag_shift=1e-4, variational_state=vs, use_ntk=True,
+                    linear_solver=solver.cholesky)
+    return drv, H, vs
+
+
+def run_workload(name, steps, warmup):
+    """Time one of the larger workloads (eager steps) and print its own JSON line."""
+    import torch
+
+    from netket_amd import _ops
+
+    builders = {"j1j2-ffnn": (build_j1j2_ffnn, 16384,
+                              "j1j2-10x10 fp32-ffnn-100-400-400 metropolis-exchange-nn+nnn 8192ch "
+                              "16384smp minsr-structured-ntk-16384 fp64-cholesky"),
+                "tfim2d-rbm-a8": (build_tfim2d_rbm, 32768,
+                                  "tfim-20x20-h3.04 fp32-rbm-a8 metropolis-local 8192ch 32768smp "
+                                  "minsr-structured-ntk-32768 fp64-cholesky")}
+    build, ns, desc = builders[name]
+    drv, H, vs = build()
+    for _ in range(warmup):
+        step(drv)
+    torch.cuda.synchronize()
+    _ops.TIMER.reset()
+    _ops.TIMER.enabled = True
+    t0 = time.perf_counter()
+    stats = [step(drv) for _ in range(steps)]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    _ops.TIMER.enabled = False
+    stages = _ops.TIMER.summary()
+    out = {
+        "metric": "SR steps/s & samples/s",
+        "value": steps / dt,
+        "unit": "SR steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": dt / steps * 1e3,
+        "samples_per_s": steps * ns / dt,
+        "higher_is_better": True,
+        "dtype": "f32 model, FP64 NTK / Cholesky",
+        "data": "synthetic (random-init model, sampler-generated configurations)",
+        "config": {"workload": desc, "n_samples": ns, "n_params": vs.n_parameters},
+        "stages_ms_per_step": {k: round(v["total_ms"] / steps, 3) for k, v in sorted(
+            stages.items(), key=lambda kv: -kv[1]["total_ms"])},
+        "energy_per_step": [s.mean for s in stats],
+        "acceptance": vs.sampler_state.acceptance,
+    }
+    print(json.dumps(out))
+
+
+def run_serving(n_jobs, n_steps, rate=4.0):
+    """The serving workload: TFIM jobs of uneven size arriving as a Poisson
+    stream (rate jobs/s), each running n_steps SR / minSR steps, admitted by
+    HBM footprint and stepped as captured HIP graphs on per-job streams."""
+    from netket_amd import serving
+
+    specs = serving.make_stream(n_jobs=n_jobs, rate=rate, seed=0, n_steps=n_steps)
+    def progress(done, active, pending):
+        print(f"serving: {done} done, {active} active, {pending} pending", file=sys.stderr, flush=True)
+
+    rep = serving.Server(hbm_budget_gb=200.0, n_streams=8, use_graph=True, max_active=64).run(
+        specs, on_progress=progress)
+    out = {
+        "metric": "p50/p99 job latency & aggregate samples/s (serving)",
+        "value": rep.samples_per_s,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "job_latency_s_p50": rep.job_latency_s_p50,
+        "job_latency_s_p99": rep.job_latency_s_p99,
+        "step_latency_ms_p50": rep.step_latency_ms_p50,
+        "step_latency_ms_p99": rep.step_latency_ms_p99,
+        "steps_per_s": rep.steps_per_s,
+        "wall_s": rep.wall_s,
+        "max_concurrent": rep.max_concurrent,
+        "dtype": "f32 model, bf16x3 / FP64 NTK, FP64 Cholesky",
+        "data": "synthetic (random-init RBMs, sampler-generated configurations)",
+        "config": {"workload": f"serving tfim-1d/2d fp32-rbm-a{{1,2,4}} {n_jobs} jobs x {n_steps} steps, "
+                               f"poisson {rate}/s arrivals, samples/step in {{1024,4096,8192}}",
+                   "n_jobs": n_jobs, "n_steps": n_steps, "rate": rate},
+    }
+    print(json.dumps(out))
+
+
 def step(drv):
     drv.reset_step()
     st, dp = drv.compute_loss_and_update()
@@ -200,6 +323,12 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="skip the captured-graph replays")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
+    ap.add_argument("--workload", default="heisenberg-l64",
+                    choices=["heisenberg-l64", "j1j2-ffnn", "tfim2d-rbm-a8", "serving"],
+                    help="the default Heisenberg L=64 SR step, one of the larger minSR workloads, "
+                         "or the serving stream")
+    ap.add_argument("--serving-jobs", type=int, default=256)
+    ap.add_argument("--serving-steps", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -213,6 +342,12 @@ def main():
 
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.workload == "serving":
+        run_serving(args.serving_jobs, args.serving_steps)
+        return
+    if args.workload != "heisenberg-l64":
+        run_workload(args.workload, args.steps, args.warmup)
         return
     drv, H, vs = build_cfg2()
     for _ in range(args.warmup):

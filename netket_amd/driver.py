@@ -187,7 +187,11 @@ class VMC_SR(AbstractVariationalDriver):
         elif on_the_fly and not use_ntk:
             raise ValueError("`on_the_fly=True` is only supported when `use_ntk=True`.")
         if on_the_fly:
-            raise NotImplementedError("on-the-fly NTK is not supported; pass on_the_fly=False")
+            # T is formed from the model's layer structure without storing the
+            # Jacobian (netket/_src/ngd/srt_onthefly.py:31 computes the same T
+            # with vjp/jvp products)
+            if ntk_method != "structured":
+                raise ValueError("on_the_fly=True needs ntk_method='structured'")
         if momentum is not None and not use_ntk:
             pass
         self._ham = hamiltonian
@@ -195,7 +199,7 @@ class VMC_SR(AbstractVariationalDriver):
         self.proj_reg = proj_reg
         self.momentum = momentum
         self.use_ntk = bool(use_ntk)
-        self.on_the_fly = False
+        self.on_the_fly = bool(on_the_fly)
         self.linear_solver = linear_solver
         # NTK T from the RBM's structure (FP64, inner sizes n_hidden and n_visible)
         # or from the explicit Jacobian (bf16x3 / FP64 Gram over n_params)

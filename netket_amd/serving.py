@@ -165,7 +165,8 @@ class Server:
         self.use_graph = use_graph
         self.max_active = max_active
 
-    def run(self, specs, time_scale: float = 1.0, on_admit=None) -> ServingReport:
+    def run(self, specs, time_scale: float = 1.0, on_admit=None, on_progress=None,
+            progress_every_s: float = 10.0) -> ServingReport:
         pending = sorted(specs, key=lambda s: s.arrival)
         active = []
         used = 0
@@ -178,7 +179,11 @@ class Server:
         finals = {}
         params = {}
         k = 0
+        last_report = time.perf_counter()
         while pending or active:
+            if on_progress is not None and time.perf_counter() - last_report >= progress_every_s:
+                last_report = time.perf_counter()
+                on_progress(len(finished_at), len(active), len(pending))
             now = time.perf_counter() - t0
             while (pending and pending[0].arrival * time_scale <= now and len(active) < self.max_active
                    and used + footprint_bytes(pending[0]) <= self.budget):
