@@ -217,6 +217,15 @@ void set_error(const char* fmt, ...);
     }                                 \
   } while (0)
 
+#define NKA_CHECK_HIP(expr, what)                                      \
+  do {                                                                 \
+    hipError_t e_ = (expr);                                            \
+    if (e_ != hipSuccess) {                                            \
+      nka::set_error("%s: %s", what, hipGetErrorString(e_));           \
+      return NKA_ERR_HIP;                                              \
+    }                                                                  \
+  } while (0)
+
 #define NKA_CHECK_LAUNCH(what)                                               \
   do {                                                                       \
     hipError_t e_ = hipGetLastError();                                       \

@@ -267,17 +267,10 @@ constexpr int G2_K = 16;
 constexpr int G2_LD = G2_T + 4;   // doubles per k row of an LDS tile
 
 template <bool VEC>
-__global__ void __launch_bounds__(256, 2) gemm_nt_f64_big_kernel(const double* __restrict__ a, int64_t lda,
-                                                              const double* __restrict__ b, int64_t ldb,
-                                                              double* __restrict__ c, int64_t ldc, int64_t m,
-                                                              int64_t n, int64_t K, double alpha, double beta,
-                                                              int lower_only, int64_t tiles_n,
-                                                              int64_t n_tiles_total, const int32_t* info) {
-  if (info && *info != 0) return;
-  __shared__ double sA[G2_K * G2_LD];
-  __shared__ double sB[G2_K * G2_LD];
-  const int64_t bid = xcd_remap(blockIdx.x, gridDim.x);
-  if (bid >= n_tiles_total) return;
+__device__ inline void gemm_big_tile(const double* __restrict__ a, int64_t lda, const double* __restrict__ b,
+                                     int64_t ldb, double* __restrict__ c, int64_t ldc, int64_t m, int64_t n,
+                                     int64_t K, double alpha, double beta, int lower_only, int64_t tiles_n,
+                                     int64_t bid, double* sA, double* sB) {
   int ti, tj;
   if (lower_only) {
     tri_supertile(bid, tiles_n, ti, tj);
@@ -359,12 +352,32 @@ __global__ void __launch_bounds__(256, 2) gemm_nt_f64_big_kernel(const double* _
       }
 }
 
+// One tile per workgroup (a grid smaller than the tile count makes each
+// workgroup walk tiles bid, bid + G, ..., consecutive in the super-block order).
+template <bool VEC>
+__global__ void __launch_bounds__(256, 2) gemm_nt_f64_big_kernel(const double* __restrict__ a, int64_t lda,
+                                                              const double* __restrict__ b, int64_t ldb,
+                                                              double* __restrict__ c, int64_t ldc, int64_t m,
+                                                              int64_t n, int64_t K, double alpha, double beta,
+                                                              int lower_only, int64_t tiles_n,
+                                                              int64_t n_tiles_total, const int32_t* info) {
+  if (info && *info != 0) return;
+  __shared__ double sA[G2_K * G2_LD];
+  __shared__ double sB[G2_K * G2_LD];
+  const int64_t G = gridDim.x;
+  for (int64_t t = xcd_remap(blockIdx.x, G); t < n_tiles_total; t += G) {
+    gemm_big_tile<VEC>(a, lda, b, ldb, c, ldc, m, n, K, alpha, beta, lower_only, tiles_n, t, sA, sB);
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Cholesky building blocks (block size 64).
 // ---------------------------------------------------------------------------
 constexpr int CH_NB = 64;
 
 constexpr int CH_LT = CH_NB + 2;  // LDS row stride of the transposed factor (16-byte rows)
+constexpr int TS_LD = CH_NB + 1;  // LDS row stride of a 64x64 block (doubles)
 
 // Lane `src`'s value of v in every lane (src a compile-time constant after
 // unrolling).
@@ -385,100 +398,175 @@ __device__ inline double rsqrt_refined(double a) {
 }
 
 // Factor the diagonal block A[k0:k0+nb, k0:k0+nb] in place (lower triangle)
-// with one wavefront: lane i keeps row i in registers, right-looking.  The
-// pivot of step j comes from lane j by readlane (the short dependency chain);
-// the unscaled column j is broadcast through LDS for the update
-// a_c -= L_ij L_cj = a_ij a_cj / a_jj.  The factor is also written
-// transposed and packed to lt (lt[c*64 + r] = L[r][c]) with the reciprocal
-// diagonal in rd, for the panel solve.  A non-positive or non-finite pivot
-// sets *info (first failing column + 1) and is replaced by 1 so the kernel
-// completes.  Rows/columns past nb are an identity padding.
+// with one wavefront, lane i = row i, by 16-column sub-panels:
+//  * sub-panel p (columns 16p..16p+15, rows 16p..63) is factored
+//    right-looking with the 16 values of each row in registers; the unscaled
+//    column of each step is broadcast through LDS and the update is
+//    a_ic -= L_ij L_cj = a_ij a_cj / a_jj, with 1/sqrt(a_jj) from the
+//    hardware estimate and two Newton steps;
+//  * the block's remaining lower part (rows and columns >= 16(p+1)) then
+//    receives the rank-16 update on FP64 MFMA (v_mfma_f64_16x16x4).
+// The factor is also written transposed and packed to lt
+// (lt[c*64 + r] = L[r][c]) with the reciprocal diagonal in rd, for the panel
+// solve.  A non-positive or non-finite pivot sets *info (first failing column
+// + 1) and is replaced by 1 so the kernel completes.  Rows/columns past nb are
+// an identity padding.
 __global__ void __launch_bounds__(64) potf2_kernel(double* __restrict__ A, int64_t lda, int64_t n,
                                                    int64_t k0, double* __restrict__ lt,
                                                    double* __restrict__ rd, int32_t* __restrict__ info) {
   if (*info != 0) return;
+  __shared__ double Ab[CH_NB * TS_LD];   // the block, row-major
   __shared__ double colbuf[CH_NB];
-  __shared__ double blk[CH_NB * CH_LT];
+  __shared__ double yv[CH_NB];
   const int lane = threadIdx.x;
   const int nb = (int)min<int64_t>(CH_NB, n - k0);
-  // stage the (identity-padded) lower triangle through LDS with coalesced reads
   for (int idx = lane; idx < CH_NB * CH_NB; idx += 64) {
     const int r = idx >> 6, c = idx & 63;
     double v = (r == c) ? 1.0 : 0.0;
     if (r < nb && c < nb && c <= r) v = A[(k0 + r) * lda + k0 + c];
-    blk[r * CH_LT + c] = v;
+    Ab[r * TS_LD + c] = v;
   }
   wave_lds_sync();
-  double a[CH_NB];
-#pragma unroll
-  for (int c = 0; c < CH_NB; ++c) a[c] = blk[lane * CH_LT + c];
   int bad = 0;
-  double myrd = 1.0;
 #pragma unroll
-  for (int j = 0; j < CH_NB; ++j) {
-    colbuf[lane] = a[j];
-    const double ajj = readlane_f64(a[j], j);
-    const bool ok = (ajj > 0.0) && isfinite(ajj);
-    if (!ok && bad == 0) bad = (int)(k0 + j + 1);
-    const double piv = ok ? ajj : 1.0;
-    const double y = rsqrt_refined(piv);
-    const double lij = (lane > j) ? a[j] * y : (lane == j ? piv * y : 0.0);
-    const double sij = (lane > j) ? lij * y : 0.0;
-    if (lane == j) myrd = y;
-    a[j] = lij;
+  for (int p = 0; p < 4; ++p) {
+    double x[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) x[q] = Ab[lane * TS_LD + 16 * p + q];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int jj = 16 * p + j;
+      colbuf[lane] = x[j];
+      wave_lds_sync();
+      const double ajj = colbuf[jj];
+      const bool ok = (ajj > 0.0) && isfinite(ajj);
+      if (!ok && bad == 0) bad = (int)(k0 + jj + 1);
+      const double piv = ok ? ajj : 1.0;
+      const double y = rsqrt_refined(piv);
+      const double sij = (lane > jj) ? x[j] * (y * y) : 0.0;
+      x[j] = (lane > jj) ? x[j] * y : (lane == jj ? piv * y : 0.0);
+      if (lane == jj) yv[jj] = y;
+#pragma unroll
+      for (int q = j + 1; q < 16; ++q) x[q] -= sij * colbuf[16 * p + q];
+      wave_lds_sync();
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) Ab[lane * TS_LD + 16 * p + q] = (16 * p + q <= lane) ? x[q] : 0.0;
     wave_lds_sync();
+    // rank-16 update of the lower blocks (bi, bj), p < bj <= bi < 4
 #pragma unroll
-    for (int c = j + 1; c < CH_NB; ++c) a[c] -= sij * colbuf[c];
+    for (int bi = p + 1; bi < 4; ++bi)
+#pragma unroll
+      for (int bj = p + 1; bj <= bi; ++bj) {
+        doublex4 acc = (doublex4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kb = 0; kb < 16; kb += 4) {
+          const int kk = 16 * p + kb + (lane >> 4);
+          const double av = Ab[(bi * 16 + (lane & 15)) * TS_LD + kk];
+          const double bv = Ab[(bj * 16 + (lane & 15)) * TS_LD + kk];
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+        }
+        // element e of lane l is (row l/16 + 4e, col l%16) of the 16x16 block
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int r = bi * 16 + (lane >> 4) + 4 * e, c = bj * 16 + (lane & 15);
+          Ab[r * TS_LD + c] -= acc[e];
+        }
+      }
     wave_lds_sync();
   }
-  if (lane < nb) {
-    double* wrow = A + (k0 + lane) * lda + k0;
-#pragma unroll
-    for (int c = 0; c < CH_NB; ++c)
-      if (c <= lane && c < nb) wrow[c] = a[c];
+  for (int idx = lane; idx < CH_NB * CH_NB; idx += 64) {
+    const int r = idx >> 6, c = idx & 63;
+    const double v = Ab[r * TS_LD + c];
+    if (r < nb && c < nb && c <= r) A[(k0 + r) * lda + k0 + c] = v;
+    lt[c * CH_NB + r] = (c <= r) ? v : 0.0;
   }
-#pragma unroll
-  for (int c = 0; c < CH_NB; ++c) lt[c * CH_NB + lane] = (c <= lane) ? a[c] : 0.0;
-  rd[lane] = myrd;
+  rd[lane] = yv[lane];
   if (lane == 0 && bad) atomicCAS(info, 0, bad);
 }
 
 // Panel solve L21 = A21 L11^{-T} in place for the rows below the diagonal
-// block k0 (row k0+64+64 b+t in lane t of workgroup b), each lane
-// substituting its row column by column (x_j = (a_j - sum_{c<j} x_c L_jc) / L_jj).
-// L11^T (lt, packed by the factor kernel) is staged in LDS and read at
-// wave-uniform addresses; one wavefront per workgroup keeps the LDS read
-// bandwidth of a CU for that wave.  Rows exist below a block only when the
-// block is a full 64 columns wide, so the row loads are unmasked.
-__global__ void __launch_bounds__(64) trsm_panel_kernel(double* __restrict__ A, int64_t lda, int64_t n,
-                                                        int64_t k0, const double* __restrict__ lt,
-                                                        const double* __restrict__ rd,
-                                                        const int32_t* __restrict__ info) {
+// block k0.  Each wavefront owns 64 rows (row k0+64+128 b+64 w+lane) held in
+// LDS, and solves them by 16-column blocks: block p first receives
+// X_p -= X_{<p} L_{p,<p}^T on FP64 MFMA (v_mfma_f64_16x16x4), then the 16x16
+// triangular solve X_p <- X_p L_pp^{-T} by substitution, one row per lane with
+// 16 values in registers.  L11 (from lt = L11^T packed by the factor kernel)
+// and 1/diag are staged in LDS.  Rows exist below a block only when the block
+// is a full 64 columns wide, so every column load is in bounds; rows past n
+// re-read row k0+64 and are not stored.
+
+__global__ void __launch_bounds__(128) trsm_panel_kernel(double* __restrict__ A, int64_t lda, int64_t n,
+                                                         int64_t k0, const double* __restrict__ lt,
+                                                         const double* __restrict__ rd,
+                                                         const int32_t* __restrict__ info) {
   if (*info != 0) return;
-  __shared__ double LT[CH_NB * CH_NB];
+  __shared__ double Ls[CH_NB * TS_LD];        // Ls[r * TS_LD + c] = L11[r][c]
   __shared__ double rdiag[CH_NB];
-  const int lane = threadIdx.x;
-#pragma unroll 4
-  for (int q = lane; q < CH_NB * CH_NB / 2; q += 64)
-    reinterpret_cast<double2*>(LT)[q] = reinterpret_cast<const double2*>(lt)[q];
-  rdiag[lane] = rd[lane];
-  const int64_t row = k0 + CH_NB + (int64_t)blockIdx.x * CH_NB + lane;
-  const bool live = row < n;
-  double* ar = A + (live ? row : k0 + CH_NB) * lda + k0;
-  double x[CH_NB];
-#pragma unroll
-  for (int c = 0; c < CH_NB; ++c) x[c] = ar[c];
-  wave_lds_sync();
-#pragma unroll
-  for (int j = 0; j < CH_NB; ++j) {
-    const double xj = x[j] * rdiag[j];
-    x[j] = xj;
-#pragma unroll
-    for (int c = j + 1; c < CH_NB; ++c) x[c] -= xj * LT[j * CH_NB + c];
+  __shared__ double Xs[2][CH_NB * TS_LD];     // one 64x64 row tile per wavefront
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+#pragma unroll 8
+  for (int idx = tid; idx < CH_NB * CH_NB; idx += 128) {
+    const int c = idx >> 6, r = idx & 63;     // lt[c*64 + r] = L[r][c]
+    Ls[r * TS_LD + c] = lt[idx];
   }
-  if (live) {
+  if (tid < CH_NB) rdiag[tid] = rd[tid];
+  double* X = Xs[w];
+  const int64_t r0 = k0 + CH_NB + (int64_t)blockIdx.x * 128 + 64 * w;
+  // coalesced tile load: lanes walk the columns of one row at a time; 16 rows
+  // per batch so the loads are in flight together
 #pragma unroll
-    for (int c = 0; c < CH_NB; ++c) ar[c] = x[c];
+  for (int r0b = 0; r0b < CH_NB; r0b += 16) {
+    double v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = A[min<int64_t>(r0 + r0b + r, n - 1) * lda + k0 + lane];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) X[(r0b + r) * TS_LD + lane] = v[r];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    if (p > 0) {
+      // X[:, 16p..16p+16) -= X[:, 0..16p) L[16p..16p+16, 0..16p)^T
+      doublex4 acc[4];
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb) acc[rb] = (doublex4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kb = 0; kb < 16 * p; kb += 4) {
+        const int kk = kb + (lane >> 4);
+        const double bv = Ls[(16 * p + (lane & 15)) * TS_LD + kk];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+          const double av = X[(rb * 16 + (lane & 15)) * TS_LD + kk];
+          acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[rb], 0, 0, 0);
+        }
+      }
+      // element e of lane l in a 16x16 block is (row l/16 + 4e, col l%16)
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int r = rb * 16 + (lane >> 4) + 4 * e, c = 16 * p + (lane & 15);
+          X[r * TS_LD + c] -= acc[rb][e];
+        }
+      wave_lds_sync();
+    }
+    // X[lane, 16p..16p+16) <- X[lane, 16p..) L_pp^{-T} (substitution)
+    double x[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) x[q] = X[lane * TS_LD + 16 * p + q];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      x[j] *= rdiag[16 * p + j];
+#pragma unroll
+      for (int q = j + 1; q < 16; ++q) x[q] -= x[j] * Ls[(16 * p + q) * TS_LD + 16 * p + j];
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) X[lane * TS_LD + 16 * p + q] = x[q];
+    wave_lds_sync();
+  }
+#pragma unroll 16
+  for (int r = 0; r < CH_NB; ++r) {
+    if (r0 + r < n) A[(r0 + r) * lda + k0 + lane] = X[r * TS_LD + lane];
   }
 }
 
@@ -876,11 +964,12 @@ static void gemm_big_launch(const double* a, int64_t lda, const double* b, int64
   const int64_t tm = (m + G2_T - 1) / G2_T, tn = (n + G2_T - 1) / G2_T;
   const int64_t tiles = lower ? tm * (tm + 1) / 2 : tm * tn;
   const bool vec = ((uintptr_t)a % 16 == 0) && ((uintptr_t)b % 16 == 0) && lda % 2 == 0 && ldb % 2 == 0;
+  const unsigned grid = grid_mult8(tiles);
   if (vec)
-    hipLaunchKernelGGL(gemm_nt_f64_big_kernel<true>, dim3(grid_mult8(tiles)), dim3(256), 0, s, a, lda, b,
+    hipLaunchKernelGGL(gemm_nt_f64_big_kernel<true>, dim3(grid), dim3(256), 0, s, a, lda, b,
                        ldb, c, ldc, m, n, k, alpha, beta, lower, tn, tiles, info);
   else
-    hipLaunchKernelGGL(gemm_nt_f64_big_kernel<false>, dim3(grid_mult8(tiles)), dim3(256), 0, s, a, lda, b,
+    hipLaunchKernelGGL(gemm_nt_f64_big_kernel<false>, dim3(grid), dim3(256), 0, s, a, lda, b,
                        ldb, c, ldc, m, n, k, alpha, beta, lower, tn, tiles, info);
 }
 
@@ -921,11 +1010,15 @@ static inline uint32_t* ws_flags(void* ws, int64_t n, int64_t nblk) {
   return (uint32_t*)(ws_vec(ws, nblk) + n + 64);
 }
 
+// A second stream per device for the trailing updates of the Cholesky
+// (lookahead), with the two events that order it against the caller's stream.
+// Created on the first call that is not inside a graph capture; default
+// priority.  Without it the factorisation runs on the caller's stream alone.
 // Right-looking Cholesky with two levels of blocking: 256-column panels are
-// factored by 64-column steps (diagonal block, panel solve by substitution,
-// then the update of the panel's remaining columns as an FP64 MFMA product),
-// then the trailing matrix receives one rank-256 update.  The inverses of the
-// diagonal blocks, used by the triangular solves, are formed at the end.
+// factored by 64-column steps (diagonal block, panel solve, then the update
+// of the panel's remaining columns as an FP64 MFMA product), then the
+// trailing matrix receives one rank-256 update.  The inverses of the diagonal
+// blocks, used by the triangular solves, are formed at the end.
 int nka_potrf_f64(double* a, int64_t n, int64_t lda, void* workspace, int32_t* info,
                   void* stream) {
   NKA_CHECK_ARG(n >= 0 && lda >= n && a && workspace && info, "nka_potrf_f64: bad arguments");
@@ -945,8 +1038,8 @@ int nka_potrf_f64(double* a, int64_t n, int64_t lda, void* workspace, int32_t* i
       double* rd = ws_vec(workspace, nblk);
       hipLaunchKernelGGL(potf2_kernel, dim3(1), dim3(64), 0, s, a, lda, n, k0, lt, rd, info);
       if (rest <= 0) continue;  // rows below imply nb == 64
-      const unsigned grid = (unsigned)((rest + CH_NB - 1) / CH_NB);
-      hipLaunchKernelGGL(trsm_panel_kernel, dim3(grid), dim3(64), 0, s, a, lda, n, k0, lt, rd, info);
+      const unsigned grid = (unsigned)((rest + 127) / 128);
+      hipLaunchKernelGGL(trsm_panel_kernel, dim3(grid), dim3(128), 0, s, a, lda, n, k0, lt, rd, info);
       // update of the remaining columns of this panel
       const int64_t cols = K0 + nbo - (k0 + nb);
       if (cols > 0) {
