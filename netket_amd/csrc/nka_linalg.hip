@@ -624,9 +624,12 @@ __global__ void zero_u32_kernel(uint32_t* __restrict__ p, int64_t n) {
 // ---------------------------------------------------------------------------
 constexpr int64_t kSpinLimit = 1LL << 26;
 
+// Polls with relaxed agent-scope loads (they read the L2 copy without
+// invalidating the CU's L1, which other waves on the CU are streaming
+// through); the caller issues one acquire fence after the flag is seen.
 __device__ inline void wait_flag(const uint32_t* flag, uint32_t want, int32_t* err) {
   int64_t spins = 0;
-  while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != want) {
+  while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != want) {
     __builtin_amdgcn_s_sleep(1);
     if (++spins > kSpinLimit) {
       atomicOr(err, 1);
