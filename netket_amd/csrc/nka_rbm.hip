@@ -475,11 +475,108 @@ static void launch_eloc_f64(const nka_rbm& m, const OpArgs& op, const int8_t* sg
   launch_eloc<JL, double>(m, op, sg, n, e, (double*)lp, s);
 }
 
+// theta = sigma W + b for models whose W does not fit in LDS, as a tiled
+// matrix product: 64 samples x 64 hidden units per workgroup, K = N in steps
+// of 16 through LDS (sigma as exact +-1), so W is read once per 64 samples
+// instead of once per sample.  FP32 on v_mfma_f32_32x32x2f32 (one 32x32
+// block per wave), FP64 on v_mfma_f64_16x16x4f64 (2x2 blocks per wave).
+// The epilogue writes t = tanh(theta) to t (ns, M) and/or tT (M, ns).
+typedef float ffx16 __attribute__((ext_vector_type(16)));
+typedef double ddx4 __attribute__((ext_vector_type(4)));
+constexpr int PG_T = 64, PG_K = 16;
+
+template <class T>
+__global__ void __launch_bounds__(256) rbm_prepare_gemm_kernel(nka_rbm m, const int8_t* __restrict__ sigma,
+                                                               int64_t ns, T* __restrict__ t, T* __restrict__ tT) {
+  __shared__ T sA[PG_T][PG_K + 1];
+  __shared__ T sB[PG_K][PG_T + 1];
+  const RbmView<T> v = rbm_view<T>(m);
+  const int N = v.N, M = v.M;
+  const int64_t s0 = (int64_t)blockIdx.x * PG_T;
+  const int j0 = blockIdx.y * PG_T;
+  const int tid = threadIdx.x, wid = tid >> 6, lane = lane_id();
+  const int wm = (wid >> 1) * 32, wn = (wid & 1) * 32;
+  ffx16 accf;
+  ddx4 accd[2][2];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) accf[e] = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) accd[a][b] = (ddx4){0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < N; k0 += PG_K) {
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+      const int idx = tid + it * 256;
+      const int r = idx >> 4, kc = idx & 15;            // A: 64 rows x 16 k
+      const int kk = k0 + kc;
+      T va = (T)0;
+      if (s0 + r < ns && kk < N) va = sigma[(s0 + r) * N + kk] > 0 ? (T)1 : (T)-1;
+      sA[r][kc] = va;
+      const int kr = idx >> 6, c = idx & 63;            // B: 16 k x 64 cols
+      T vb = (T)0;
+      if (k0 + kr < N && j0 + c < M) vb = v.W[(int64_t)(k0 + kr) * M + j0 + c];
+      sB[kr][c] = vb;
+    }
+    __syncthreads();
+    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+      for (int k = 0; k < PG_K; k += 2) {
+        const float a = sA[wm + (lane & 31)][k + (lane >> 5)];
+        const float b = sB[k + (lane >> 5)][wn + (lane & 31)];
+        accf = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, accf, 0, 0, 0);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < PG_K; k += 4) {
+        double av[2], bv[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) av[a] = sA[wm + a * 16 + (lane & 15)][k + (lane >> 4)];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) bv[b] = sB[k + (lane >> 4)][wn + b * 16 + (lane & 15)];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b)
+            accd[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], accd[a][b], 0, 0, 0);
+      }
+    }
+  }
+  auto store = [&](int64_t sr, int j, T th) {
+    if (sr < ns && j < M) {
+      const T tv = tanh_t<T>(th + (v.b ? v.b[j] : (T)0));
+      if (t) t[sr * M + j] = tv;
+      if (tT) tT[(int64_t)j * ns + sr] = tv;
+    }
+  };
+  if constexpr (sizeof(T) == 4) {
+    // element e of lane l: row (e/4)*8 + (l/32)*4 + e%4, col l%32
+#pragma unroll
+    for (int e = 0; e < 16; ++e)
+      store(s0 + wm + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3), j0 + wn + (lane & 31), (T)accf[e]);
+  } else {
+    // element e of lane l in a 16x16 block: row l/16 + 4e, col l%16
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          store(s0 + wm + a * 16 + (lane >> 4) + 4 * e, j0 + wn + b * 16 + (lane & 15), (T)accd[a][b][e]);
+  }
+}
+
 template <int JL, class T>
 static void launch_prepare(const nka_rbm& m, const int8_t* sigma, int64_t ns, T* t, T* tT,
                            hipStream_t s) {
   const int waves = 4;
   const bool stage = stage_fits<T>(m, waves);
+  if (!stage) {
+    dim3 grid((unsigned)((ns + PG_T - 1) / PG_T), (unsigned)((m.n_hidden + PG_T - 1) / PG_T));
+    hipLaunchKernelGGL((rbm_prepare_gemm_kernel<T>), grid, dim3(256), 0, s, m, sigma, ns, t, tT);
+    return;
+  }
   const size_t lds = (stage ? pad16((size_t)m.n_visible * m.n_hidden * sizeof(T)) : 0) +
                      waves * pad16(m.n_visible);
   const unsigned g = grid_for(ns, waves, 2048);

@@ -139,6 +139,24 @@ def test_rbm_ntk_structured(gpu, dtype, ns, biases):
     assert np.all(err <= bound)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_prepare_large_model(gpu, dtype):
+    # W larger than the LDS budget takes the tiled matrix-product path
+    print(f"case prepare large {np.dtype(dtype)}", flush=True)
+    N, alpha, ns = 70, 5, 333
+    model = nk.models.RBM(alpha=alpha, param_dtype=dtype)
+    M = model.n_hidden(N)
+    flat = np.random.default_rng(3).normal(scale=0.2, size=model.layout(N)["n_params"]).astype(dtype)
+    x = np.random.default_rng(4).choice(np.array([-1, 1], dtype=np.int8), size=(ns, N))
+    t, tT = eng.prepare(model, torch.as_tensor(flat, device="cuda"), torch.as_tensor(x, device="cuda"),
+                        row_major=True, transposed=True)
+    p = orbm.RBMParams(flat.astype(np.float64), N, M)
+    ref = np.tanh(orbm.theta(p, x))
+    tol = 2e-6 if dtype == np.float32 else 1e-13
+    np.testing.assert_allclose(t.cpu().numpy(), ref, rtol=0, atol=tol)
+    np.testing.assert_allclose(tT.cpu().numpy(), ref.T, rtol=0, atol=tol)
+
+
 @pytest.mark.parametrize("n", [1, 63, 64, 200, 333, 777, 1500])
 def test_cholesky_solve(gpu, n):
     print(f"case cholesky n={n}", flush=True)
