@@ -397,8 +397,8 @@ __device__ inline double rsqrt_refined(double a) {
   return y;
 }
 
-// Factor the diagonal block A[k0:k0+nb, k0:k0+nb] in place (lower triangle)
-// with one wavefront, lane i = row i, by 16-column sub-panels:
+// Factor a 64x64 block held in LDS (Ab, row-major, zeros above the diagonal)
+// in place with one wavefront, lane i = row i, by 16-column sub-panels:
 //  * sub-panel p (columns 16p..16p+15, rows 16p..63) is factored
 //    right-looking with the 16 values of each row in registers; the unscaled
 //    column of each step is broadcast through LDS and the update is
@@ -406,27 +406,11 @@ __device__ inline double rsqrt_refined(double a) {
 //    hardware estimate and two Newton steps;
 //  * the block's remaining lower part (rows and columns >= 16(p+1)) then
 //    receives the rank-16 update on FP64 MFMA (v_mfma_f64_16x16x4).
-// The factor is also written transposed and packed to lt
-// (lt[c*64 + r] = L[r][c]) with the reciprocal diagonal in rd, for the panel
-// solve.  A non-positive or non-finite pivot sets *info (first failing column
-// + 1) and is replaced by 1 so the kernel completes.  Rows/columns past nb are
-// an identity padding.
-__global__ void __launch_bounds__(64) potf2_kernel(double* __restrict__ A, int64_t lda, int64_t n,
-                                                   int64_t k0, double* __restrict__ lt,
-                                                   double* __restrict__ rd, int32_t* __restrict__ info) {
-  if (*info != 0) return;
-  __shared__ double Ab[CH_NB * TS_LD];   // the block, row-major
-  __shared__ double colbuf[CH_NB];
-  __shared__ double yv[CH_NB];
-  const int lane = threadIdx.x;
-  const int nb = (int)min<int64_t>(CH_NB, n - k0);
-  for (int idx = lane; idx < CH_NB * CH_NB; idx += 64) {
-    const int r = idx >> 6, c = idx & 63;
-    double v = (r == c) ? 1.0 : 0.0;
-    if (r < nb && c < nb && c <= r) v = A[(k0 + r) * lda + k0 + c];
-    Ab[r * TS_LD + c] = v;
-  }
-  wave_lds_sync();
+// On return Ab holds L (exact zeros above the diagonal) and yv[j] = 1/L_jj.
+// A non-positive or non-finite pivot is replaced by 1 so the factor
+// completes; the return value is then the first failing column + 1 + k0
+// (0 when every pivot is positive).
+__device__ int factor_block_wave(double* Ab, double* colbuf, double* yv, int lane, int64_t k0) {
   int bad = 0;
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
@@ -475,54 +459,15 @@ __global__ void __launch_bounds__(64) potf2_kernel(double* __restrict__ A, int64
       }
     wave_lds_sync();
   }
-  for (int idx = lane; idx < CH_NB * CH_NB; idx += 64) {
-    const int r = idx >> 6, c = idx & 63;
-    const double v = Ab[r * TS_LD + c];
-    if (r < nb && c < nb && c <= r) A[(k0 + r) * lda + k0 + c] = v;
-    lt[c * CH_NB + r] = (c <= r) ? v : 0.0;
-  }
-  rd[lane] = yv[lane];
-  if (lane == 0 && bad) atomicCAS(info, 0, bad);
+  return bad;
 }
 
-// Panel solve L21 = A21 L11^{-T} in place for the rows below the diagonal
-// block k0.  Each wavefront owns 64 rows (row k0+64+128 b+64 w+lane) held in
-// LDS, and solves them by 16-column blocks: block p first receives
+// Solve the 64 rows of X (LDS, row-major) in place, X <- X L^{-T}, with one
+// wavefront by 16-column blocks: block p first receives
 // X_p -= X_{<p} L_{p,<p}^T on FP64 MFMA (v_mfma_f64_16x16x4), then the 16x16
 // triangular solve X_p <- X_p L_pp^{-T} by substitution, one row per lane with
-// 16 values in registers.  L11 (from lt = L11^T packed by the factor kernel)
-// and 1/diag are staged in LDS.  Rows exist below a block only when the block
-// is a full 64 columns wide, so every column load is in bounds; rows past n
-// re-read row k0+64 and are not stored.
-
-__global__ void __launch_bounds__(128) trsm_panel_kernel(double* __restrict__ A, int64_t lda, int64_t n,
-                                                         int64_t k0, const double* __restrict__ lt,
-                                                         const double* __restrict__ rd,
-                                                         const int32_t* __restrict__ info) {
-  if (*info != 0) return;
-  __shared__ double Ls[CH_NB * TS_LD];        // Ls[r * TS_LD + c] = L11[r][c]
-  __shared__ double rdiag[CH_NB];
-  __shared__ double Xs[2][CH_NB * TS_LD];     // one 64x64 row tile per wavefront
-  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-#pragma unroll 8
-  for (int idx = tid; idx < CH_NB * CH_NB; idx += 128) {
-    const int c = idx >> 6, r = idx & 63;     // lt[c*64 + r] = L[r][c]
-    Ls[r * TS_LD + c] = lt[idx];
-  }
-  if (tid < CH_NB) rdiag[tid] = rd[tid];
-  double* X = Xs[w];
-  const int64_t r0 = k0 + CH_NB + (int64_t)blockIdx.x * 128 + 64 * w;
-  // coalesced tile load: lanes walk the columns of one row at a time; 16 rows
-  // per batch so the loads are in flight together
-#pragma unroll
-  for (int r0b = 0; r0b < CH_NB; r0b += 16) {
-    double v[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) v[r] = A[min<int64_t>(r0 + r0b + r, n - 1) * lda + k0 + lane];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) X[(r0b + r) * TS_LD + lane] = v[r];
-  }
-  __syncthreads();
+// 16 values in registers.  Ls = L (row-major, LDS), rdiag[j] = 1/L_jj.
+__device__ void solve_rows_wave(const double* Ls, const double* rdiag, double* X, int lane) {
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
     if (p > 0) {
@@ -564,10 +509,184 @@ __global__ void __launch_bounds__(128) trsm_panel_kernel(double* __restrict__ A,
     for (int q = 0; q < 16; ++q) X[lane * TS_LD + 16 * p + q] = x[q];
     wave_lds_sync();
   }
+}
+
+// Factor the diagonal block A[k0:k0+nb, k0:k0+nb] in place (lower triangle)
+// with one wavefront; used for the last block, which has no rows below it.
+// A non-positive or non-finite pivot sets *info (first failing column + 1).
+// Rows/columns past nb are an identity padding.
+__global__ void __launch_bounds__(64) potf2_kernel(double* __restrict__ A, int64_t lda, int64_t n,
+                                                   int64_t k0, int32_t* __restrict__ info) {
+  if (*info != 0) return;
+  __shared__ double Ab[CH_NB * TS_LD];   // the block, row-major
+  __shared__ double colbuf[CH_NB];
+  __shared__ double yv[CH_NB];
+  const int lane = threadIdx.x;
+  const int nb = (int)min<int64_t>(CH_NB, n - k0);
+  if (nb == CH_NB) {
+    // lane = column; all 64 row loads are issued before the first LDS store
+    const double* pc = A + k0 * lda + k0 + lane;
+    double v[CH_NB];
+#pragma unroll
+    for (int r = 0; r < CH_NB; ++r) v[r] = pc[r * lda];
+#pragma unroll
+    for (int r = 0; r < CH_NB; ++r) Ab[r * TS_LD + lane] = lane <= r ? v[r] : 0.0;
+  } else {
+    for (int idx = lane; idx < CH_NB * CH_NB; idx += 64) {
+      const int r = idx >> 6, c = idx & 63;
+      double v = (r == c) ? 1.0 : 0.0;
+      if (r < nb && c < nb && c <= r) v = A[(k0 + r) * lda + k0 + c];
+      Ab[r * TS_LD + c] = v;
+    }
+  }
+  wave_lds_sync();
+  const int bad = factor_block_wave(Ab, colbuf, yv, lane, k0);
 #pragma unroll 16
+  for (int r = 0; r < nb; ++r) {
+    const double v = Ab[r * TS_LD + lane];
+    if (lane < nb && lane <= r) A[(k0 + r) * lda + k0 + lane] = v;
+  }
+  if (lane == 0 && bad) atomicCAS(info, 0, bad);
+}
+
+// One 64-column step of the factorisation for a block with rows below it:
+// factor the diagonal block A[k0:k0+64, k0:k0+64] and solve the rows below,
+// L21 = A21 L11^{-T}, in place.  Every workgroup (two wavefronts, 128 rows)
+// reads the diagonal block and factors it in LDS itself (wavefront 0), so the
+// solve needs no second launch and no exchange through global memory;
+// workgroup 0 writes the factored block back and reports a failed pivot in
+// *info.  All global loads of a workgroup are issued before its first LDS
+// store.  Rows past n re-read the last row and are not stored.
+__global__ void __launch_bounds__(128) panel_factor_kernel(double* __restrict__ A, int64_t lda, int64_t n,
+                                                           int64_t k0, int32_t* __restrict__ info) {
+  if (*info != 0) return;
+  __shared__ double Ls[CH_NB * TS_LD];        // the diagonal block, then L11 (row-major)
+  __shared__ double colbuf[CH_NB];
+  __shared__ double rdiag[CH_NB];
+  __shared__ double Xs[2][CH_NB * TS_LD];     // one 64x64 row tile per wavefront
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  double* X = Xs[w];
+  const int64_t r0 = k0 + CH_NB + (int64_t)blockIdx.x * 128 + 64 * w;
+  {
+    // wavefront w reads rows 32w..32w+31 of the diagonal block and its own 64
+    // rows, lanes along the columns
+    double vd[32];
+    const double* pd = A + (k0 + 32 * w) * lda + k0 + lane;
+#pragma unroll
+    for (int r = 0; r < 32; ++r) vd[r] = pd[r * lda];
+    double v[CH_NB];
+#pragma unroll
+    for (int r = 0; r < CH_NB; ++r) v[r] = A[min<int64_t>(r0 + r, n - 1) * lda + k0 + lane];
+#pragma unroll
+    for (int r = 0; r < 32; ++r) {
+      const int R = 32 * w + r;
+      Ls[R * TS_LD + lane] = lane <= R ? vd[r] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < CH_NB; ++r) X[r * TS_LD + lane] = v[r];
+  }
+  __syncthreads();
+  if (w == 0) {
+    const int bad = factor_block_wave(Ls, colbuf, rdiag, lane, k0);
+    if (blockIdx.x == 0) {
+#pragma unroll
+      for (int r = 0; r < CH_NB; ++r)
+        if (lane <= r) A[(k0 + r) * lda + k0 + lane] = Ls[r * TS_LD + lane];
+      if (lane == 0 && bad) atomicCAS(info, 0, bad);
+    }
+  }
+  __syncthreads();
+  solve_rows_wave(Ls, rdiag, X, lane);
+#pragma unroll
   for (int r = 0; r < CH_NB; ++r) {
     if (r0 + r < n) A[(r0 + r) * lda + k0 + lane] = X[r * TS_LD + lane];
   }
+}
+
+// Update of the panel columns to the right of a 64-column step:
+// C[i][j] -= sum_{k<64} P[i][k] P[j][k] for i < rest, j < cols, where
+// P = A[k0+64.., k0..k0+64) holds the solved rows and C = A[k0+64.., k0+64..).
+// One 64x64 tile per workgroup.  The operand tiles and the C tile are read
+// with every load in flight (one memory latency per tile); the product runs on
+// 4 waves of 32x32 (2x2 v_mfma_f64_16x16x4) out of LDS in two K halves.
+// Rows and columns past the edge re-read the last valid one and are not stored.
+constexpr int PU_LD = 33;  // doubles per row of a 64x32 LDS half tile
+
+__global__ void __launch_bounds__(256) panel_update_kernel(double* __restrict__ A, int64_t lda, int64_t k0,
+                                                           int64_t rest, int64_t cols, int64_t tiles_n,
+                                                           int64_t n_tiles, const int32_t* __restrict__ info) {
+  if (*info != 0) return;
+  __shared__ double sA[CH_NB * PU_LD];
+  __shared__ double sB[CH_NB * PU_LD];
+  const int64_t bid = xcd_remap(blockIdx.x, gridDim.x);
+  if (bid >= n_tiles) return;
+  const int64_t i0 = (bid / tiles_n) * 64, j0 = (bid % tiles_n) * 64;
+  const double* P = A + (k0 + CH_NB) * lda + k0;
+  double* C = A + (k0 + CH_NB) * lda + k0 + CH_NB;
+  const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
+  const int wm = (wid >> 1) * 32, wn = (wid & 1) * 32;
+  // operand loads: wave w reads rows w, w+4, ... with lanes along k
+  double va[16], vb[16], vc[2][2][4];
+#pragma unroll
+  for (int it = 0; it < 16; ++it) {
+    const int r = wid + 4 * it;
+    va[it] = P[min<int64_t>(i0 + r, rest - 1) * lda + lane];
+    vb[it] = P[min<int64_t>(j0 + r, cols - 1) * lda + lane];
+  }
+  // C in the MFMA output layout: element e of lane l in a 16x16 block is
+  // (row l/16 + 4e, col l%16)
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t gi = min<int64_t>(i0 + wm + x * 16 + (lane >> 4) + 4 * e, rest - 1);
+        const int64_t gj = min<int64_t>(j0 + wn + y * 16 + (lane & 15), cols - 1);
+        vc[x][y][e] = C[gi * lda + gj];
+      }
+  doublex4 acc[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) acc[x][y] = (doublex4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (h) __syncthreads();
+    if ((lane >> 5) == h) {
+#pragma unroll
+      for (int it = 0; it < 16; ++it) {
+        const int r = wid + 4 * it;
+        sA[r * PU_LD + (lane & 31)] = va[it];
+        sB[r * PU_LD + (lane & 31)] = vb[it];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < 32; ks += 4) {
+      const int kq = ks + (lane >> 4);
+      double av[2], bv[2];
+#pragma unroll
+      for (int x = 0; x < 2; ++x) av[x] = sA[(wm + x * 16 + (lane & 15)) * PU_LD + kq];
+#pragma unroll
+      for (int y = 0; y < 2; ++y) bv[y] = sB[(wn + y * 16 + (lane & 15)) * PU_LD + kq];
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+          acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x], bv[y], acc[x][y], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t gi = i0 + wm + x * 16 + (lane >> 4) + 4 * e;
+        const int64_t gj = j0 + wn + y * 16 + (lane & 15);
+        if (gi < rest && gj < cols) C[gi * lda + gj] = vc[x][y][e] - acc[x][y][e];
+      }
 }
 
 // inv_b = L_bb^{-1} for every 64x64 diagonal block b of the factor (row-major
@@ -582,11 +701,14 @@ __global__ void __launch_bounds__(64) trtri_blocks_kernel(const double* __restri
   const int k = threadIdx.x;
   const int64_t k0 = (int64_t)blockIdx.x * CH_NB;
   const int nb = (int)min<int64_t>(CH_NB, n - k0);
-  for (int idx = k; idx < CH_NB * CH_NB; idx += 64) {
-    const int r = idx >> 6, c = idx & 63;
-    double v = (r == c) ? 1.0 : 0.0;
-    if (r < nb && c < nb && c <= r) v = A[(k0 + r) * lda + k0 + c];
-    LT[c * CH_LT + r] = v;
+  {
+    // lane = column; all row loads in flight before the LDS stores
+    double v[CH_NB];
+#pragma unroll
+    for (int r = 0; r < CH_NB; ++r)
+      v[r] = (r < nb && k < nb && k <= r) ? A[(k0 + r) * lda + k0 + k] : (r == k ? 1.0 : 0.0);
+#pragma unroll
+    for (int r = 0; r < CH_NB; ++r) LT[k * CH_LT + r] = v[r];
   }
   wave_lds_sync();
   rdiag[k] = 1.0 / LT[k * CH_LT + k];
@@ -996,7 +1118,6 @@ int nka_gemm_nt_f64(const double* a, int64_t lda, const double* b, int64_t ldb, 
 
 // Workspace: 64x64 inverses of the diagonal blocks, the intermediate vector
 // of the solves, one flag per block and one error word.
-constexpr int CH_NBO = 256;  // outer panel width of the two-level blocking
 
 int64_t nka_potrf_workspace(int64_t n) {
   if (n <= 0) return 0;
@@ -1013,15 +1134,16 @@ static inline uint32_t* ws_flags(void* ws, int64_t n, int64_t nblk) {
   return (uint32_t*)(ws_vec(ws, nblk) + n + 64);
 }
 
-// A second stream per device for the trailing updates of the Cholesky
-// (lookahead), with the two events that order it against the caller's stream.
-// Created on the first call that is not inside a graph capture; default
-// priority.  Without it the factorisation runs on the caller's stream alone.
-// Right-looking Cholesky with two levels of blocking: 256-column panels are
-// factored by 64-column steps (diagonal block, panel solve, then the update
-// of the panel's remaining columns as an FP64 MFMA product), then the
-// trailing matrix receives one rank-256 update.  The inverses of the diagonal
-// blocks, used by the triangular solves, are formed at the end.
+// Right-looking Cholesky with up to three levels of blocking.  An outer
+// panel of NB0 columns is factored as panels of NB1 columns, each by 64-column
+// steps (diagonal block and panel solve in one launch, then the update of the
+// NB1-panel's remaining columns); after each NB1-panel the rest of the outer
+// panel receives a rank-NB1 update, and after the outer panel the trailing
+// matrix one rank-NB0 update.  A long K keeps the MFMA GEMM near its peak and
+// reads and writes the trailing matrix fewer times; the widths trade that
+// against the tile-count quantisation of the narrower updates.  The inverses
+// of the diagonal blocks, used by the triangular solves, are formed at the
+// end.
 int nka_potrf_f64(double* a, int64_t n, int64_t lda, void* workspace, int32_t* info,
                   void* stream) {
   NKA_CHECK_ARG(n >= 0 && lda >= n && a && workspace && info, "nka_potrf_f64: bad arguments");
@@ -1030,32 +1152,49 @@ int nka_potrf_f64(double* a, int64_t n, int64_t lda, void* workspace, int32_t* i
   double* inv = ws_inv(workspace);
   const int64_t nblk = (n + CH_NB - 1) / CH_NB;
   NKA_CHECK_ARG(nblk < (1LL << 31), "nka_potrf_f64: too large");
-  for (int64_t K0 = 0; K0 < n; K0 += CH_NBO) {
-    const int64_t nbo = std::min<int64_t>(CH_NBO, n - K0);
-    for (int64_t k0 = K0; k0 < K0 + nbo; k0 += CH_NB) {
-      const int64_t nb = std::min<int64_t>(CH_NB, n - k0);
-      const int64_t rest = n - k0 - nb;
-      // the block's inverse slot and the vector area serve as scratch for
-      // L11^T and 1/diag until the inverses are formed at the end
-      double* lt = inv + (k0 / CH_NB) * CH_NB * CH_NB;
-      double* rd = ws_vec(workspace, nblk);
-      hipLaunchKernelGGL(potf2_kernel, dim3(1), dim3(64), 0, s, a, lda, n, k0, lt, rd, info);
-      if (rest <= 0) continue;  // rows below imply nb == 64
-      const unsigned grid = (unsigned)((rest + 127) / 128);
-      hipLaunchKernelGGL(trsm_panel_kernel, dim3(grid), dim3(128), 0, s, a, lda, n, k0, lt, rd, info);
-      // update of the remaining columns of this panel
-      const int64_t cols = K0 + nbo - (k0 + nb);
-      if (cols > 0) {
-        const double* a21 = a + (k0 + nb) * lda + k0;
-        double* c = a + (k0 + nb) * lda + (k0 + nb);
-        gemm_launch(a21, lda, a21, lda, c, lda, rest, cols, nb, -1.0, 1.0, 0.0, 0, info, s);
+  // panel widths measured best on MI355X: n = 8384 -> (512, 512), n = 32768 -> (4096, 512)
+  const int64_t NB0 = n >= 16384 ? 4096 : 512, NB1 = 512;
+  for (int64_t K0 = 0; K0 < n; K0 += NB0) {
+    const int64_t n0 = std::min<int64_t>(NB0, n - K0);
+    for (int64_t K1 = K0; K1 < K0 + n0; K1 += NB1) {
+      const int64_t n1 = std::min<int64_t>(NB1, K0 + n0 - K1);
+      for (int64_t k0 = K1; k0 < K1 + n1; k0 += CH_NB) {
+        const int64_t nb = std::min<int64_t>(CH_NB, n - k0);
+        const int64_t rest = n - k0 - nb;
+        if (rest <= 0) {
+          hipLaunchKernelGGL(potf2_kernel, dim3(1), dim3(64), 0, s, a, lda, n, k0, info);
+          continue;
+        }
+        // rows below imply nb == 64
+        const unsigned grid = (unsigned)((rest + 127) / 128);
+        hipLaunchKernelGGL(panel_factor_kernel, dim3(grid), dim3(128), 0, s, a, lda, n, k0, info);
+        // update of the remaining columns of this 256-column panel
+        const int64_t cols = K1 + n1 - (k0 + nb);
+        if (cols > 0) {
+          const int64_t tm = (rest + 63) / 64, tn = (cols + 63) / 64;
+          hipLaunchKernelGGL(panel_update_kernel, dim3(grid_mult8(tm * tn)), dim3(256), 0, s, a, lda, k0,
+                             rest, cols, tn, tm * tn, info);
+        }
+      }
+      // update of the remaining columns of this 1024-column panel (K = 256):
+      // the lower triangle of its square top, then the rows below it
+      const int64_t cols1 = K0 + n0 - (K1 + n1);
+      if (cols1 > 0) {
+        const int64_t rows1 = n - (K1 + n1);
+        const double* l = a + (K1 + n1) * lda + K1;
+        double* c = a + (K1 + n1) * lda + (K1 + n1);
+        gemm_big_launch(l, lda, l, lda, c, lda, cols1, cols1, n1, -1.0, 1.0, 1, info, s);
+        if (rows1 > cols1)
+          gemm_big_launch(l + cols1 * lda, lda, l, lda, c + cols1 * lda, lda, rows1 - cols1, cols1, n1, -1.0,
+                          1.0, 0, info, s);
       }
     }
-    const int64_t rest2 = n - K0 - nbo;
+    // trailing update (K = 1024)
+    const int64_t rest2 = n - K0 - n0;
     if (rest2 > 0) {
-      const double* l21 = a + (K0 + nbo) * lda + K0;
-      double* a22 = a + (K0 + nbo) * lda + (K0 + nbo);
-      gemm_big_launch(l21, lda, l21, lda, a22, lda, rest2, rest2, nbo, -1.0, 1.0, 1, info, s);
+      const double* l21 = a + (K0 + n0) * lda + K0;
+      double* a22 = a + (K0 + n0) * lda + (K0 + n0);
+      gemm_big_launch(l21, lda, l21, lda, a22, lda, rest2, rest2, n0, -1.0, 1.0, 1, info, s);
     }
   }
   hipLaunchKernelGGL(trtri_blocks_kernel, dim3((unsigned)nblk), dim3(64), 0, s, a, lda, n, inv, info);
