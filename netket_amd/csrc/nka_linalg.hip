@@ -400,10 +400,11 @@ __device__ inline double rsqrt_refined(double a) {
 // Factor a 64x64 block held in LDS (Ab, row-major, zeros above the diagonal)
 // in place with one wavefront, lane i = row i, by 16-column sub-panels:
 //  * sub-panel p (columns 16p..16p+15, rows 16p..63) is factored
-//    right-looking with the 16 values of each row in registers; the unscaled
-//    column of each step is broadcast through LDS and the update is
-//    a_ic -= L_ij L_cj = a_ij a_cj / a_jj, with 1/sqrt(a_jj) from the
-//    hardware estimate and two Newton steps;
+//    right-looking with the 16 values of each row in registers; at step j the
+//    pivot a_jj is read from its lane with v_readlane and the column entries
+//    a_cj of the sub-panel's rows c are broadcast through LDS (colbuf), and
+//    the update is a_ic -= L_ij L_cj = a_ij a_cj / a_jj, with 1/sqrt(a_jj)
+//    from the hardware estimate and two Newton steps;
 //  * the block's remaining lower part (rows and columns >= 16(p+1)) then
 //    receives the rank-16 update on FP64 MFMA (v_mfma_f64_16x16x4).
 // On return Ab holds L (exact zeros above the diagonal) and yv[j] = 1/L_jj.
@@ -420,9 +421,14 @@ __device__ int factor_block_wave(double* Ab, double* colbuf, double* yv, int lan
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       const int jj = 16 * p + j;
+      // the pivot by v_readlane (on the critical path), the rest of the
+      // column through LDS while the reciprocal square root is formed
       colbuf[lane] = x[j];
+      const double ajj = readlane_f64(x[j], jj);
       wave_lds_sync();
-      const double ajj = colbuf[jj];
+      double cj[16];
+#pragma unroll
+      for (int q = j + 1; q < 16; ++q) cj[q] = colbuf[16 * p + q];
       const bool ok = (ajj > 0.0) && isfinite(ajj);
       if (!ok && bad == 0) bad = (int)(k0 + jj + 1);
       const double piv = ok ? ajj : 1.0;
@@ -431,7 +437,7 @@ __device__ int factor_block_wave(double* Ab, double* colbuf, double* yv, int lan
       x[j] = (lane > jj) ? x[j] * y : (lane == jj ? piv * y : 0.0);
       if (lane == jj) yv[jj] = y;
 #pragma unroll
-      for (int q = j + 1; q < 16; ++q) x[q] -= sij * colbuf[16 * p + q];
+      for (int q = j + 1; q < 16; ++q) x[q] -= sij * cj[q];
       wave_lds_sync();
     }
 #pragma unroll
@@ -462,51 +468,46 @@ __device__ int factor_block_wave(double* Ab, double* colbuf, double* yv, int lan
   return bad;
 }
 
-// Solve the 64 rows of X (LDS, row-major) in place, X <- X L^{-T}, with one
+// Solve 16 rows of X (LDS, row-major) in place, X <- X L^{-T}, with one
 // wavefront by 16-column blocks: block p first receives
-// X_p -= X_{<p} L_{p,<p}^T on FP64 MFMA (v_mfma_f64_16x16x4), then the 16x16
-// triangular solve X_p <- X_p L_pp^{-T} by substitution, one row per lane with
-// 16 values in registers.  Ls = L (row-major, LDS), rdiag[j] = 1/L_jj.
-__device__ void solve_rows_wave(const double* Ls, const double* rdiag, double* X, int lane) {
+// X_p -= X_{<p} L_{p,<p}^T on FP64 MFMA (v_mfma_f64_16x16x4, one 16x16
+// accumulator), then the 16x16 triangular solve X_p <- X_p L_pp^{-T} by
+// substitution, one row per lane with 16 values in registers (lanes 16..63
+// repeat the work of lane & 15 and do not store).  Ls = L (row-major, LDS),
+// rdiag[j] = 1/L_jj.
+__device__ void solve_rows16_wave(const double* Ls, const double* rdiag, double* X, int lane) {
+  const int row = lane & 15;
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
     if (p > 0) {
       // X[:, 16p..16p+16) -= X[:, 0..16p) L[16p..16p+16, 0..16p)^T
-      doublex4 acc[4];
-#pragma unroll
-      for (int rb = 0; rb < 4; ++rb) acc[rb] = (doublex4){0.0, 0.0, 0.0, 0.0};
+      doublex4 acc = (doublex4){0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int kb = 0; kb < 16 * p; kb += 4) {
         const int kk = kb + (lane >> 4);
-        const double bv = Ls[(16 * p + (lane & 15)) * TS_LD + kk];
-#pragma unroll
-        for (int rb = 0; rb < 4; ++rb) {
-          const double av = X[(rb * 16 + (lane & 15)) * TS_LD + kk];
-          acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[rb], 0, 0, 0);
-        }
+        const double av = X[row * TS_LD + kk];
+        const double bv = Ls[(16 * p + row) * TS_LD + kk];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
       }
-      // element e of lane l in a 16x16 block is (row l/16 + 4e, col l%16)
+      // element e of lane l is (row l/16 + 4e, col l%16)
 #pragma unroll
-      for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int r = rb * 16 + (lane >> 4) + 4 * e, c = 16 * p + (lane & 15);
-          X[r * TS_LD + c] -= acc[rb][e];
-        }
+      for (int e = 0; e < 4; ++e) X[((lane >> 4) + 4 * e) * TS_LD + 16 * p + row] -= acc[e];
       wave_lds_sync();
     }
-    // X[lane, 16p..16p+16) <- X[lane, 16p..) L_pp^{-T} (substitution)
+    // X[row, 16p..16p+16) <- X[row, 16p..) L_pp^{-T} (substitution)
     double x[16];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) x[q] = X[lane * TS_LD + 16 * p + q];
+    for (int q = 0; q < 16; ++q) x[q] = X[row * TS_LD + 16 * p + q];
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       x[j] *= rdiag[16 * p + j];
 #pragma unroll
       for (int q = j + 1; q < 16; ++q) x[q] -= x[j] * Ls[(16 * p + q) * TS_LD + 16 * p + j];
     }
+    if (lane < 16) {
 #pragma unroll
-    for (int q = 0; q < 16; ++q) X[lane * TS_LD + 16 * p + q] = x[q];
+      for (int q = 0; q < 16; ++q) X[row * TS_LD + 16 * p + q] = x[q];
+    }
     wave_lds_sync();
   }
 }
@@ -551,54 +552,58 @@ __global__ void __launch_bounds__(64) potf2_kernel(double* __restrict__ A, int64
 
 // One 64-column step of the factorisation for a block with rows below it:
 // factor the diagonal block A[k0:k0+64, k0:k0+64] and solve the rows below,
-// L21 = A21 L11^{-T}, in place.  Every workgroup (two wavefronts, 128 rows)
-// reads the diagonal block and factors it in LDS itself (wavefront 0), so the
-// solve needs no second launch and no exchange through global memory;
-// workgroup 0 writes the factored block back and reports a failed pivot in
-// *info.  All global loads of a workgroup are issued before its first LDS
-// store.  Rows past n re-read the last row and are not stored.
-__global__ void __launch_bounds__(128) panel_factor_kernel(double* __restrict__ A, int64_t lda, int64_t n,
+// L21 = A21 L11^{-T}, in place.  Every workgroup (four wavefronts of 16 rows,
+// 64 rows) reads the diagonal block and factors it in LDS itself (wavefront
+// 0), so the solve needs no second launch and no exchange through global
+// memory; workgroup 0 writes the factored block back and reports a failed
+// pivot in *info.  All global loads of a workgroup are issued before its
+// first LDS store.  Rows past n re-read the last row and are not stored.
+constexpr int PF_ROWS = 16;  // rows of the panel per wavefront
+
+__global__ void __launch_bounds__(256) panel_factor_kernel(double* __restrict__ A, int64_t lda, int64_t n,
                                                            int64_t k0, int32_t* __restrict__ info) {
   if (*info != 0) return;
-  __shared__ double Ls[CH_NB * TS_LD];        // the diagonal block, then L11 (row-major)
+  __shared__ double Ls[CH_NB * TS_LD];           // the diagonal block, then L11 (row-major)
   __shared__ double colbuf[CH_NB];
   __shared__ double rdiag[CH_NB];
-  __shared__ double Xs[2][CH_NB * TS_LD];     // one 64x64 row tile per wavefront
+  __shared__ double Xs[4][PF_ROWS * TS_LD];      // 16 rows per wavefront
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   double* X = Xs[w];
-  const int64_t r0 = k0 + CH_NB + (int64_t)blockIdx.x * 128 + 64 * w;
+  const int64_t r0 = k0 + CH_NB + (int64_t)blockIdx.x * (4 * PF_ROWS) + PF_ROWS * w;
   {
-    // wavefront w reads rows 32w..32w+31 of the diagonal block and its own 64
-    // rows, lanes along the columns
-    double vd[32];
-    const double* pd = A + (k0 + 32 * w) * lda + k0 + lane;
+    // wavefront w reads rows 16w..16w+15 of the diagonal block and its own
+    // 16 rows, lanes along the columns
+    double vd[PF_ROWS], v[PF_ROWS];
+    const double* pd = A + (k0 + PF_ROWS * w) * lda + k0 + lane;
 #pragma unroll
-    for (int r = 0; r < 32; ++r) vd[r] = pd[r * lda];
-    double v[CH_NB];
+    for (int r = 0; r < PF_ROWS; ++r) vd[r] = pd[r * lda];
 #pragma unroll
-    for (int r = 0; r < CH_NB; ++r) v[r] = A[min<int64_t>(r0 + r, n - 1) * lda + k0 + lane];
+    for (int r = 0; r < PF_ROWS; ++r) v[r] = A[min<int64_t>(r0 + r, n - 1) * lda + k0 + lane];
 #pragma unroll
-    for (int r = 0; r < 32; ++r) {
-      const int R = 32 * w + r;
+    for (int r = 0; r < PF_ROWS; ++r) {
+      const int R = PF_ROWS * w + r;
       Ls[R * TS_LD + lane] = lane <= R ? vd[r] : 0.0;
     }
 #pragma unroll
-    for (int r = 0; r < CH_NB; ++r) X[r * TS_LD + lane] = v[r];
+    for (int r = 0; r < PF_ROWS; ++r) X[r * TS_LD + lane] = v[r];
   }
   __syncthreads();
   if (w == 0) {
     const int bad = factor_block_wave(Ls, colbuf, rdiag, lane, k0);
-    if (blockIdx.x == 0) {
-#pragma unroll
-      for (int r = 0; r < CH_NB; ++r)
-        if (lane <= r) A[(k0 + r) * lda + k0 + lane] = Ls[r * TS_LD + lane];
-      if (lane == 0 && bad) atomicCAS(info, 0, bad);
-    }
+    if (blockIdx.x == 0 && lane == 0 && bad) atomicCAS(info, 0, bad);
   }
   __syncthreads();
-  solve_rows_wave(Ls, rdiag, X, lane);
+  if (blockIdx.x == 0) {
+    // wavefront w writes rows 16w..16w+15 of L11 back
 #pragma unroll
-  for (int r = 0; r < CH_NB; ++r) {
+    for (int r = 0; r < PF_ROWS; ++r) {
+      const int R = PF_ROWS * w + r;
+      if (lane <= R) A[(k0 + R) * lda + k0 + lane] = Ls[R * TS_LD + lane];
+    }
+  }
+  solve_rows16_wave(Ls, rdiag, X, lane);
+#pragma unroll
+  for (int r = 0; r < PF_ROWS; ++r) {
     if (r0 + r < n) A[(r0 + r) * lda + k0 + lane] = X[r * TS_LD + lane];
   }
 }
@@ -1166,8 +1171,8 @@ int nka_potrf_f64(double* a, int64_t n, int64_t lda, void* workspace, int32_t* i
           continue;
         }
         // rows below imply nb == 64
-        const unsigned grid = (unsigned)((rest + 127) / 128);
-        hipLaunchKernelGGL(panel_factor_kernel, dim3(grid), dim3(128), 0, s, a, lda, n, k0, info);
+        const unsigned grid = (unsigned)((rest + 4 * PF_ROWS - 1) / (4 * PF_ROWS));
+        hipLaunchKernelGGL(panel_factor_kernel, dim3(grid), dim3(256), 0, s, a, lda, n, k0, info);
         // update of the remaining columns of this 256-column panel
         const int64_t cols = K1 + n1 - (k0 + nb);
         if (cols > 0) {
