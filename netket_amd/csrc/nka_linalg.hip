@@ -514,10 +514,12 @@ __device__ void solve_rows16_wave(const double* Ls, const double* rdiag, double*
 
 // Factor the diagonal block A[k0:k0+nb, k0:k0+nb] in place (lower triangle)
 // with one wavefront; used for the last block, which has no rows below it.
+// The factor (64x64 row-major, identity padding past nb, zeros above the
+// diagonal) is also written to `slot`, where the inversion kernel reads it.
 // A non-positive or non-finite pivot sets *info (first failing column + 1).
-// Rows/columns past nb are an identity padding.
 __global__ void __launch_bounds__(64) potf2_kernel(double* __restrict__ A, int64_t lda, int64_t n,
-                                                   int64_t k0, int32_t* __restrict__ info) {
+                                                   int64_t k0, double* __restrict__ slot,
+                                                   int32_t* __restrict__ info) {
   if (*info != 0) return;
   __shared__ double Ab[CH_NB * TS_LD];   // the block, row-major
   __shared__ double colbuf[CH_NB];
@@ -547,6 +549,8 @@ __global__ void __launch_bounds__(64) potf2_kernel(double* __restrict__ A, int64
     const double v = Ab[r * TS_LD + lane];
     if (lane < nb && lane <= r) A[(k0 + r) * lda + k0 + lane] = v;
   }
+#pragma unroll
+  for (int r = 0; r < CH_NB; ++r) slot[r * CH_NB + lane] = Ab[r * TS_LD + lane];
   if (lane == 0 && bad) atomicCAS(info, 0, bad);
 }
 
@@ -555,13 +559,16 @@ __global__ void __launch_bounds__(64) potf2_kernel(double* __restrict__ A, int64
 // L21 = A21 L11^{-T}, in place.  Every workgroup (four wavefronts of 16 rows,
 // 64 rows) reads the diagonal block and factors it in LDS itself (wavefront
 // 0), so the solve needs no second launch and no exchange through global
-// memory; workgroup 0 writes the factored block back and reports a failed
-// pivot in *info.  All global loads of a workgroup are issued before its
-// first LDS store.  Rows past n re-read the last row and are not stored.
+// memory.  Workgroup 0 writes L11 to `slot` (64x64 row-major) and reports a
+// failed pivot in *info; L11 reaches A's diagonal block in the inversion
+// kernel at the end, since a workgroup that starts late still reads the
+// unfactored block from A.  All global loads of a workgroup are issued before
+// its first LDS store.  Rows past n re-read the last row and are not stored.
 constexpr int PF_ROWS = 16;  // rows of the panel per wavefront
 
 __global__ void __launch_bounds__(256) panel_factor_kernel(double* __restrict__ A, int64_t lda, int64_t n,
-                                                           int64_t k0, int32_t* __restrict__ info) {
+                                                           int64_t k0, double* __restrict__ slot,
+                                                           int32_t* __restrict__ info) {
   if (*info != 0) return;
   __shared__ double Ls[CH_NB * TS_LD];           // the diagonal block, then L11 (row-major)
   __shared__ double colbuf[CH_NB];
@@ -594,11 +601,13 @@ __global__ void __launch_bounds__(256) panel_factor_kernel(double* __restrict__ 
   }
   __syncthreads();
   if (blockIdx.x == 0) {
-    // wavefront w writes rows 16w..16w+15 of L11 back
+    // wavefront w writes rows 16w..16w+15 of L11 (zeros above the diagonal)
+    // to the slot; A's diagonal block is left unchanged here because other
+    // workgroups of this launch may still be reading it
 #pragma unroll
     for (int r = 0; r < PF_ROWS; ++r) {
       const int R = PF_ROWS * w + r;
-      if (lane <= R) A[(k0 + R) * lda + k0 + lane] = Ls[R * TS_LD + lane];
+      slot[R * CH_NB + lane] = Ls[R * TS_LD + lane];
     }
   }
   solve_rows16_wave(Ls, rdiag, X, lane);
@@ -694,11 +703,13 @@ __global__ void __launch_bounds__(256) panel_update_kernel(double* __restrict__ 
       }
 }
 
-// inv_b = L_bb^{-1} for every 64x64 diagonal block b of the factor (row-major
-// 64x64 each, identity padding); one wavefront per block, lane k computes
+// inv_b = L_bb^{-1} for every 64x64 diagonal block b of the factor; one
+// wavefront per block.  The factored block is read from its slot in `inv`
+// (row-major, identity padding, zeros above the diagonal), copied into A's
+// diagonal block, and replaced by its inverse (row-major): lane k computes
 // column k by substitution with the block's columns broadcast from LDS.
-__global__ void __launch_bounds__(64) trtri_blocks_kernel(const double* __restrict__ A, int64_t lda,
-                                                          int64_t n, double* __restrict__ inv,
+__global__ void __launch_bounds__(64) trtri_blocks_kernel(double* __restrict__ A, int64_t lda, int64_t n,
+                                                          double* __restrict__ inv,
                                                           const int32_t* __restrict__ info) {
   if (*info != 0) return;
   __shared__ double LT[CH_NB * CH_LT];
@@ -706,14 +717,17 @@ __global__ void __launch_bounds__(64) trtri_blocks_kernel(const double* __restri
   const int k = threadIdx.x;
   const int64_t k0 = (int64_t)blockIdx.x * CH_NB;
   const int nb = (int)min<int64_t>(CH_NB, n - k0);
+  double* blk = inv + (int64_t)blockIdx.x * CH_NB * CH_NB;
   {
-    // lane = column; all row loads in flight before the LDS stores
+    // lane = column; all row loads in flight before the stores
     double v[CH_NB];
 #pragma unroll
-    for (int r = 0; r < CH_NB; ++r)
-      v[r] = (r < nb && k < nb && k <= r) ? A[(k0 + r) * lda + k0 + k] : (r == k ? 1.0 : 0.0);
+    for (int r = 0; r < CH_NB; ++r) v[r] = blk[r * CH_NB + k];
 #pragma unroll
     for (int r = 0; r < CH_NB; ++r) LT[k * CH_LT + r] = v[r];
+#pragma unroll
+    for (int r = 0; r < CH_NB; ++r)
+      if (r < nb && k < nb && k <= r) A[(k0 + r) * lda + k0 + k] = v[r];
   }
   wave_lds_sync();
   rdiag[k] = 1.0 / LT[k * CH_LT + k];
@@ -728,9 +742,8 @@ __global__ void __launch_bounds__(64) trtri_blocks_kernel(const double* __restri
 #pragma unroll
     for (int r = m + 1; r < CH_NB; ++r) x[r] -= xm * LT[m * CH_LT + r];
   }
-  double* out = inv + (int64_t)blockIdx.x * CH_NB * CH_NB;
 #pragma unroll
-  for (int r = 0; r < CH_NB; ++r) out[r * CH_NB + k] = x[r];
+  for (int r = 0; r < CH_NB; ++r) blk[r * CH_NB + k] = x[r];
 }
 
 // Zero n 32-bit words (a fill kernel rather than a memset node, so that a
@@ -781,34 +794,46 @@ __global__ void __launch_bounds__(256) trsv_fwd_lookback(const double* __restric
   if (*info != 0) return;
   __shared__ double part[4][CH_NB];
   __shared__ double t[CH_NB];
+  __shared__ double xs[4][CH_NB];
   __shared__ double Xs[CH_NB * (CH_NB + 1)];
   const int kb = blockIdx.x;
   const int64_t k0 = (int64_t)kb * CH_NB;
   const int nb = (int)min<int64_t>(CH_NB, n - k0);
   const int wid = threadIdx.x / 64, r = lane_id();
   const double* invk = inv + (int64_t)kb * CH_NB * CH_NB;
-  for (int idx = threadIdx.x; idx < CH_NB * CH_NB; idx += 256)
+#pragma unroll
+  for (int it = 0; it < CH_NB * CH_NB / 256; ++it) {
+    const int idx = threadIdx.x + it * 256;
     Xs[(idx >> 6) * (CH_NB + 1) + (idx & 63)] = invk[idx];
+  }
   const double bk = (wid == 0 && r < nb) ? b[k0 + r] : 0.0;
+  // rows past nb re-read the last row; their sums are never used
+  const double* arow = A + (k0 + min(r, nb - 1)) * lda;
   double acc = 0.0;
   for (int jb = wid; jb < kb; jb += 4) {
     const int64_t j0 = (int64_t)jb * CH_NB;
     double Lrow[CH_NB];
 #pragma unroll
-    for (int c = 0; c < CH_NB; ++c) Lrow[c] = r < nb ? A[(k0 + r) * lda + j0 + c] : 0.0;
+    for (int c = 0; c < CH_NB; ++c) Lrow[c] = arow[j0 + c];
     if (r == 0) wait_flag(flags + jb, 1u, err);
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    // block jb < kb is a full block: y[j0 + lane] is in bounds
+    xs[wid][r] = y[j0 + r];
+    wave_lds_sync();
 #pragma unroll
-    for (int c = 0; c < CH_NB; ++c) acc += Lrow[c] * y[j0 + c];
+    for (int c = 0; c < CH_NB; ++c) acc += Lrow[c] * xs[wid][c];
+    wave_lds_sync();
   }
   part[wid][r] = acc;
   __syncthreads();
   if (wid == 0) {
-    t[r] = bk - (part[0][r] + part[1][r] + part[2][r] + part[3][r]);
+    t[r] = (r < nb) ? bk - (part[0][r] + part[1][r] + part[2][r] + part[3][r]) : 0.0;
     wave_lds_sync();
+    // inv_kk is lower triangular (zeros above the diagonal)
     double s = 0.0;
-    for (int c = 0; c <= r; ++c) s += Xs[r * (CH_NB + 1) + c] * t[c];
+#pragma unroll
+    for (int c = 0; c < CH_NB; ++c) s += Xs[r * (CH_NB + 1) + c] * t[c];
     if (r < nb) y[k0 + r] = s;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __builtin_amdgcn_wave_barrier();
@@ -827,6 +852,7 @@ __global__ void __launch_bounds__(256) trsv_bwd_lookback(const double* __restric
   if (*info != 0) return;
   __shared__ double part[4][CH_NB];
   __shared__ double t[CH_NB];
+  __shared__ double xs[4][CH_NB];
   __shared__ double Xs[CH_NB * (CH_NB + 1)];
   const int nblk = gridDim.x;
   const int kb = nblk - 1 - blockIdx.x;
@@ -834,31 +860,41 @@ __global__ void __launch_bounds__(256) trsv_bwd_lookback(const double* __restric
   const int nb = (int)min<int64_t>(CH_NB, n - k0);
   const int wid = threadIdx.x / 64, r = lane_id();
   const double* invk = inv + (int64_t)kb * CH_NB * CH_NB;
-  for (int idx = threadIdx.x; idx < CH_NB * CH_NB; idx += 256)
+#pragma unroll
+  for (int it = 0; it < CH_NB * CH_NB / 256; ++it) {
+    const int idx = threadIdx.x + it * 256;
     Xs[(idx >> 6) * (CH_NB + 1) + (idx & 63)] = invk[idx];
+  }
   const double yk = (wid == 0 && r < nb) ? y[k0 + r] : 0.0;
+  // columns past nb re-read the last column; their sums are never used
+  const int64_t col = k0 + min(r, nb - 1);
   double acc = 0.0;
   for (int jb = nblk - 1 - wid; jb > kb; jb -= 4) {
     const int64_t j0 = (int64_t)jb * CH_NB;
     const int mb = (int)min<int64_t>(CH_NB, n - j0);
-    // column r of the tile L[j0:j0+mb, k0:k0+nb] (coalesced over r)
+    // column r of the tile L[j0:j0+mb, k0:k0+nb] (coalesced over r); rows
+    // past mb re-read the last row and meet a zero entry of x below
     double Lcol[CH_NB];
 #pragma unroll
-    for (int c = 0; c < CH_NB; ++c) Lcol[c] = (c < mb && r < nb) ? A[(j0 + c) * lda + k0 + r] : 0.0;
+    for (int c = 0; c < CH_NB; ++c) Lcol[c] = A[(j0 + min(c, mb - 1)) * lda + col];
     if (r == 0) wait_flag(flags + (nblk - 1 - jb), 1u, err);
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    xs[wid][r] = (r < mb) ? x[j0 + min(r, mb - 1)] : 0.0;
+    wave_lds_sync();
 #pragma unroll
-    for (int c = 0; c < CH_NB; ++c)
-      if (c < mb) acc += Lcol[c] * x[j0 + c];
+    for (int c = 0; c < CH_NB; ++c) acc += Lcol[c] * xs[wid][c];
+    wave_lds_sync();
   }
   part[wid][r] = acc;
   __syncthreads();
   if (wid == 0) {
-    t[r] = yk - (part[0][r] + part[1][r] + part[2][r] + part[3][r]);
+    t[r] = (r < nb) ? yk - (part[0][r] + part[1][r] + part[2][r] + part[3][r]) : 0.0;
     wave_lds_sync();
+    // column r of inv_kk (zeros above the diagonal)
     double s = 0.0;
-    for (int c = r; c < CH_NB; ++c) s += Xs[c * (CH_NB + 1) + r] * t[c];
+#pragma unroll
+    for (int c = 0; c < CH_NB; ++c) s += Xs[c * (CH_NB + 1) + r] * t[c];
     if (r < nb) x[k0 + r] = s;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __builtin_amdgcn_wave_barrier();
@@ -1166,13 +1202,14 @@ int nka_potrf_f64(double* a, int64_t n, int64_t lda, void* workspace, int32_t* i
       for (int64_t k0 = K1; k0 < K1 + n1; k0 += CH_NB) {
         const int64_t nb = std::min<int64_t>(CH_NB, n - k0);
         const int64_t rest = n - k0 - nb;
+        double* slot = inv + (k0 / CH_NB) * CH_NB * CH_NB;
         if (rest <= 0) {
-          hipLaunchKernelGGL(potf2_kernel, dim3(1), dim3(64), 0, s, a, lda, n, k0, info);
+          hipLaunchKernelGGL(potf2_kernel, dim3(1), dim3(64), 0, s, a, lda, n, k0, slot, info);
           continue;
         }
         // rows below imply nb == 64
         const unsigned grid = (unsigned)((rest + 4 * PF_ROWS - 1) / (4 * PF_ROWS));
-        hipLaunchKernelGGL(panel_factor_kernel, dim3(grid), dim3(256), 0, s, a, lda, n, k0, info);
+        hipLaunchKernelGGL(panel_factor_kernel, dim3(grid), dim3(256), 0, s, a, lda, n, k0, slot, info);
         // update of the remaining columns of this 256-column panel
         const int64_t cols = K1 + n1 - (k0 + nb);
         if (cols > 0) {

@@ -169,6 +169,33 @@ def test_cholesky_solve(gpu, n):
     np.testing.assert_allclose(x.cpu().numpy(), ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max())
 
 
+def test_cholesky_large_many_panel_workgroups(gpu):
+    # n > 512 * 64: the first panel launches more workgroups than the chip
+    # holds at once, so some start after workgroup 0 has factored the diagonal
+    # block; the factor and the solve must not depend on that order.  Checked
+    # by the residual of the solve and by L L^T on a sample of rows/columns.
+    n = 33000
+    print(f"case cholesky large n={n}", flush=True)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    X = torch.randn(n, n // 2, dtype=torch.float64, device="cuda", generator=g)
+    A0 = X @ X.T / n + 0.1 * torch.eye(n, dtype=torch.float64, device="cuda")
+    del X
+    a = A0.clone()
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ws = _ops.potrf_(a, info)
+    b = torch.ones(n, dtype=torch.float64, device="cuda")
+    _ops.potrs_(a, ws, b, info)
+    assert int(info.item()) == 0
+    assert (A0 @ b - 1.0).abs().max().item() < 1e-9
+    rows = torch.arange(0, n, 331, device="cuda")
+    cols = torch.arange(n, device="cuda")
+    Ls = torch.where(cols[None, :] <= rows[:, None], a[rows], 0.0)   # sampled rows of L
+    LLt = Ls @ Ls.T
+    ref = A0[rows][:, rows]
+    assert (LLt - ref).abs().max().item() < 1e-11
+    del a, A0, ws
+
+
 def test_cholesky_failure_and_fallback(gpu):
     # netket/_src/solvers/nan_fallback.py: NaN output triggers pinv_smooth
     n = 96
