@@ -230,6 +230,266 @@ __global__ void __launch_bounds__(256) rbm_metropolis_kernel(nka_rbm m, SweepArg
   }
 }
 
+// ---------------------------------------------------------------------------
+// Several chains per wavefront.  For small RBMs (M <= 8 * 64/G hidden units)
+// a proposal's arithmetic is a few hundred instructions that the per-proposal
+// overhead (random draw, cluster bookkeeping, reduction, acceptance) would
+// dominate on a wavefront per chain.  Here the wavefront holds G chains, one
+// per group of LPC = 64/G lanes: lane gl of group g owns hidden units
+// j = gl + LPC*jj, the spins of each chain are in LDS, and all per-chain
+// quantities (the draw, the selected move, the sums) are equal across the
+// group's lanes, so the groups run the same instruction stream on different
+// chains.  Sums over a group use DPP row permutations whose partner lanes
+// add the same two operands, so every lane of a group holds the identical sum.
+// Same random streams, rules and acceptance as rbm_metropolis_kernel.
+// ---------------------------------------------------------------------------
+template <int CTRL>
+__device__ inline float dpp_perm_f32(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ inline double dpp_perm_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int CTRL, class T> __device__ inline T dpp_perm(T v);
+template <> __device__ inline float dpp_perm<0xb1, float>(float v) { return dpp_perm_f32<0xb1>(v); }
+template <> __device__ inline float dpp_perm<0x4e, float>(float v) { return dpp_perm_f32<0x4e>(v); }
+template <> __device__ inline float dpp_perm<0x141, float>(float v) { return dpp_perm_f32<0x141>(v); }
+template <> __device__ inline float dpp_perm<0x140, float>(float v) { return dpp_perm_f32<0x140>(v); }
+template <> __device__ inline double dpp_perm<0xb1, double>(double v) { return dpp_perm_f64<0xb1>(v); }
+template <> __device__ inline double dpp_perm<0x4e, double>(double v) { return dpp_perm_f64<0x4e>(v); }
+template <> __device__ inline double dpp_perm<0x141, double>(double v) { return dpp_perm_f64<0x141>(v); }
+template <> __device__ inline double dpp_perm<0x140, double>(double v) { return dpp_perm_f64<0x140>(v); }
+
+// Sum over the LPC lanes of this lane's group (LPC = 16 or 32), in every lane.
+// log of a small positive integer count: hardware log2 for FP32, log for FP64.
+template <class T> __device__ inline T log_fast(T x);
+template <> __device__ inline float log_fast<float>(float x) {
+  return 0.69314718055994531f * __builtin_amdgcn_logf(x);
+}
+template <> __device__ inline double log_fast<double>(double x) { return log(x); }
+
+template <int LPC, class T>
+__device__ inline T group_sum(T v) {
+  v += dpp_perm<0xb1, T>(v);    // quad_perm [1,0,3,2]
+  v += dpp_perm<0x4e, T>(v);    // quad_perm [2,3,0,1]
+  v += dpp_perm<0x141, T>(v);   // row_half_mirror: quads 0 <-> 1 of each half-row
+  v += dpp_perm<0x140, T>(v);   // row_mirror: half-rows 0 <-> 1 of each row
+  if (LPC == 32) v += __shfl_xor(v, 16, NKA_WAVE);
+  return v;
+}
+
+template <class T, int G, int JG, int RSG, bool STAGE>
+__global__ void __launch_bounds__(256) rbm_metropolis_group_kernel(nka_rbm m, SweepArgs sa,
+                                                                   int8_t* __restrict__ sigma,
+                                                                   T* __restrict__ log_prob,
+                                                                   int8_t* __restrict__ samples,
+                                                                   T* __restrict__ lp_samples,
+                                                                   int64_t* __restrict__ n_accepted) {
+  constexpr int LPC = NKA_WAVE / G;
+  constexpr uint64_t GMASK = (LPC == 64) ? ~0ull : ((1ull << LPC) - 1ull);
+  extern __shared__ unsigned char smem[];
+  const RbmView<T> v = rbm_view<T>(m);
+  const int N = v.N, M = v.M;
+  const int wid = threadIdx.x / NKA_WAVE, waves = blockDim.x / NKA_WAVE;
+  const int lane = lane_id(), g = lane / LPC, gl = lane % LPC;
+  const size_t wbytes = STAGE ? pad16((size_t)N * M * sizeof(T)) : 0;
+  // LDS: [W (staged)] [visible bias a] [cluster pairs] [spins of each chain]
+  T* a_s = (T*)(smem + wbytes);
+  int32_t* cl_s = (int32_t*)(smem + wbytes + pad16((size_t)N * sizeof(T)));
+  int8_t* xs = (int8_t*)(smem + wbytes + pad16((size_t)N * sizeof(T)) +
+                         pad16((size_t)2 * sa.n_clusters * sizeof(int32_t))) +
+               (wid * G + g) * pad16(N);
+  const T* W = v.W;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) a_s[i] = v.a ? v.a[i] : (T)0;
+  for (int i = threadIdx.x; i < 2 * sa.n_clusters; i += blockDim.x) cl_s[i] = sa.clusters[i];
+  if (STAGE) {
+    stage_w<T>((T*)smem, v.W, N, M);  // ends with a workgroup barrier
+    W = (const T*)smem;
+  } else {
+    __syncthreads();
+  }
+  const int c_raw = (blockIdx.x * waves + wid) * G + g;
+  if ((blockIdx.x * waves + wid) * G >= sa.C) return;  // whole wavefront past the last chain
+  const bool live = c_raw < sa.C;                        // groups past the last chain repeat it
+  const int c = live ? c_raw : sa.C - 1;
+
+  for (int i = gl; i < N; i += LPC) xs[i] = sigma[(int64_t)c * N + i];
+  wave_lds_sync();
+  // register slot jj of a lane holds hidden unit gl + LPC * ((jj + g) mod JG):
+  // the rotation by the group index makes the groups of a wavefront read
+  // different LDS bank ranges of their W rows in the same instruction
+  T th[JG], lc[JG];
+  T part = (T)0;
+#pragma unroll
+  for (int jj = 0; jj < JG; ++jj) {
+    const int j = gl + LPC * ((jj + g) & (JG - 1));
+    th[jj] = (T)0;
+    lc[jj] = (T)0;
+    if (j < M) {
+      T acc = v.b ? v.b[j] : (T)0;
+      for (int i = 0; i < N; ++i) {
+        const T w = W[(size_t)i * M + j];
+        acc += (xs[i] > 0) ? w : -w;
+      }
+      th[jj] = acc;
+      lc[jj] = log_cosh<T>(acc);
+      part += lc[jj];
+    }
+  }
+  for (int i = gl; i < N; i += LPC) part += (xs[i] > 0) ? a_s[i] : -a_s[i];
+  const T pw = (T)sa.machine_pow;
+  T lp = pw * group_sum<LPC, T>(part);
+
+  // cluster slots of this lane: cl = gl + LPC * r, r < RSG (RSG * LPC >= n_clusters)
+  const bool exch = sa.rule == NKA_RULE_EXCHANGE;
+  int cu[RSG], cv[RSG];
+#pragma unroll
+  for (int r = 0; r < RSG; ++r) {
+    const int cl = gl + LPC * r;
+    if (exch && cl < sa.n_clusters) {
+      cu[r] = cl_s[2 * cl];
+      cv[r] = cl_s[2 * cl + 1];
+    } else {
+      cu[r] = -1;
+      cv[r] = -1;
+    }
+  }
+
+  int64_t nacc = 0;
+  uint64_t q = sa.counter;
+  if (sa.counter_dev) {
+    const uint64_t cd = *sa.counter_dev;
+    q = ((uint64_t)uniform_value<uint32_t>((uint32_t)(cd >> 32)) << 32) |
+        (uint64_t)uniform_value<uint32_t>((uint32_t)cd);
+  }
+  const int n_sweeps = sa.n_discard + sa.chain_length;
+  u32x4 rnd_next = philox4x32_10({(uint32_t)q, (uint32_t)(q >> 32), (uint32_t)c, TAG_METROPOLIS},
+                                 sa.k0, sa.k1);
+  for (int sw = 0; sw < n_sweeps; ++sw) {
+    for (int p = 0; p < sa.sweep_size; ++p, ++q) {
+      const u32x4 rnd = rnd_next;
+      {
+        const uint64_t qn = q + 1;
+        rnd_next = philox4x32_10({(uint32_t)qn, (uint32_t)(qn >> 32), (uint32_t)c, TAG_METROPOLIS},
+                                 sa.k0, sa.k1);
+      }
+      int s0, s1 = 0, nf;
+      T corr = (T)0;
+      bool valid = true;
+      if (!exch) {
+        s0 = (int)bounded(rnd.x, (uint32_t)N);
+        nf = 1;
+      } else {
+        // differing clusters of this group's chain as a bit set in cluster
+        // order: cluster cl = gl + LPC * r is bit cl % 64 of word cl / 64
+        constexpr int NW = (RSG * LPC + 63) / 64;
+        uint64_t word[NW];
+        int8_t xu[RSG], xv[RSG];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) word[w] = 0;
+#pragma unroll
+        for (int r = 0; r < RSG; ++r) {
+          xu[r] = cu[r] >= 0 ? xs[cu[r]] : (int8_t)0;
+          xv[r] = cu[r] >= 0 ? xs[cv[r]] : (int8_t)0;
+          const uint64_t gbits = (__ballot(xu[r] != xv[r]) >> (g * LPC)) & GMASK;
+          word[(LPC * r) / 64] |= gbits << ((LPC * r) % 64);
+        }
+        int n_diff = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) n_diff += __popcll(word[w]);
+        valid = n_diff > 0;
+        int k = (int)bounded(rnd.x, (uint32_t)max(n_diff, 1));
+        int sel = 0;
+        bool found = false;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+          const int cnt = __popcll(word[w]);
+          if (!found && k < cnt) {
+            sel = 64 * w + kth_set_bit(word[w], k);
+            found = true;
+          } else if (!found) {
+            k -= cnt;
+          }
+        }
+        if (!valid || sel >= sa.n_clusters) sel = 0;
+        s0 = cl_s[2 * sel];
+        s1 = cl_s[2 * sel + 1];
+        nf = 2;
+        // differing clusters after the swap (sigma'_u = sigma_v, sigma'_v = sigma_u)
+        const int8_t y0 = xs[s0], y1 = xs[s1];
+        int n_new = 0;
+#pragma unroll
+        for (int r = 0; r < RSG; ++r) {
+          const int a = cu[r], bb = cv[r];
+          const int8_t xa = a == s0 ? y1 : (a == s1 ? y0 : xu[r]);
+          const int8_t xb = bb == s0 ? y1 : (bb == s1 ? y0 : xv[r]);
+          n_new += __popcll((__ballot(a >= 0 && xa != xb) >> (g * LPC)) & GMASK);
+        }
+        corr = valid ? log_fast<T>((T)n_diff) - log_fast<T>((T)max(n_new, 1)) : (T)0;
+      }
+      // change of log psi with the new theta / log_cosh kept for a commit
+      const int8_t x0 = xs[s0], x1 = xs[s1];
+      T nth[JG], nlc[JG];
+      T dpart = (T)0;
+#pragma unroll
+      for (int jj = 0; jj < JG; ++jj) {
+        const int j = gl + LPC * ((jj + g) & (JG - 1));
+        nth[jj] = th[jj];
+        nlc[jj] = lc[jj];
+        if (j < M) {
+          const T w0 = W[(size_t)s0 * M + j];
+          T d = (x0 > 0) ? -2 * w0 : 2 * w0;
+          if (nf == 2) {
+            const T w1 = W[(size_t)s1 * M + j];
+            d += (x1 > 0) ? -2 * w1 : 2 * w1;
+          }
+          const T t = th[jj] + d;
+          const T l = log_cosh<T>(t);
+          nth[jj] = t;
+          nlc[jj] = l;
+          dpart += l - lc[jj];
+        }
+      }
+      T delta = group_sum<LPC, T>(dpart);
+      delta += (x0 > 0) ? -2 * a_s[s0] : 2 * a_s[s0];
+      if (nf == 2) delta += (x1 > 0) ? -2 * a_s[s1] : 2 * a_s[s1];
+      const T u = uniform01<T>(rnd);
+      const bool acc = valid && (u < exp_t<T>(pw * delta + corr));
+      if (acc) {
+#pragma unroll
+        for (int jj = 0; jj < JG; ++jj) {
+          th[jj] = nth[jj];
+          lc[jj] = nlc[jj];
+        }
+        if (gl == 0) {
+          xs[s0] = (int8_t)(-x0);
+          if (nf == 2) xs[s1] = (int8_t)(-x1);
+        }
+        lp += pw * delta;
+        ++nacc;
+      }
+      wave_lds_sync();
+    }
+    if (sw >= sa.n_discard && live) {
+      const int t = sw - sa.n_discard;
+      int8_t* dst = samples + ((int64_t)c * sa.chain_length + t) * N;
+      for (int i = gl; i < N; i += LPC) dst[i] = xs[i];
+      if (lp_samples && gl == 0) lp_samples[(int64_t)c * sa.chain_length + t] = lp;
+    }
+  }
+  if (live) {
+    for (int i = gl; i < N; i += LPC) sigma[(int64_t)c * N + i] = xs[i];
+    if (gl == 0) {
+      log_prob[c] = lp;
+      n_accepted[c] += nacc;
+    }
+  }
+}
+
 // counter += inc (one thread), after the sweep that consumed the counter
 __global__ void advance_counter_kernel(uint64_t* __restrict__ c, uint64_t inc) {
   if (threadIdx.x == 0 && blockIdx.x == 0) c[0] = c[0] + inc;
@@ -241,10 +501,65 @@ static bool stage_fits_s(const nka_rbm& m, int waves) {
          kLdsBudget;
 }
 
+template <class T>
+static size_t group_lds(const nka_rbm& m, const SweepArgs& sa, int waves, int G) {
+  return pad16((size_t)m.n_visible * m.n_hidden * sizeof(T)) + pad16((size_t)m.n_visible * sizeof(T)) +
+         pad16((size_t)2 * sa.n_clusters * sizeof(int32_t)) + (size_t)waves * G * pad16(m.n_visible);
+}
+
+// Chains per wavefront for the grouped kernel (0: one wavefront per chain):
+// 4 chains of 16 lanes when M <= 128, 2 of 32 lanes when M <= 256, provided
+// the clusters fit the per-lane slots and W is staged in LDS.
+template <class T>
+static int pick_group(const nka_rbm& m, const SweepArgs& sa, int waves) {
+  for (int G : {4, 2}) {
+    const int lpc = NKA_WAVE / G;
+    if (m.n_hidden > 8 * lpc) continue;
+    if (sa.rule == NKA_RULE_EXCHANGE && sa.n_clusters > kMaxClusterSlots * lpc) continue;
+    if (group_lds<T>(m, sa, waves, G) > kLdsBudget) continue;
+    return G;
+  }
+  return 0;
+}
+
+template <class T, int G, int JG, int RSG>
+static void launch_group_rs(const nka_rbm& m, const SweepArgs& sa, int8_t* sigma, T* lp, int8_t* smp,
+                            T* lps, int64_t* nacc, int waves, hipStream_t s) {
+  const size_t lds = group_lds<T>(m, sa, waves, G);
+  const int per_block = waves * G;
+  const unsigned g = (unsigned)((sa.C + per_block - 1) / per_block);
+  hipLaunchKernelGGL((rbm_metropolis_group_kernel<T, G, JG, RSG, true>), dim3(g), dim3(64 * waves), lds, s,
+                     m, sa, sigma, lp, smp, lps, nacc);
+}
+
+// cluster slots per lane rounded up to a power of two (1 for the local rule)
+template <class T, int G, int JG>
+static void launch_group(const nka_rbm& m, const SweepArgs& sa, int8_t* sigma, T* lp, int8_t* smp,
+                         T* lps, int64_t* nacc, int waves, hipStream_t s) {
+  const int lpc = NKA_WAVE / G;
+  const int need = sa.rule == NKA_RULE_EXCHANGE ? (sa.n_clusters + lpc - 1) / lpc : 1;
+  if (need <= 1) launch_group_rs<T, G, JG, 1>(m, sa, sigma, lp, smp, lps, nacc, waves, s);
+  else if (need <= 2) launch_group_rs<T, G, JG, 2>(m, sa, sigma, lp, smp, lps, nacc, waves, s);
+  else if (need <= 4) launch_group_rs<T, G, JG, 4>(m, sa, sigma, lp, smp, lps, nacc, waves, s);
+  else if (need <= 8) launch_group_rs<T, G, JG, 8>(m, sa, sigma, lp, smp, lps, nacc, waves, s);
+  else launch_group_rs<T, G, JG, 16>(m, sa, sigma, lp, smp, lps, nacc, waves, s);
+}
+
 template <int JL, class T>
 static void launch_sweep(const nka_rbm& m, const SweepArgs& sa, int8_t* sigma, T* lp, int8_t* smp,
                          T* lps, int64_t* nacc, hipStream_t s) {
   const int waves = 4;
+  const int G = pick_group<T>(m, sa, waves);
+  if (G) {
+    const int lpc = NKA_WAVE / G;
+    const int need = (m.n_hidden + lpc - 1) / lpc;
+    const int jg = need <= 1 ? 1 : need <= 2 ? 2 : need <= 4 ? 4 : 8;
+#define NKA_GROUP_CASE(GG, JJ) \
+    if (G == GG && jg == JJ) { launch_group<T, GG, JJ>(m, sa, sigma, lp, smp, lps, nacc, waves, s); return; }
+    NKA_GROUP_CASE(4, 1) NKA_GROUP_CASE(4, 2) NKA_GROUP_CASE(4, 4) NKA_GROUP_CASE(4, 8)
+    NKA_GROUP_CASE(2, 1) NKA_GROUP_CASE(2, 2) NKA_GROUP_CASE(2, 4) NKA_GROUP_CASE(2, 8)
+#undef NKA_GROUP_CASE
+  }
   const bool stage = stage_fits_s<T>(m, waves);
   const size_t lds = (stage ? pad16((size_t)m.n_visible * m.n_hidden * sizeof(T)) : 0) +
                      waves * pad16(m.n_visible);
