@@ -45,11 +45,13 @@ __device__ inline int64_t xcd_remap(int64_t b, int64_t G) {
 inline unsigned grid_mult8(int64_t tiles) { return (unsigned)(((tiles + 7) / 8) * 8); }
 
 // ---------------------------------------------------------------------------
-// bf16x3 Gram: 128x128 output tile per workgroup, 4 waves of 64x64, K step 32.
+// bf16x3 Gram: 128x128 output tile per workgroup, 4 waves of 64x64, K step 64.
 // ---------------------------------------------------------------------------
 constexpr int GB_T = 128;   // tile edge
-constexpr int GB_K = 32;    // K step (bf16 elements)
-constexpr int GB_LD = 40;   // LDS row stride in bf16 (32 + 8 pad: 80 bytes)
+constexpr int GB_K = 64;    // K step (bf16 elements): one 128-byte line per row and plane
+constexpr int GB_LD = 72;   // LDS row stride in bf16 (64 + 8 pad: 144 bytes)
+constexpr int GB_CPR = GB_K / 8;              // 16-byte chunks per row and plane
+constexpr int GB_NIT = GB_T * GB_CPR / 256;   // chunks per thread and plane
 
 __global__ void __launch_bounds__(256) gram_bf16x3_kernel(const uint16_t* __restrict__ xh,
                                                           const uint16_t* __restrict__ xl, int64_t n,
@@ -75,15 +77,16 @@ __global__ void __launch_bounds__(256) gram_bf16x3_kernel(const uint16_t* __rest
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[a][b][e] = 0.0f;
 
-  // Each operand slab is 128 rows x 32 k = 512 chunks of 8 bf16; thread tid
-  // moves chunks tid and tid + 256 of each of the (up to) four planes.  The
-  // next slab is fetched into registers while the current one is used.
-  uint4 pf[2][4];
+  // Each operand slab is 128 rows x GB_K k = 128 GB_CPR chunks of 8 bf16;
+  // thread tid moves chunks tid + 256 it (it < GB_NIT) of each of the (up to)
+  // four planes.  The next slab is fetched into registers while the current
+  // one is used.
+  uint4 pf[GB_NIT][4];
   auto fetch = [&](int64_t k0) {
 #pragma unroll
-    for (int it = 0; it < 2; ++it) {
+    for (int it = 0; it < GB_NIT; ++it) {
       const int chunk = tid + it * 256;
-      const int r = chunk >> 2, kc = (chunk & 3) * 8;
+      const int r = chunk / GB_CPR, kc = (chunk % GB_CPR) * 8;
       const int64_t kk = k0 + kc;
       uint4 z = {0, 0, 0, 0};
       pf[it][0] = z;
@@ -104,9 +107,9 @@ __global__ void __launch_bounds__(256) gram_bf16x3_kernel(const uint16_t* __rest
   for (int64_t k0 = 0; k0 < K; k0 += GB_K) {
     __syncthreads();
 #pragma unroll
-    for (int it = 0; it < 2; ++it) {
+    for (int it = 0; it < GB_NIT; ++it) {
       const int chunk = tid + it * 256;
-      const int r = chunk >> 2, kc = (chunk & 3) * 8;
+      const int r = chunk / GB_CPR, kc = (chunk % GB_CPR) * 8;
       *(uint4*)(&sA[0][r * GB_LD + kc]) = pf[it][0];
       *(uint4*)(&sA[1][r * GB_LD + kc]) = pf[it][1];
       if (!diag_tile) {
@@ -411,7 +414,8 @@ __device__ inline double rsqrt_refined(double a) {
 // A non-positive or non-finite pivot is replaced by 1 so the factor
 // completes; the return value is then the first failing column + 1 + k0
 // (0 when every pivot is positive).
-__device__ int factor_block_wave(double* Ab, double* colbuf, double* yv, int lane, int64_t k0) {
+__device__ __forceinline__ int factor_block_wave(double* Ab, double* colbuf, double* yv, int lane,
+                                                 int64_t k0) {
   int bad = 0;
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
@@ -475,7 +479,7 @@ __device__ int factor_block_wave(double* Ab, double* colbuf, double* yv, int lan
 // substitution, one row per lane with 16 values in registers (lanes 16..63
 // repeat the work of lane & 15 and do not store).  Ls = L (row-major, LDS),
 // rdiag[j] = 1/L_jj.
-__device__ void solve_rows16_wave(const double* Ls, const double* rdiag, double* X, int lane) {
+__device__ __forceinline__ void solve_rows16_wave(const double* Ls, const double* rdiag, double* X, int lane) {
   const int row = lane & 15;
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
