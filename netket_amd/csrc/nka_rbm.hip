@@ -178,10 +178,12 @@ __global__ void __launch_bounds__(256) rbm_prepare_kernel(nka_rbm m, const int8_
 // Weighted contraction over samples, split over sample chunks:
 //   part[c, v, p] = sum_{s in chunk c} O[s,p] w[v,s]
 // O[s, b_j] = t_sj, O[s, W_ij] = s_i t_sj, O[s, a_i] = s_i.
-// Each block owns a chunk of samples and a tile of hidden units; threads own
-// hidden units j and loop over the visible index i for the W block.
+// A workgroup owns a chunk of 64 samples and a tile of 64 hidden units; the
+// chunk's weights, spins and tanh values are staged in LDS (every global load
+// of the stage in flight at once), then lane j of wave w accumulates the W
+// entries of rows i = i0 + w + 4r of each 64-row slab over the 64 samples.
 // ---------------------------------------------------------------------------
-constexpr int kContractChunk = 256;   // samples per partial
+constexpr int kContractChunk = 64;    // samples per partial
 constexpr int kContractMaxW = 2;      // weight vectors per call
 
 template <class T, int NW>
@@ -189,20 +191,29 @@ __global__ void __launch_bounds__(256) rbm_contract_partial(nka_rbm m, const int
                                                             const T* __restrict__ t, int64_t ns,
                                                             const double* __restrict__ w,
                                                             double* __restrict__ part, int64_t np) {
-  // grid: x = chunk of kContractChunk samples, y = tile of 64 hidden units;
-  // block = 4 waves, lane = hidden unit j, wave wid takes visible rows
-  // i0 + wid + 4r of each 64-row slab.  The tanh values of the thread's hidden
-  // unit are read once per slab; partial sums stay in registers.
   __shared__ double ws[NW][kContractChunk];
   __shared__ int8_t sg[kContractChunk][64];
+  __shared__ T tl[kContractChunk][64];
   const int N = m.n_visible, M = m.n_hidden;
   const int64_t s0 = (int64_t)blockIdx.x * kContractChunk;
   const int cn = (int)min((int64_t)kContractChunk, ns - s0);
-  const int j = blockIdx.y * 64 + lane_id();
+  const int lane = lane_id();
+  const int j = blockIdx.y * 64 + lane;
   const int wid = threadIdx.x / 64;
-  for (int idx = threadIdx.x; idx < cn * NW; idx += blockDim.x) {
-    const int vv = idx / cn, s = idx % cn;
-    ws[vv][s] = w[(int64_t)vv * ns + s0 + s];
+  {
+    // tanh tile: wave w loads samples w + 4q (q < 16), lanes along j
+    T tv[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int s = wid + 4 * q;
+      tv[q] = (s < cn && j < M) ? t[(s0 + s) * M + j] : (T)0;
+    }
+    if (threadIdx.x < NW * kContractChunk) {
+      const int vv = threadIdx.x / kContractChunk, s = threadIdx.x % kContractChunk;
+      ws[vv][s] = s < cn ? w[(int64_t)vv * ns + s0 + s] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) tl[wid + 4 * q][lane] = tv[q];
   }
   // partial layout: [chunk][v][np + 1]; slot np holds the chunk's sum of w[v]
   const int64_t ld = np + 1;
@@ -210,9 +221,16 @@ __global__ void __launch_bounds__(256) rbm_contract_partial(nka_rbm m, const int
   for (int i0 = 0; i0 < N; i0 += 64) {
     const int ni = min(64, N - i0);
     __syncthreads();
-    for (int idx = threadIdx.x; idx < cn * 64; idx += blockDim.x) {
-      const int s = idx / 64, ii = idx % 64;
-      sg[s][ii] = ii < ni ? sigma[(s0 + s) * N + i0 + ii] : (int8_t)0;
+    {
+      // spins of the slab: wave w loads samples w + 4q, lanes along i
+      int8_t sv[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int s = wid + 4 * q;
+        sv[q] = (s < cn && lane < ni) ? sigma[(s0 + s) * N + i0 + lane] : (int8_t)0;
+      }
+#pragma unroll
+      for (int q = 0; q < 16; ++q) sg[wid + 4 * q][lane] = sv[q];
     }
     __syncthreads();
     double acc[NW][16];
@@ -220,19 +238,20 @@ __global__ void __launch_bounds__(256) rbm_contract_partial(nka_rbm m, const int
     for (int vv = 0; vv < NW; ++vv)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[vv][r] = 0.0;
-    if (j < M) {
-      for (int s = 0; s < cn; ++s) {
-        const double tv = (double)t[(s0 + s) * M + j];
-        double tw[NW];
+#pragma unroll 4
+    for (int s = 0; s < kContractChunk; ++s) {
+      const double tvs = (double)tl[s][lane];
+      double tw[NW];
 #pragma unroll
-        for (int vv = 0; vv < NW; ++vv) tw[vv] = tv * ws[vv][s];
+      for (int vv = 0; vv < NW; ++vv) tw[vv] = tvs * ws[vv][s];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int8_t sv = sg[s][wid + 4 * r];
+      for (int r = 0; r < 16; ++r) {
+        const int8_t sv = sg[s][wid + 4 * r];
 #pragma unroll
-          for (int vv = 0; vv < NW; ++vv) acc[vv][r] += sv > 0 ? tw[vv] : (sv < 0 ? -tw[vv] : 0.0);
-        }
+        for (int vv = 0; vv < NW; ++vv) acc[vv][r] += sv > 0 ? tw[vv] : (sv < 0 ? -tw[vv] : 0.0);
       }
+    }
+    if (j < M) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int ii = wid + 4 * r;
@@ -263,9 +282,9 @@ __global__ void __launch_bounds__(256) rbm_contract_partial(nka_rbm m, const int
 #pragma unroll
     for (int vv = 0; vv < NW; ++vv) {
       double sw = 0.0;
-      for (int s = lane_id(); s < cn; s += 64) sw += ws[vv][s];
+      for (int s = lane; s < cn; s += 64) sw += ws[vv][s];
       sw = wave_sum(sw);
-      if (lane_id() == 0) out[(int64_t)vv * ld + np] = sw;
+      if (lane == 0) out[(int64_t)vv * ld + np] = sw;
     }
   }
   // hidden-bias entries
@@ -274,9 +293,9 @@ __global__ void __launch_bounds__(256) rbm_contract_partial(nka_rbm m, const int
 #pragma unroll
     for (int vv = 0; vv < NW; ++vv) b2[vv] = 0.0;
     for (int s = 0; s < cn; ++s) {
-      const double tv = (double)t[(s0 + s) * M + j];
+      const double tvs = (double)tl[s][lane];
 #pragma unroll
-      for (int vv = 0; vv < NW; ++vv) b2[vv] += tv * ws[vv][s];
+      for (int vv = 0; vv < NW; ++vv) b2[vv] += tvs * ws[vv][s];
     }
 #pragma unroll
     for (int vv = 0; vv < NW; ++vv) out[(int64_t)vv * ld + m.bias_off + j] = b2[vv];
@@ -326,32 +345,94 @@ __device__ inline void store_out(int kind, void* a, void* b, int64_t idx, double
   }
 }
 
-// Parameter-major layout: X[p, s]; one thread per (p, s) with s fastest.
-template <class T>
+// Parameter-major layout: X[p, s].  A workgroup writes the rows of one
+// parameter group for a chunk of 512 samples: group g < N is the W block of
+// visible unit i = g (rows kernel_off + i M + j, j < M), group N the hidden
+// bias (rows bias_off + j), group N + 1 the visible bias (rows visible_off +
+// i).  Thread t covers 8 consecutive samples (s0 + 8 (t % 64) ..) of the rows
+// r = t / 64 + 4k, so each row is written as 8-element runs (one 16-byte
+// store per bf16 plane) and tanh values are read as 8-element runs of the
+// transposed tanh tT[j, s]; the chunk's spins of unit i are staged in LDS.
+// VEC: ns is a multiple of 8 (aligned 8-element runs of tT).
+constexpr int kJacChunk = 512;
+
+template <class T, bool VEC>
 __global__ void __launch_bounds__(256) rbm_jac_ps_kernel(nka_rbm m, const int8_t* __restrict__ sigma,
                                                          const T* __restrict__ tT, int64_t ns,
                                                          const double* __restrict__ mean, double scale,
                                                          int kind, void* out_a, void* out_b, int64_t np,
                                                          int64_t ldo) {
+  __shared__ int8_t sgi[kJacChunk];
   const int N = m.n_visible, M = m.n_hidden;
-  const int64_t total = np * ns;
-  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t p = idx / ns, s = idx % ns;
-    const int64_t o_idx = p * ldo + s;
-    double o;
-    if (m.bias_off >= 0 && p >= m.bias_off && p < m.bias_off + M) {
-      o = (double)tT[(p - m.bias_off) * ns + s];
-    } else if (p >= m.kernel_off && p < m.kernel_off + (int64_t)N * M) {
-      const int64_t q = p - m.kernel_off;
-      const int i = (int)(q / M), j = (int)(q % M);
-      const double tv = (double)tT[(int64_t)j * ns + s];
-      o = sigma[s * N + i] > 0 ? tv : -tv;
+  const int g = blockIdx.y;
+  const int64_t s0 = (int64_t)blockIdx.x * kJacChunk;
+  const int tid = threadIdx.x;
+  const int s8 = (tid & 63) * 8;
+  const int rsub = tid >> 6;
+  if (g < N) {
+    for (int q = tid; q < kJacChunk; q += 256)
+      sgi[q] = (s0 + q < ns) ? sigma[(s0 + q) * N + g] : (int8_t)1;
+    __syncthreads();
+  } else if ((g == N && m.bias_off < 0) || (g == N + 1 && m.visible_off < 0)) {
+    return;
+  }
+  const int rows = (g == N + 1) ? N : M;
+  const bool full = s0 + s8 + 8 <= ns;
+  for (int r = rsub; r < rows; r += 4) {
+    int64_t p;
+    T tv[8];
+    if (g == N + 1) {
+      p = m.visible_off + r;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int64_t s = s0 + s8 + e;
+        tv[e] = (s < ns && sigma[s * N + r] > 0) ? (T)1 : (T)-1;
+      }
     } else {
-      const int i = (int)(p - m.visible_off);
-      o = sigma[s * N + i] > 0 ? 1.0 : -1.0;
+      p = (g == N) ? m.bias_off + r : m.kernel_off + (int64_t)g * M + r;
+      const T* src = tT + (int64_t)r * ns + s0 + s8;
+      if (VEC && full) {
+        if (sizeof(T) == 4) {
+          const float4 a = *(const float4*)src, b = *(const float4*)(src + 4);
+          tv[0] = a.x; tv[1] = a.y; tv[2] = a.z; tv[3] = a.w;
+          tv[4] = b.x; tv[5] = b.y; tv[6] = b.z; tv[7] = b.w;
+        } else {
+#pragma unroll
+          for (int e = 0; e < 8; e += 2) {
+            const double2 a = *(const double2*)(src + e);
+            tv[e] = (T)a.x;
+            tv[e + 1] = (T)a.y;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) tv[e] = (s0 + s8 + e < ns) ? src[e] : (T)0;
+      }
+      if (g < N) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) tv[e] = sgi[s8 + e] > 0 ? tv[e] : -tv[e];
+      }
     }
-    store_out(kind, out_a, out_b, o_idx, (o - mean[p]) * scale);
+    const double mp = mean[p];
+    const int64_t o_idx = p * ldo + s0 + s8;
+    if (kind == NKA_OUT_BF16X2 && full) {
+      uint32_t hw[4], lw[4];
+#pragma unroll
+      for (int e = 0; e < 8; e += 2) {
+        const float f0 = (float)(((double)tv[e] - mp) * scale);
+        const float f1 = (float)(((double)tv[e + 1] - mp) * scale);
+        const uint16_t h0 = bf16_rne(f0), h1 = bf16_rne(f1);
+        const uint16_t l0 = bf16_rne(f0 - bf16_to_f(h0)), l1 = bf16_rne(f1 - bf16_to_f(h1));
+        hw[e / 2] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+        lw[e / 2] = (uint32_t)l0 | ((uint32_t)l1 << 16);
+      }
+      *(uint4*)((uint16_t*)out_a + o_idx) = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+      *(uint4*)((uint16_t*)out_b + o_idx) = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        if (s0 + s8 + e < ns) store_out(kind, out_a, out_b, o_idx + e, ((double)tv[e] - mp) * scale);
+    }
   }
 }
 
@@ -726,12 +807,27 @@ int nka_rbm_jacobian(const nka_rbm* m, const int8_t* sigma, const void* t, const
   const int64_t total = np * ns;
   const unsigned g = (unsigned)std::min<int64_t>((total + 255) / 256, 65536);
   if (layout == NKA_LAYOUT_PS) {
-    if (m->dtype == NKA_F32)
-      hipLaunchKernelGGL(rbm_jac_ps_kernel<float>, dim3(g), dim3(256), 0, s, *m, sigma,
-                         (const float*)tT, ns, mean, scale, out_kind, out_a, out_b, np, ldo);
-    else
-      hipLaunchKernelGGL(rbm_jac_ps_kernel<double>, dim3(g), dim3(256), 0, s, *m, sigma,
-                         (const double*)tT, ns, mean, scale, out_kind, out_a, out_b, np, ldo);
+    // 16-byte runs of the bf16 planes need 8-aligned rows of the output
+    NKA_CHECK_ARG(out_kind != NKA_OUT_BF16X2 || (ldo % 8 == 0 && (uintptr_t)out_a % 16 == 0 &&
+                                                 (uintptr_t)out_b % 16 == 0),
+                  "nka_rbm_jacobian: bf16 planes need ldo %% 8 == 0 and 16-byte alignment");
+    const dim3 grid((unsigned)((ns + kJacChunk - 1) / kJacChunk), (unsigned)(m->n_visible + 2));
+    const bool vec = ns % 8 == 0 && (uintptr_t)tT % 16 == 0;
+    if (m->dtype == NKA_F32) {
+      if (vec)
+        hipLaunchKernelGGL((rbm_jac_ps_kernel<float, true>), grid, dim3(256), 0, s, *m, sigma,
+                           (const float*)tT, ns, mean, scale, out_kind, out_a, out_b, np, ldo);
+      else
+        hipLaunchKernelGGL((rbm_jac_ps_kernel<float, false>), grid, dim3(256), 0, s, *m, sigma,
+                           (const float*)tT, ns, mean, scale, out_kind, out_a, out_b, np, ldo);
+    } else {
+      if (vec)
+        hipLaunchKernelGGL((rbm_jac_ps_kernel<double, true>), grid, dim3(256), 0, s, *m, sigma,
+                           (const double*)tT, ns, mean, scale, out_kind, out_a, out_b, np, ldo);
+      else
+        hipLaunchKernelGGL((rbm_jac_ps_kernel<double, false>), grid, dim3(256), 0, s, *m, sigma,
+                           (const double*)tT, ns, mean, scale, out_kind, out_a, out_b, np, ldo);
+    }
   } else {
     if (m->dtype == NKA_F32)
       hipLaunchKernelGGL(rbm_jac_sp_kernel<float>, dim3(g), dim3(256), 0, s, *m, sigma,
