@@ -270,10 +270,10 @@ constexpr int G2_K = 16;
 constexpr int G2_LD = G2_T + 4;   // doubles per k row of an LDS tile
 
 template <bool VEC>
-__device__ inline void gemm_big_tile(const double* __restrict__ a, int64_t lda, const double* __restrict__ b,
+__device__ __forceinline__ void gemm_big_tile(const double* __restrict__ a, int64_t lda, const double* __restrict__ b,
                                      int64_t ldb, double* __restrict__ c, int64_t ldc, int64_t m, int64_t n,
                                      int64_t K, double alpha, double beta, int lower_only, int64_t tiles_n,
-                                     int64_t bid, double* sA, double* sB) {
+                                     int64_t bid, double* sA, double* sB, double* __restrict__ wout) {
   int ti, tj;
   if (lower_only) {
     tri_supertile(bid, tiles_n, ti, tj);
@@ -339,6 +339,17 @@ __device__ inline void gemm_big_tile(const double* __restrict__ a, int64_t lda, 
     }
   }
   // element e of lane l in a 16x16 block is (row l/16 + 4e, col l%16)
+  if (wout) {
+    // partial product of a K slice: the raw 128x128 tile, row-major
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+      for (int y = 0; y < 4; ++y)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          wout[(wm + x * 16 + (lane >> 4) + 4 * e) * G2_T + wn + y * 16 + (lane & 15)] = acc[x][y][e];
+    return;
+  }
 #pragma unroll
   for (int x = 0; x < 4; ++x)
 #pragma unroll
@@ -355,22 +366,67 @@ __device__ inline void gemm_big_tile(const double* __restrict__ a, int64_t lda, 
       }
 }
 
+
 // One tile per workgroup (a grid smaller than the tile count makes each
 // workgroup walk tiles bid, bid + G, ..., consecutive in the super-block order).
-template <bool VEC>
+// SPLIT = false: one tile per unit u < n_units (tile ids from 0), whole K,
+// C updated.  SPLIT = true: unit u is slice u % S (kc columns of K) of tile
+// n_main + u / S, whose raw product goes to work + u * 128 * 128 for
+// gemm_big_reduce_kernel.
+template <bool VEC, bool SPLIT>
 __global__ void __launch_bounds__(256, 2) gemm_nt_f64_big_kernel(const double* __restrict__ a, int64_t lda,
                                                               const double* __restrict__ b, int64_t ldb,
                                                               double* __restrict__ c, int64_t ldc, int64_t m,
                                                               int64_t n, int64_t K, double alpha, double beta,
                                                               int lower_only, int64_t tiles_n,
-                                                              int64_t n_tiles_total, const int32_t* info) {
+                                                              int64_t n_units, int64_t n_main, int S,
+                                                              int64_t kc, double* __restrict__ work,
+                                                              const int32_t* info) {
   if (info && *info != 0) return;
   __shared__ double sA[G2_K * G2_LD];
   __shared__ double sB[G2_K * G2_LD];
   const int64_t G = gridDim.x;
-  for (int64_t t = xcd_remap(blockIdx.x, G); t < n_tiles_total; t += G) {
-    gemm_big_tile<VEC>(a, lda, b, ldb, c, ldc, m, n, K, alpha, beta, lower_only, tiles_n, t, sA, sB);
+  for (int64_t u = xcd_remap(blockIdx.x, G); u < n_units; u += G) {
+    if (!SPLIT) {
+      gemm_big_tile<VEC>(a, lda, b, ldb, c, ldc, m, n, K, alpha, beta, lower_only, tiles_n, u, sA, sB, nullptr);
+    } else {
+      const int64_t tile = n_main + u / S;
+      const int64_t kb = (u % S) * kc, kl = min<int64_t>(kc, K - kb);
+      gemm_big_tile<VEC>(a + kb, lda, b + kb, ldb, c, ldc, m, n, kl, alpha, beta, lower_only, tiles_n, tile,
+                         sA, sB, work + u * (int64_t)(G2_T * G2_T));
+    }
     __syncthreads();
+  }
+}
+
+// C = beta C + alpha sum_part W[part] for the split tail tiles, parts summed
+// in a fixed order.  One workgroup per (tail tile, 32-row band).
+__global__ void __launch_bounds__(256) gemm_big_reduce_kernel(const double* __restrict__ work, double* __restrict__ c,
+                                                              int64_t ldc, int64_t m, int64_t n, double alpha,
+                                                              double beta, int lower_only, int64_t tiles_n,
+                                                              int64_t n_main, int S, const int32_t* info) {
+  if (info && *info != 0) return;
+  const int64_t r = blockIdx.x / 4;  // tail tile index
+  const int band = blockIdx.x % 4;
+  const int64_t tile = n_main + r;
+  int ti, tj;
+  if (lower_only) {
+    tri_supertile(tile, tiles_n, ti, tj);
+  } else {
+    ti = (int)(tile / tiles_n);
+    tj = (int)(tile % tiles_n);
+  }
+  const double* w = work + r * S * (int64_t)(G2_T * G2_T);
+  for (int idx = threadIdx.x; idx < 32 * G2_T; idx += 256) {
+    const int li = band * 32 + idx / G2_T, lj = idx % G2_T;
+    const int64_t gi = (int64_t)ti * G2_T + li, gj = (int64_t)tj * G2_T + lj;
+    if (gi < m && gj < n && (!lower_only || gj <= gi)) {
+      double acc = 0.0;
+      for (int p = 0; p < S; ++p) acc += w[(int64_t)p * G2_T * G2_T + li * G2_T + lj];
+      double v = alpha * acc;
+      if (beta != 0.0) v += beta * c[gi * ldc + gj];
+      c[gi * ldc + gj] = v;
+    }
   }
 }
 
@@ -1123,9 +1179,32 @@ static int gemm_launch(const double* a, int64_t lda, const double* b, int64_t ld
 
 // The 128-tile kernel needs K to be a multiple of its K step; other shapes
 // take the 64-tile kernel.
+// Workgroups of the 128-tile kernel resident at once (two per CU), from the
+// device's CU count (queried once per process for the current device).
+static int64_t big_gemm_slots() {
+  static int64_t slots = 0;
+  if (slots == 0) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+      slots = 2 * (int64_t)cus;
+    else
+      slots = 512;
+  }
+  return slots;
+}
+
+// Tail-split workspace of the 128-tile GEMM: at most kMaxSplitUnits raw tiles.
+constexpr int64_t kMaxSplitUnits = 1024;
+
+// The 128-tile kernel needs K to be a multiple of its K step; other shapes
+// take the 64-tile kernel.  With a workspace, the tiles of the last, partial
+// round of the grid are split along K so that the round is filled, and their
+// slices are summed into C by a second launch.
 static void gemm_big_launch(const double* a, int64_t lda, const double* b, int64_t ldb, double* c,
                             int64_t ldc, int64_t m, int64_t n, int64_t k, double alpha, double beta,
-                            int lower, const int32_t* info, hipStream_t s) {
+                            int lower, const int32_t* info, hipStream_t s, double* work = nullptr,
+                            int64_t work_units = 0) {
   if (m <= 0 || n <= 0) return;
   if (k % G2_K != 0) {
     gemm_launch(a, lda, b, ldb, c, ldc, m, n, k, alpha, beta, 0.0, lower, info, s);
@@ -1133,14 +1212,42 @@ static void gemm_big_launch(const double* a, int64_t lda, const double* b, int64
   }
   const int64_t tm = (m + G2_T - 1) / G2_T, tn = (n + G2_T - 1) / G2_T;
   const int64_t tiles = lower ? tm * (tm + 1) / 2 : tm * tn;
+  const int64_t slots = big_gemm_slots();
+  const int64_t n_main = (tiles / slots) * slots;
+  const int64_t tail = tiles - n_main;
+  // slices per tail tile: fill the round, keep >= 4 K steps per slice
+  int S = 1;
+  if (work && tail > 0) {
+    const int64_t max_s = std::max<int64_t>(1, k / (4 * G2_K));
+    S = (int)std::min<int64_t>(std::min<int64_t>(slots / tail, max_s), work_units / tail);
+    if (S < 2) S = 1;
+  }
+  const int64_t kc = S > 1 ? ((k + S - 1) / S + G2_K - 1) / G2_K * G2_K : k;
+  if (S > 1) S = (int)((k + kc - 1) / kc);  // slices actually needed with kc columns each
+  const int64_t main_tiles = S > 1 ? n_main : tiles;
   const bool vec = ((uintptr_t)a % 16 == 0) && ((uintptr_t)b % 16 == 0) && lda % 2 == 0 && ldb % 2 == 0;
-  const unsigned grid = grid_mult8(tiles);
-  if (vec)
-    hipLaunchKernelGGL(gemm_nt_f64_big_kernel<true>, dim3(grid), dim3(256), 0, s, a, lda, b,
-                       ldb, c, ldc, m, n, k, alpha, beta, lower, tn, tiles, info);
-  else
-    hipLaunchKernelGGL(gemm_nt_f64_big_kernel<false>, dim3(grid), dim3(256), 0, s, a, lda, b,
-                       ldb, c, ldc, m, n, k, alpha, beta, lower, tn, tiles, info);
+  if (main_tiles > 0) {
+    const unsigned grid = grid_mult8(main_tiles);
+    if (vec)
+      hipLaunchKernelGGL((gemm_nt_f64_big_kernel<true, false>), dim3(grid), dim3(256), 0, s, a, lda, b, ldb, c,
+                         ldc, m, n, k, alpha, beta, lower, tn, main_tiles, (int64_t)0, 1, k, work, info);
+    else
+      hipLaunchKernelGGL((gemm_nt_f64_big_kernel<false, false>), dim3(grid), dim3(256), 0, s, a, lda, b, ldb,
+                         c, ldc, m, n, k, alpha, beta, lower, tn, main_tiles, (int64_t)0, 1, k, work, info);
+  }
+  if (S > 1) {
+    const int64_t su = tail * S;
+    const unsigned grid = grid_mult8(su);
+    if (vec)
+      hipLaunchKernelGGL((gemm_nt_f64_big_kernel<true, true>), dim3(grid), dim3(256), 0, s, a, lda, b, ldb, c,
+                         ldc, m, n, k, alpha, beta, lower, tn, su, main_tiles, S, kc, work, info);
+    else
+      hipLaunchKernelGGL((gemm_nt_f64_big_kernel<false, true>), dim3(grid), dim3(256), 0, s, a, lda, b, ldb,
+                         c, ldc, m, n, k, alpha, beta, lower, tn, su, main_tiles, S, kc, work, info);
+  }
+  if (S > 1)
+    hipLaunchKernelGGL(gemm_big_reduce_kernel, dim3((unsigned)(tail * 4)), dim3(256), 0, s, work, c, ldc, m, n,
+                       alpha, beta, lower, tn, main_tiles, S, info);
 }
 
 int nka_syrk_f64(const double* x, int64_t n, int64_t k, int64_t ldx, double alpha, double beta,
@@ -1162,13 +1269,20 @@ int nka_gemm_nt_f64(const double* a, int64_t lda, const double* b, int64_t ldb, 
 }
 
 // Workspace: 64x64 inverses of the diagonal blocks, the intermediate vector
-// of the solves, one flag per block and one error word.
+// of the solves, one flag per block and one error word, and the K-slices of
+// the trailing products' split tail tiles (up to kMaxSplitUnits raw 128x128
+// tiles, fewer for small n).
+static int64_t split_units(int64_t n) {
+  const int64_t t = (n + G2_T - 1) / G2_T;
+  return std::min<int64_t>(kMaxSplitUnits, 2 * t * (t + 1) / 2);
+}
 
 int64_t nka_potrf_workspace(int64_t n) {
   if (n <= 0) return 0;
   const int64_t nblk = (n + CH_NB - 1) / CH_NB;
-  return nblk * CH_NB * CH_NB * (int64_t)sizeof(double) + (n + 64) * (int64_t)sizeof(double) +
-         (nblk + 64) * (int64_t)sizeof(uint32_t);
+  const int64_t head = nblk * CH_NB * CH_NB * (int64_t)sizeof(double) + (n + 64) * (int64_t)sizeof(double) +
+                       (nblk + 64) * (int64_t)sizeof(uint32_t);
+  return ((head + 255) / 256) * 256 + split_units(n) * G2_T * G2_T * (int64_t)sizeof(double);
 }
 
 static inline double* ws_inv(void* ws) { return (double*)ws; }
@@ -1177,6 +1291,11 @@ static inline double* ws_vec(void* ws, int64_t nblk) {
 }
 static inline uint32_t* ws_flags(void* ws, int64_t n, int64_t nblk) {
   return (uint32_t*)(ws_vec(ws, nblk) + n + 64);
+}
+static inline double* ws_split(void* ws, int64_t n, int64_t nblk) {
+  const int64_t head = nblk * CH_NB * CH_NB * (int64_t)sizeof(double) + (n + 64) * (int64_t)sizeof(double) +
+                       (nblk + 64) * (int64_t)sizeof(uint32_t);
+  return (double*)((char*)ws + ((head + 255) / 256) * 256);
 }
 
 // Right-looking Cholesky with up to three levels of blocking.  An outer
@@ -1199,6 +1318,8 @@ int nka_potrf_f64(double* a, int64_t n, int64_t lda, void* workspace, int32_t* i
   NKA_CHECK_ARG(nblk < (1LL << 31), "nka_potrf_f64: too large");
   // panel widths measured best on MI355X: n = 8384 -> (512, 512), n = 32768 -> (4096, 512)
   const int64_t NB0 = n >= 16384 ? 4096 : 512, NB1 = 512;
+  double* split = ws_split(workspace, n, nblk);
+  const int64_t n_split = split_units(n);
   for (int64_t K0 = 0; K0 < n; K0 += NB0) {
     const int64_t n0 = std::min<int64_t>(NB0, n - K0);
     for (int64_t K1 = K0; K1 < K0 + n0; K1 += NB1) {
@@ -1229,10 +1350,10 @@ int nka_potrf_f64(double* a, int64_t n, int64_t lda, void* workspace, int32_t* i
         const int64_t rows1 = n - (K1 + n1);
         const double* l = a + (K1 + n1) * lda + K1;
         double* c = a + (K1 + n1) * lda + (K1 + n1);
-        gemm_big_launch(l, lda, l, lda, c, lda, cols1, cols1, n1, -1.0, 1.0, 1, info, s);
+        gemm_big_launch(l, lda, l, lda, c, lda, cols1, cols1, n1, -1.0, 1.0, 1, info, s, split, n_split);
         if (rows1 > cols1)
           gemm_big_launch(l + cols1 * lda, lda, l, lda, c + cols1 * lda, lda, rows1 - cols1, cols1, n1, -1.0,
-                          1.0, 0, info, s);
+                          1.0, 0, info, s, split, n_split);
       }
     }
     // trailing update (K = 1024)
@@ -1240,7 +1361,7 @@ int nka_potrf_f64(double* a, int64_t n, int64_t lda, void* workspace, int32_t* i
     if (rest2 > 0) {
       const double* l21 = a + (K0 + n0) * lda + K0;
       double* a22 = a + (K0 + n0) * lda + (K0 + n0);
-      gemm_big_launch(l21, lda, l21, lda, a22, lda, rest2, rest2, n0, -1.0, 1.0, 1, info, s);
+      gemm_big_launch(l21, lda, l21, lda, a22, lda, rest2, rest2, n0, -1.0, 1.0, 1, info, s, split, n_split);
     }
   }
   hipLaunchKernelGGL(trtri_blocks_kernel, dim3((unsigned)nblk), dim3(64), 0, s, a, lda, n, inv, info);
