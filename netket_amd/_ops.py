@@ -7,6 +7,8 @@ return torch tensors resident on the GPU.
 
 from __future__ import annotations
 
+import contextlib
+
 import ctypes as C
 from contextlib import contextmanager
 
@@ -260,6 +262,22 @@ class Workspace:
 
 
 WORKSPACE = Workspace()
+
+
+@contextlib.contextmanager
+def workspace_scope(ws: Workspace):
+    """Route every scratch request of the enclosed calls to ``ws``.  A captured
+    HIP graph keeps the raw addresses of the scratch buffers it was recorded
+    with, so the capturing driver owns its buffers for the graph's lifetime
+    instead of sharing the grow-only global cache, whose buffers are replaced
+    (and freed) when a later request on the same stream handle needs more."""
+    global WORKSPACE
+    prev = WORKSPACE
+    WORKSPACE = ws
+    try:
+        yield ws
+    finally:
+        WORKSPACE = prev
 
 
 def potrf_(a: torch.Tensor, info: torch.Tensor, ws: torch.Tensor | None = None):

@@ -125,15 +125,19 @@ class AbstractVariationalDriver:
         """
         self._graph_stream = torch.cuda.Stream()
         self._graph_stream.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(self._graph_stream):
-            for _ in range(warmup):
-                self._eager_step()
-                self._step_count += 1
-        torch.cuda.current_stream().wait_stream(self._graph_stream)
-        torch.cuda.synchronize()
-        g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g, stream=self._graph_stream):
-            st = self._eager_step()
+        # scratch buffers owned by this driver for as long as the graph lives
+        # (stream handles come from a pool and are shared between drivers)
+        self._graph_ws = _ops.Workspace()
+        with _ops.workspace_scope(self._graph_ws):
+            with torch.cuda.stream(self._graph_stream):
+                for _ in range(warmup):
+                    self._eager_step()
+                    self._step_count += 1
+            torch.cuda.current_stream().wait_stream(self._graph_stream)
+            torch.cuda.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=self._graph_stream):
+                st = self._eager_step()
         self._graph = g
         self._graph_stats = st
         return g
