@@ -117,9 +117,15 @@ typedef struct nka_ising {
  * get_conn_padded: eloc[s] = sum_k mel_k exp(log psi(x'_k) - log psi(x_s)),
  * evaluated with the hidden-unit update theta' = theta - 2 sigma_i W_i. so the
  * connected configurations are never materialised.  eloc is FP64; log_psi
- * (model dtype) may be NULL. */
+ * (model dtype) may be NULL.  An Ising operator with a transverse field on an
+ * FP32 model whose W exceeds 64 KiB takes a path tiled over samples and hidden
+ * units that needs a workspace: nka_rbm_local_energy_workspace returns its
+ * size for n samples (0 when no workspace is needed); a smaller one, of at
+ * least the size for 64 samples, processes the batch in chunks. */
+int64_t nka_rbm_local_energy_workspace(const nka_rbm* m, int op_kind, const void* op, int64_t n);
 int nka_rbm_local_energy(const nka_rbm* m, int op_kind, const void* op, const int8_t* sigma,
-                         int64_t n, double* eloc, void* log_psi, void* stream);
+                         int64_t n, double* eloc, void* log_psi, void* workspace,
+                         int64_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
  * Two-layer dense network: log psi(x) = sum_k log_cosh(z2_k) with

@@ -28,8 +28,11 @@ def local_energies(model, flat, op, sigma2d: torch.Tensor, log_psi: bool = False
             nat.call("nka_ffnn_local_energy", C.byref(m), kind, op_ptr, _ops.ptr(sigma2d), n,
                      _ops.ptr(eloc), _ops.ptr(lp), _ops.ptr(ws), ws.numel(), _ops.stream())
         else:
+            wsb = nat.lib().nka_rbm_local_energy_workspace(C.byref(m), kind, op_ptr, n)
+            ws = _ops.WORKSPACE.get("rbm_eloc", wsb) if wsb > 0 else None
             nat.call("nka_rbm_local_energy", C.byref(m), kind, op_ptr, _ops.ptr(sigma2d), n,
-                     _ops.ptr(eloc), _ops.ptr(lp), _ops.stream())
+                     _ops.ptr(eloc), _ops.ptr(lp), _ops.ptr(ws), 0 if ws is None else ws.numel(),
+                     _ops.stream())
     del keep
     return (eloc, lp) if log_psi else eloc
 

@@ -108,7 +108,8 @@ SIGNATURES = {
     "nka_ffnn_ntk": (i32, [C.POINTER(NkaFfnn), vp, vp, vp, vp, i64, vp, f64, f64, f64, vp, i64, vp, vp]),
     "nka_ffnn_jacobian": (i32, [C.POINTER(NkaFfnn), vp, vp, vp, vp, i64, vp, f64, i32, i32, vp, vp, i64,
                                 vp]),
-    "nka_rbm_local_energy": (i32, [C.POINTER(NkaRbm), i32, vp, vp, i64, vp, vp, vp]),
+    "nka_rbm_local_energy_workspace": (i64, [C.POINTER(NkaRbm), i32, vp, i64]),
+    "nka_rbm_local_energy": (i32, [C.POINTER(NkaRbm), i32, vp, vp, i64, vp, vp, vp, i64, vp]),
     "nka_random_states": (i32, [vp, i32, i32, i32, u64, u64, vp]),
     "nka_rbm_metropolis": (
         i32,

@@ -566,7 +566,7 @@ typedef float ffx16 __attribute__((ext_vector_type(16)));
 typedef double ddx4 __attribute__((ext_vector_type(4)));
 constexpr int PG_T = 64, PG_K = 16;
 
-template <class T>
+template <class T, bool RAW>
 __global__ void __launch_bounds__(256) rbm_prepare_gemm_kernel(nka_rbm m, const int8_t* __restrict__ sigma,
                                                                int64_t ns, T* __restrict__ t, T* __restrict__ tT) {
   __shared__ T sA[PG_T][PG_K + 1];
@@ -624,8 +624,13 @@ __global__ void __launch_bounds__(256) rbm_prepare_gemm_kernel(nka_rbm m, const 
       }
     }
   }
+  // RAW: theta itself into tT, with the columns up to the next multiple of 64
+  // (zero beyond M) written too
+  const int Mp = (M + PG_T - 1) / PG_T * PG_T;
   auto store = [&](int64_t sr, int j, T th) {
-    if (sr < ns && j < M) {
+    if (RAW) {
+      if (sr < ns && j < Mp) tT[(int64_t)j * ns + sr] = j < M ? th + (v.b ? v.b[j] : (T)0) : (T)0;
+    } else if (sr < ns && j < M) {
       const T tv = tanh_t<T>(th + (v.b ? v.b[j] : (T)0));
       if (t) t[sr * M + j] = tv;
       if (tT) tT[(int64_t)j * ns + sr] = tv;
@@ -655,7 +660,7 @@ static void launch_prepare(const nka_rbm& m, const int8_t* sigma, int64_t ns, T*
   const bool stage = stage_fits<T>(m, waves);
   if (!stage) {
     dim3 grid((unsigned)((ns + PG_T - 1) / PG_T), (unsigned)((m.n_hidden + PG_T - 1) / PG_T));
-    hipLaunchKernelGGL((rbm_prepare_gemm_kernel<T>), grid, dim3(256), 0, s, m, sigma, ns, t, tT);
+    hipLaunchKernelGGL((rbm_prepare_gemm_kernel<T, false>), grid, dim3(256), 0, s, m, sigma, ns, t, tT);
     return;
   }
   const size_t lds = (stage ? pad16((size_t)m.n_visible * m.n_hidden * sizeof(T)) : 0) +
@@ -677,6 +682,217 @@ template <int JL>
 static void launch_prepare_f64(const nka_rbm& m, const int8_t* sg, int64_t ns, void* t, void* tT,
                                hipStream_t s) {
   launch_prepare<JL, double>(m, sg, ns, (double*)t, (double*)tT, s);
+}
+
+// ---------------------------------------------------------------------------
+// Ising local energy, tiled over samples and hidden units, for FP32 models
+// whose W does not fit in LDS.  For a flip of site i,
+//   log psi(x') - log psi(x) = -2 x_i a_i + sum_j [lc(t_ij) - lc(theta_j)],
+//   t_ij = theta_j - 2 x_i W_ij,  lc(t) = |t| + log(1 + e^{-2|t|}) - log 2,
+// so with t' = kappa t (kappa = 2 / ln 2) one unit costs an FMA, the change of
+// |t'|, one exp2 and a product update:
+//   sum_j [lc(t_j) - lc(theta_j)] = (1/kappa) sum_j (|t'_j| - |theta'_j|)
+//                                  + ln2 log2( prod_j (1 + 2^-|t'_j|) / P0 ),
+// with P0 the same product at theta (< 2^64 over 64 units).  A workgroup takes
+// 64 samples (one per lane) x 256 units (64 per wave, theta' in registers) and
+// walks all sites; W' rows are wave-uniform scalar loads.  The four waves'
+// partial sums are added in LDS in a fixed order per block of sites and
+// written as part[unit tile][site][sample]; a second kernel adds the tiles,
+// exponentiates and adds the diagonal.  W is read once per 64 samples instead
+// of once per sample.
+// ---------------------------------------------------------------------------
+constexpr int IF_IB = 8;          // sites per reduction block
+constexpr int IF_JT = 256;        // hidden units per workgroup
+constexpr float kKappa = 2.8853900817779268f;    // 2 / ln 2
+constexpr float kLn2f = 0.69314718055994531f;
+
+__global__ void __launch_bounds__(256) ising_scale_w_kernel(const float* __restrict__ W, int N, int M,
+                                                            int Mp, float* __restrict__ Wp) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (int64_t)N * Mp) return;
+  const int i = (int)(e / Mp), j = (int)(e % Mp);
+  Wp[e] = j < M ? kKappa * W[(int64_t)i * M + j] : 0.0f;
+}
+
+__global__ void __launch_bounds__(256) ising_flip_tile_kernel(
+    const float* __restrict__ thT, const float* __restrict__ Wp, const int8_t* __restrict__ sigma,
+    int64_t ns, int N, int Mp, const float* __restrict__ a, nka_ising is, int want_lp,
+    float* __restrict__ part, double* __restrict__ diag) {
+  extern __shared__ uint32_t bits[];             // [words][64]: bit i%32 of word i/32 = (x_i > 0)
+  __shared__ float red[2][4][IF_IB][64];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int64_t s0 = (int64_t)blockIdx.x * 64;
+  const int jt = blockIdx.y;
+  const int jc = jt * IF_JT + w * 64;
+  const int nw = (N + 31) >> 5;
+  for (int e = tid; e < nw * 64; e += 256) {
+    const int r = e & 63, k = e >> 6;
+    uint32_t word = 0;
+    if (s0 + r < ns) {
+      const int8_t* row = sigma + (s0 + r) * N + 32 * k;
+      const int lim = min(32, N - 32 * k);
+      for (int b = 0; b < lim; ++b) word |= (row[b] > 0 ? 1u : 0u) << b;
+    }
+    bits[k * 64 + r] = word;
+  }
+  __syncthreads();
+
+  const int64_t s = min(s0 + lane, ns - 1);
+  const bool active = jc < Mp;                   // wave-uniform
+  float th[64];
+  float rP0 = 1.0f, lcs = 0.0f;
+  if (active) {
+#pragma unroll
+    for (int u = 0; u < 64; ++u) th[u] = thT[(int64_t)(jc + u) * ns + s];
+    float P0 = 1.0f;
+#pragma unroll
+    for (int u = 0; u < 64; ++u) {
+      if (want_lp) lcs += log_cosh<float>(th[u]);
+      th[u] *= kKappa;
+      P0 = fmaf(P0, __builtin_amdgcn_exp2f(-fabsf(th[u])), P0);
+    }
+    rP0 = 1.0f / P0;
+  }
+
+  int buf = 0;
+  for (int ib = 0; ib < N; ib += IF_IB) {
+    for (int ii = 0; ii < IF_IB; ++ii) {
+      const int i = ib + ii;
+      float val = 0.0f;
+      if (active && i < N) {
+        const float g = ((bits[(i >> 5) * 64 + lane] >> (i & 31)) & 1u) ? -2.0f : 2.0f;
+        const float* wr = Wp + (int64_t)i * Mp + jc;
+        float A = 0.0f, P = 1.0f;
+#pragma unroll
+        for (int u = 0; u < 64; ++u) {
+          const float t = fmaf(g, wr[u], th[u]);
+          A += fabsf(t) - fabsf(th[u]);
+          P = fmaf(P, __builtin_amdgcn_exp2f(-fabsf(t)), P);
+        }
+        val = A * (1.0f / kKappa) + kLn2f * __builtin_amdgcn_logf(P * rP0);
+      }
+      red[buf][w][ii][lane] = val;
+    }
+    __syncthreads();
+    for (int e = tid; e < IF_IB * 64; e += 256) {
+      const int ii = e >> 6, r = e & 63, i = ib + ii;
+      if (i < N && s0 + r < ns) {
+        float v = ((red[buf][0][ii][r] + red[buf][1][ii][r]) + red[buf][2][ii][r]) + red[buf][3][ii][r];
+        if (jt == 0 && a) v += ((bits[(i >> 5) * 64 + r] >> (i & 31)) & 1u) ? -2.0f * a[i] : 2.0f * a[i];
+        part[((int64_t)jt * (N + 1) + i) * ns + s0 + r] = v;
+      }
+    }
+    buf ^= 1;
+  }
+  // row N: the tile's share of log psi (sum of log_cosh(theta), plus the
+  // visible term in tile 0); tile 0 also writes the diagonal element
+  red[buf][w][0][lane] = lcs;
+  __syncthreads();
+  if (tid < 64 && s0 + tid < ns) {
+    const int r = tid;
+    if (want_lp) {
+      float v = ((red[buf][0][0][r] + red[buf][1][0][r]) + red[buf][2][0][r]) + red[buf][3][0][r];
+      if (jt == 0 && a) {
+        float vis = 0.0f;
+        for (int i = 0; i < N; ++i) vis += ((bits[(i >> 5) * 64 + r] >> (i & 31)) & 1u) ? a[i] : -a[i];
+        v += vis;
+      }
+      part[((int64_t)jt * (N + 1) + N) * ns + s0 + r] = v;
+    }
+    if (jt == 0) {
+      int zz = 0;
+      for (int e = 0; e < is.n_edges; ++e) {
+        const int i0 = is.edges[2 * e], i1 = is.edges[2 * e + 1];
+        const uint32_t b0 = (bits[(i0 >> 5) * 64 + r] >> (i0 & 31)) & 1u;
+        const uint32_t b1 = (bits[(i1 >> 5) * 64 + r] >> (i1 & 31)) & 1u;
+        zz += b0 == b1 ? 1 : -1;
+      }
+      diag[s0 + r] = is.mel_dtype == NKA_F64 ? is.J * (double)zz : (double)((float)is.J * (float)zz);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) ising_flip_finish_kernel(const float* __restrict__ part,
+                                                                const double* __restrict__ diag,
+                                                                int64_t ns, int N, int n_jt, double mh,
+                                                                double* __restrict__ eloc,
+                                                                float* __restrict__ log_psi) {
+  const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (s >= ns) return;
+  double E = diag[s];
+  for (int i = 0; i < N; ++i) {
+    double d = 0.0;
+    for (int jt = 0; jt < n_jt; ++jt) d += (double)part[((int64_t)jt * (N + 1) + i) * ns + s];
+    E += mh * exp(d);
+  }
+  eloc[s] = E;
+  if (log_psi) {
+    double l = 0.0;
+    for (int jt = 0; jt < n_jt; ++jt) l += (double)part[((int64_t)jt * (N + 1) + N) * ns + s];
+    log_psi[s] = (float)l;
+  }
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// The tiled path serves FP32 Ising operators with a transverse field on models
+// whose W does not fit in LDS.
+static bool ising_tiled(const nka_rbm& m, int op_kind, const nka_ising* is) {
+  return op_kind == NKA_OP_ISING && is && is->h != 0.0 && m.dtype == NKA_F32 &&
+         !stage_fits<float>(m, 4) && (size_t)(m.n_visible + 31) / 32 * 64 * 4 <= 32 * 1024;
+}
+
+struct IsingTiles {
+  int Mp, n_jt;
+  size_t w_bytes, per_sample;
+};
+static IsingTiles ising_tiles(const nka_rbm& m) {
+  IsingTiles t;
+  t.Mp = (m.n_hidden + 63) / 64 * 64;
+  t.n_jt = (t.Mp + IF_JT - 1) / IF_JT;
+  t.w_bytes = align256((size_t)m.n_visible * t.Mp * sizeof(float));
+  t.per_sample = ((size_t)t.Mp + (size_t)t.n_jt * (m.n_visible + 1)) * sizeof(float) + sizeof(double);
+  return t;
+}
+
+static int ising_tiled_eloc(const nka_rbm& m, const nka_ising& is, const int8_t* sigma, int64_t n,
+                            double* eloc, float* log_psi, unsigned char* ws, int64_t ws_bytes,
+                            hipStream_t s) {
+  const IsingTiles tl = ising_tiles(m);
+  const int N = m.n_visible, M = m.n_hidden;
+  const int64_t avail = ws_bytes - (int64_t)tl.w_bytes - 3 * 256;
+  int64_t chunk = avail > 0 ? avail / (int64_t)tl.per_sample : 0;
+  if (chunk < n) chunk = chunk / 64 * 64;
+  if (chunk > n) chunk = n;
+  if (chunk < 1) {
+    set_error("nka_rbm_local_energy: workspace too small (%lld bytes, need %lld)", (long long)ws_bytes,
+              (long long)(tl.w_bytes + 3 * 256 + 64 * tl.per_sample));
+    return NKA_ERR_ARG;
+  }
+  const RbmView<float> v = rbm_view<float>(m);
+  float* Wp = (float*)ws;
+  float* thT = (float*)(ws + tl.w_bytes);
+  float* part = (float*)(ws + tl.w_bytes + align256((size_t)tl.Mp * chunk * sizeof(float)));
+  double* diag = (double*)((unsigned char*)part +
+                           align256((size_t)tl.n_jt * (N + 1) * chunk * sizeof(float)));
+  const int64_t nwp = (int64_t)N * tl.Mp;
+  hipLaunchKernelGGL(ising_scale_w_kernel, dim3((unsigned)((nwp + 255) / 256)), dim3(256), 0, s, v.W,
+                     N, M, tl.Mp, Wp);
+  const double mh = is.mel_dtype == NKA_F64 ? -is.h : (double)(-(float)is.h);
+  const size_t bits_lds = (size_t)(N + 31) / 32 * 64 * sizeof(uint32_t);
+  for (int64_t c0 = 0; c0 < n; c0 += chunk) {
+    const int64_t nc = min(chunk, n - c0);
+    dim3 gg((unsigned)((nc + PG_T - 1) / PG_T), (unsigned)(tl.Mp / PG_T));
+    hipLaunchKernelGGL((rbm_prepare_gemm_kernel<float, true>), gg, dim3(256), 0, s, m, sigma + c0 * N,
+                       nc, (float*)nullptr, thT);
+    dim3 gf((unsigned)((nc + 63) / 64), (unsigned)tl.n_jt);
+    hipLaunchKernelGGL(ising_flip_tile_kernel, gf, dim3(256), bits_lds, s, thT, Wp, sigma + c0 * N, nc, N,
+                       tl.Mp, v.a, is, log_psi ? 1 : 0, part, diag);
+    hipLaunchKernelGGL(ising_flip_finish_kernel, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s,
+                       part, diag, nc, N, tl.n_jt, mh, eloc + c0, log_psi ? log_psi + c0 : nullptr);
+  }
+  return NKA_OK;
 }
 
 }  // namespace nka
@@ -701,8 +917,16 @@ int nka_rbm_log_psi(const nka_rbm* m, const int8_t* sigma, int64_t n, void* out,
   return NKA_OK;
 }
 
+int64_t nka_rbm_local_energy_workspace(const nka_rbm* m, int op_kind, const void* op, int64_t n) {
+  if (!m || !op || n <= 0 || check_rbm(m, "nka_rbm_local_energy_workspace") != NKA_OK) return 0;
+  if (!ising_tiled(*m, op_kind, op_kind == NKA_OP_ISING ? (const nka_ising*)op : nullptr)) return 0;
+  const IsingTiles tl = ising_tiles(*m);
+  return (int64_t)(tl.w_bytes + 3 * 256 + (size_t)n * tl.per_sample);
+}
+
 int nka_rbm_local_energy(const nka_rbm* m, int op_kind, const void* op, const int8_t* sigma,
-                         int64_t n, double* eloc, void* log_psi, void* stream) {
+                         int64_t n, double* eloc, void* log_psi, void* workspace,
+                         int64_t workspace_bytes, void* stream) {
   int st = check_rbm(m, "nka_rbm_local_energy");
   if (st) return st;
   NKA_CHECK_ARG(op != nullptr, "nka_rbm_local_energy: null operator");
@@ -721,8 +945,16 @@ int nka_rbm_local_energy(const nka_rbm* m, int op_kind, const void* op, const in
     return NKA_ERR_ARG;
   }
   if (n == 0) return NKA_OK;
-  const int jl = pick_jl(m->n_hidden);
   hipStream_t s = (hipStream_t)stream;
+  if (ising_tiled(*m, op_kind, &a.ising)) {
+    NKA_CHECK_ARG(workspace != nullptr, "nka_rbm_local_energy: this model and operator need a workspace");
+    st = ising_tiled_eloc(*m, a.ising, sigma, n, eloc, (float*)log_psi, (unsigned char*)workspace,
+                          workspace_bytes, s);
+    if (st) return st;
+    NKA_CHECK_LAUNCH("nka_rbm_local_energy");
+    return NKA_OK;
+  }
+  const int jl = pick_jl(m->n_hidden);
   if (m->dtype == NKA_F32) {
     NKA_DISPATCH_JL(jl, launch_eloc_f32, *m, a, sigma, n, eloc, log_psi, s);
   } else {

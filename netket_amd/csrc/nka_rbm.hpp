@@ -121,7 +121,7 @@ struct RbmView {
 };
 
 template <class T>
-__device__ inline RbmView<T> rbm_view(const nka_rbm& m) {
+__host__ __device__ inline RbmView<T> rbm_view(const nka_rbm& m) {
   const T* P = (const T*)m.params;
   RbmView<T> v;
   v.W = P + m.kernel_off;

@@ -149,3 +149,47 @@ def test_local_energy(gpu, kind, N, alpha, dtype):
         np.testing.assert_allclose(e, ref, rtol=1e-10, atol=1e-10)
     else:
         np.testing.assert_allclose(e, ref, rtol=1e-5, atol=1e-5 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("lattice,alpha,scale,bias,n", [
+    ("chain100", 4, 0.1, True, 300),       # M = 400: a partly empty last unit tile
+    ("square10", 3, 0.3, True, 200),       # M = 300, large |theta|, 2D edges
+    ("chain90", 3, 0.2, False, 131),       # no biases, N not a multiple of 8, ragged sample tile
+])
+def test_local_energy_ising_tiled(gpu, lattice, alpha, scale, bias, n):
+    # FP32 models whose W exceeds 64 KiB take the path tiled over samples and
+    # hidden units (theta GEMM + flip kernel over all sites + tile reduction)
+    print(f"case eloc ising tiled {lattice} alpha={alpha} bias={bias} n={n}", flush=True)
+    import ctypes as C
+    from netket_amd import _engine as eng, _native as nat, _ops
+    g = nk.graph.Square(10) if lattice == "square10" else nk.graph.Chain(int(lattice[5:]))
+    N = g.n_nodes
+    op = nk.operator.Ising(nk.hilbert.Spin(0.5, N=N), g, h=0.8, J=0.9, dtype=np.float32)
+    model = nk.models.RBM(alpha=alpha, param_dtype=np.float32, use_hidden_bias=bias,
+                          use_visible_bias=bias)
+    lay = model.layout(N)
+    flat_np = np.random.default_rng(30 + N).normal(scale=scale, size=lay["n_params"]).astype(np.float32)
+    flat = torch.as_tensor(flat_np, device="cuda")
+    p = orbm.RBMParams(flat_np.astype(np.float64), N, lay["M"], hidden_bias=bias, visible_bias=bias)
+    m = model.native(flat, N)
+    kind, op_ptr, keep = _ops.op_struct(op)
+    assert nat.lib().nka_rbm_local_energy_workspace(C.byref(m), kind, op_ptr, n) > 0
+    x = _states(N, n, 31)
+    ref = orbm.local_energy(p, x, lambda xx: O.ising_conn(xx, op.edges, 0.8, 0.9, np.float32))
+    e, lp = eng.local_energies(model, flat, op, torch.as_tensor(x, device="cuda"), log_psi=True)
+    e = e.cpu().numpy()
+    np.testing.assert_allclose(e, ref, rtol=1e-5, atol=1e-5 * np.abs(ref).max())
+    np.testing.assert_allclose(lp.cpu().numpy(), orbm.log_psi(p, x), rtol=1e-5, atol=1e-5)
+    # a workspace for 64 samples processes the batch in chunks, same values
+    per64 = nat.lib().nka_rbm_local_energy_workspace(C.byref(m), kind, op_ptr, 64)
+    ws = torch.empty(int(per64), dtype=torch.uint8, device="cuda")
+    xt = torch.as_tensor(x, device="cuda")
+    e2 = torch.empty(n, dtype=torch.float64, device="cuda")
+    nat.call("nka_rbm_local_energy", C.byref(m), kind, op_ptr, _ops.ptr(xt), n, _ops.ptr(e2), None,
+             _ops.ptr(ws), ws.numel(), _ops.stream())
+    np.testing.assert_array_equal(e2.cpu().numpy(), e)
+    # without a workspace the call is refused
+    with pytest.raises(nat.NativeError, match="workspace"):
+        nat.call("nka_rbm_local_energy", C.byref(m), kind, op_ptr, _ops.ptr(xt), n, _ops.ptr(e2), None,
+                 None, 0, _ops.stream())
+    del keep
