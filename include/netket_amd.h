@@ -302,7 +302,9 @@ int nka_gram_bf16x3(const uint16_t* x_hi, const uint16_t* x_lo, int64_t n, int64
  * + ha sigma_s.sigma_t (hb, ha = 1 with a hidden / visible bias), two FP64
  * products with inner dimension n_hidden and n_visible instead of n_params.
  * With scale = 1/ns this is T = O_L O_L^T of netket/_src/ngd/srt.py:55-66.
- * workspace holds nka_rbm_ntk_workspace(...) bytes. */
+ * FP64 models use FP64 matrix cores; FP32 models form t.t on bf16 matrix
+ * cores from t = hi + lo (hi hi^T + hi lo^T + lo hi^T, FP32 accumulation) and
+ * sigma.sigma exactly.  workspace holds nka_rbm_ntk_workspace(...) bytes. */
 int64_t nka_rbm_ntk_workspace(const nka_rbm* m, int64_t ns);
 int nka_rbm_ntk(const nka_rbm* m, const int8_t* sigma, const void* t, int64_t ns,
                 const double* mean, double scale, double shift, double offset, double* out,

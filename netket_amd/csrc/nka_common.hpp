@@ -159,6 +159,15 @@ template <> __device__ inline double uniform_value<double>(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// FP32 -> bf16, round to nearest even (NaN stays NaN); and back.
+__device__ inline uint16_t bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ inline float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
 // Lower-triangular tile pair (ti >= tj) of linear id b in an order made of
 // SxS super-blocks of tiles (row-major over the super-blocks of the lower
 // triangle, then row-major inside each; diagonal super-blocks are

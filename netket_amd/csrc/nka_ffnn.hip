@@ -503,13 +503,6 @@ __device__ inline double ffnn_o(const nka_ffnn& m, const int8_t* x, const T* h1,
   return (double)h1[s * H1 + a] * (double)g2[s * H2 + k];
 }
 
-__device__ inline uint16_t bf16_rne(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-
 template <class T>
 __global__ void ffnn_jacobian_kernel(nka_ffnn m, const int8_t* __restrict__ x, const T* __restrict__ h1,
                                      const T* __restrict__ d1, const T* __restrict__ g2, int64_t ns, int64_t np,

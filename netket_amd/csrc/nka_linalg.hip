@@ -53,11 +53,26 @@ constexpr int GB_LD = 72;   // LDS row stride in bf16 (64 + 8 pad: 144 bytes)
 constexpr int GB_CPR = GB_K / 8;              // 16-byte chunks per row and plane
 constexpr int GB_NIT = GB_T * GB_CPR / 256;   // chunks per thread and plane
 
+// Epilogue of the RBM neural tangent kernel, T = O_c O_c^T scaled, from
+// Theta = t t^T (the bf16x3 product) and S = sigma sigma^T:
+//   T_ij = scale ((hb + S_ij) Theta_ij + ha S_ij - v_i - v_j + c) + offset
+// (+ shift on the diagonal), with S_ij = N - 2 popcount(bits_i ^ bits_j)
+// exact from the packed spins.
+struct GramRbmEpi {
+  const uint32_t* bits;   // (n, nw): bit b of word w is (sigma_{32w+b} > 0)
+  int nw, nwp, N;         // words per row, LDS row stride (multiple of 4), sites
+  const double* v;
+  const double* c;
+  double hb, ha, scale;
+};
+
+template <bool RBM>
 __global__ void __launch_bounds__(256) gram_bf16x3_kernel(const uint16_t* __restrict__ xh,
                                                           const uint16_t* __restrict__ xl, int64_t n,
                                                           int64_t K, int64_t ldx, double shift,
                                                           double offset, double* __restrict__ out,
-                                                          int64_t ldo, int64_t n_tiles_total) {
+                                                          int64_t ldo, int64_t n_tiles_total,
+                                                          GramRbmEpi ep) {
   __shared__ __attribute__((aligned(16))) uint16_t sA[2][GB_T * GB_LD];  // [plane][row*LD + k]
   __shared__ __attribute__((aligned(16))) uint16_t sB[2][GB_T * GB_LD];
   const int64_t bid = xcd_remap(blockIdx.x, gridDim.x);
@@ -157,21 +172,84 @@ __global__ void __launch_bounds__(256) gram_bf16x3_kernel(const uint16_t* __rest
     }
   }
   // store: element e of lane l in a 32x32 tile is (row (e/4)*8 + (l/32)*4 + e%4, col l%32)
+  if constexpr (!RBM) {
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+      for (int b = 0; b < 2; ++b)
 #pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        const int64_t gi = i0 + wm + a * 32 + (e / 4) * 8 + (lane >> 5) * 4 + (e & 3);
-        const int64_t gj = j0 + wn + b * 32 + (lane & 31);
-        if (gi < n && gj <= gi) {
-          double v = (double)acc[a][b][e] + offset;
-          if (gi == gj) v += shift;
-          out[gi * ldo + gj] = v;
+        for (int e = 0; e < 16; ++e) {
+          const int64_t gi = i0 + wm + a * 32 + (e / 4) * 8 + (lane >> 5) * 4 + (e & 3);
+          const int64_t gj = j0 + wn + b * 32 + (lane & 31);
+          if (gi < n && gj <= gi) {
+            double v = (double)acc[a][b][e] + offset;
+            if (gi == gj) v += shift;
+            out[gi * ldo + gj] = v;
+          }
         }
+  } else {
+    // spins of the tile's 128 rows and 128 columns into the operand LDS
+    __syncthreads();
+    uint32_t* rb = (uint32_t*)&sA[0][0];
+    uint32_t* cb = (uint32_t*)&sB[0][0];
+    const int nwp = ep.nwp;
+    for (int idx = tid; idx < GB_T * nwp; idx += 256) {
+      const int r = idx / nwp, w = idx % nwp;
+      uint32_t x = 0, y = 0;
+      if (w < ep.nw) {
+        if (i0 + r < n) x = ep.bits[(i0 + r) * ep.nw + w];
+        if (j0 + r < n) y = ep.bits[(j0 + r) * ep.nw + w];
       }
+      rb[idx] = x;
+      cb[idx] = y;
+    }
+    __syncthreads();
+    int cnt[2][2][16];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) cnt[a][b][e] = 0;
+    for (int w = 0; w < nwp; w += 4) {
+      uint4 cq[2];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) cq[b] = *(const uint4*)&cb[(wn + b * 32 + (lane & 31)) * nwp + w];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+          const int rl = wm + a * 32 + (e / 4) * 8 + (lane >> 5) * 4 + (e & 3);
+          const uint4 rq = *(const uint4*)&rb[rl * nwp + w];
+#pragma unroll
+          for (int b = 0; b < 2; ++b)
+            cnt[a][b][e] += __builtin_popcount(rq.x ^ cq[b].x) + __builtin_popcount(rq.y ^ cq[b].y) +
+                            __builtin_popcount(rq.z ^ cq[b].z) + __builtin_popcount(rq.w ^ cq[b].w);
+        }
+    }
+    const double cc = *ep.c;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+          const int64_t gi = i0 + wm + a * 32 + (e / 4) * 8 + (lane >> 5) * 4 + (e & 3);
+          const int64_t gj = j0 + wn + b * 32 + (lane & 31);
+          if (gi < n && gj <= gi) {
+            const double S = (double)(ep.N - 2 * cnt[a][b][e]);
+            const double Th = (double)acc[a][b][e];
+            double v = ep.scale * ((ep.hb + S) * Th + ep.ha * S - ep.v[gi] - ep.v[gj] + cc) + offset;
+            if (gi == gj) v += shift;
+            out[gi * ldo + gj] = v;
+          }
+        }
+  }
 }
+
+// Largest LDS row stride (words) of the packed spins in the RBM epilogue: the
+// 128 rows must fit in one operand buffer.
+constexpr int kGramRbmMaxNwp = (int)(2 * GB_T * GB_LD * sizeof(uint16_t) / (GB_T * sizeof(uint32_t))) / 4 * 4;
 
 // ---------------------------------------------------------------------------
 // FP64 NT GEMM: C = beta C + alpha A B^T (+ shift on the diagonal),
@@ -1143,6 +1221,23 @@ __global__ void symmetrize_kernel(double* __restrict__ a, int64_t n, int64_t lda
   }
 }
 
+// RBM neural tangent kernel on the bf16x3 Gram tiles (called from the NTK
+// entry point): Theta from the split tanh values xh + xl (n x K, ldx), S from
+// the packed spins.  Returns false when the spins do not fit the epilogue.
+bool launch_gram_rbm(const uint16_t* xh, const uint16_t* xl, int64_t n, int64_t K, int64_t ldx,
+                     const uint32_t* bits, int nw, int N, const double* v, const double* c, double hb,
+                     double ha, double scale, double shift, double offset, double* out, int64_t ldo,
+                     hipStream_t s) {
+  const int nwp = (nw + 3) / 4 * 4;
+  if (nwp > kGramRbmMaxNwp) return false;
+  const int64_t T = (n + GB_T - 1) / GB_T;
+  const int64_t tiles = T * (T + 1) / 2;
+  GramRbmEpi ep{bits, nw, nwp, N, v, c, hb, ha, scale};
+  hipLaunchKernelGGL((gram_bf16x3_kernel<true>), dim3(grid_mult8(tiles)), dim3(256), 0, s, xh, xl, n, K, ldx,
+                     shift, offset, out, ldo, tiles, ep);
+  return true;
+}
+
 }  // namespace nka
 
 using namespace nka;
@@ -1160,8 +1255,8 @@ int nka_gram_bf16x3(const uint16_t* x_hi, const uint16_t* x_lo, int64_t n, int64
   const int64_t T = (n + GB_T - 1) / GB_T;
   const int64_t tiles = T * (T + 1) / 2;
   NKA_CHECK_ARG(tiles < (1LL << 31), "nka_gram_bf16x3: matrix too large");
-  hipLaunchKernelGGL(gram_bf16x3_kernel, dim3(grid_mult8(tiles)), dim3(256), 0, (hipStream_t)stream,
-                     x_hi, x_lo, n, k, ldx, shift, offset, out, ldo, tiles);
+  hipLaunchKernelGGL((gram_bf16x3_kernel<false>), dim3(grid_mult8(tiles)), dim3(256), 0, (hipStream_t)stream,
+                     x_hi, x_lo, n, k, ldx, shift, offset, out, ldo, tiles, GramRbmEpi{});
   NKA_CHECK_LAUNCH("nka_gram_bf16x3");
   return NKA_OK;
 }

@@ -7,8 +7,11 @@
 // (hb, ha = 1 when the hidden / visible bias exists), and centring with the
 // column means mu gives
 //   T_st = scale * [(O O^T)_st - v_s - v_t + c],  v = O mu,  c = |mu|^2,
-// scale = 1/Ns.  Both inner products are formed on FP64 matrix cores with
-// K = M and K = N, instead of the K = Np of the dense Jacobian product.
+// scale = 1/Ns.  The inner products have K = M and K = N instead of the
+// K = Np of the dense Jacobian product.  FP64 models form both on FP64
+// matrix cores; FP32 models form t.t on bf16 matrix cores from the split
+// t = hi + lo (three products, FP32 accumulation) and sigma.sigma exactly from
+// packed spins in the epilogue (the bf16x3 Gram kernel of nka_linalg.hip).
 #include "nka_common.hpp"
 #include "netket_amd.h"
 
@@ -54,6 +57,45 @@ __global__ void __launch_bounds__(256) ntk_vec_kernel(const double* __restrict__
     if (threadIdx.x == 0) c[0] = red[0] + red[1] + red[2] + red[3];
   }
 }
+
+// FP32 models: t -> bf16 planes hi + lo (ns x ldt, zero beyond M) for the
+// bf16x3 product t t^T, and the spins packed 32 per word (ns x nw).
+__global__ void ntk_split_kernel(const float* __restrict__ t, int64_t ns, int M, int ldt,
+                                 uint16_t* __restrict__ hi, uint16_t* __restrict__ lo) {
+  const int64_t total = ns * ldt;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = e / ldt;
+    const int k = (int)(e % ldt);
+    uint16_t h = 0, l = 0;
+    if (k < M) {
+      const float f = t[s * M + k];
+      h = bf16_rne(f);
+      l = bf16_rne(f - bf16_to_f(h));
+    }
+    hi[e] = h;
+    lo[e] = l;
+  }
+}
+
+__global__ void ntk_pack_spins_kernel(const int8_t* __restrict__ sigma, int64_t ns, int N, int nw,
+                                      uint32_t* __restrict__ bits) {
+  const int64_t total = ns * nw;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = e / nw;
+    const int w = (int)(e % nw);
+    const int lim = min(32, N - 32 * w);
+    uint32_t word = 0;
+    for (int b = 0; b < lim; ++b) word |= (sigma[s * N + 32 * w + b] > 0 ? 1u : 0u) << b;
+    bits[e] = word;
+  }
+}
+
+bool launch_gram_rbm(const uint16_t* xh, const uint16_t* xl, int64_t n, int64_t K, int64_t ldx,
+                     const uint32_t* bits, int nw, int N, const double* v, const double* c, double hb,
+                     double ha, double scale, double shift, double offset, double* out, int64_t ldo,
+                     hipStream_t s);
 
 constexpr int NT_T = 64;
 constexpr int NT_K = 16;
@@ -154,9 +196,21 @@ using namespace nka;
 
 extern "C" {
 
+static int64_t ntk_base_bytes(const nka_rbm& m, int64_t ns) {
+  return (ns * (int64_t)m.n_hidden + ns * (int64_t)m.n_visible + ns + 8) * (int64_t)sizeof(double);
+}
+static int ntk_ldt(const nka_rbm& m) { return (m.n_hidden + 7) / 8 * 8; }
+static int ntk_nw(const nka_rbm& m) { return (m.n_visible + 31) / 32; }
+
 int64_t nka_rbm_ntk_workspace(const nka_rbm* m, int64_t ns) {
   if (!m || ns <= 0) return 0;
-  return (ns * (int64_t)m->n_hidden + ns * (int64_t)m->n_visible + ns + 8) * (int64_t)sizeof(double);
+  int64_t b = ntk_base_bytes(*m, ns);
+  if (m->dtype == NKA_F32) {
+    b = (b + 255) / 256 * 256;
+    b += (2 * ns * ntk_ldt(*m) * (int64_t)sizeof(uint16_t) + 255) / 256 * 256;
+    b += ns * ntk_nw(*m) * (int64_t)sizeof(uint32_t);
+  }
+  return b;
 }
 
 int nka_rbm_ntk(const nka_rbm* m, const int8_t* sigma, const void* t, int64_t ns,
@@ -184,13 +238,32 @@ int nka_rbm_ntk(const nka_rbm* m, const int8_t* sigma, const void* t, int64_t ns
   const unsigned g2 = (unsigned)std::min<int64_t>((ns + 3) / 4, 4096);
   hipLaunchKernelGGL(ntk_vec_kernel, dim3(g2), dim3(256), 0, s, t64, sigma, U, ns, N, M, mean,
                      m->bias_off, m->visible_off, np, v, c);
+  const double hb = m->bias_off >= 0 ? 1.0 : 0.0, ha = m->visible_off >= 0 ? 1.0 : 0.0;
+  if (m->dtype == NKA_F32) {
+    // Theta = t t^T on bf16 matrix cores (hi hi^T + hi lo^T + lo hi^T, FP32
+    // accumulation); S and the centring are exact in the epilogue
+    unsigned char* p = (unsigned char*)workspace + (ntk_base_bytes(*m, ns) + 255) / 256 * 256;
+    const int ldt = ntk_ldt(*m), nw = ntk_nw(*m);
+    uint16_t* hi = (uint16_t*)p;
+    uint16_t* lo = hi + ns * ldt;
+    uint32_t* bits = (uint32_t*)(p + (2 * ns * ldt * (int64_t)sizeof(uint16_t) + 255) / 256 * 256);
+    const unsigned gs = (unsigned)std::min<int64_t>((ns * ldt + 255) / 256, 65536);
+    hipLaunchKernelGGL(ntk_split_kernel, dim3(gs), dim3(256), 0, s, (const float*)t, ns, M, ldt, hi, lo);
+    const unsigned gb = (unsigned)std::min<int64_t>((ns * nw + 255) / 256, 65536);
+    hipLaunchKernelGGL(ntk_pack_spins_kernel, dim3(gb), dim3(256), 0, s, sigma, ns, N, nw, bits);
+    const int64_t T2 = (ns + 127) / 128;
+    NKA_CHECK_ARG(T2 * (T2 + 1) / 2 < (1LL << 31), "nka_rbm_ntk: too large");
+    if (launch_gram_rbm(hi, lo, ns, ldt, ldt, bits, nw, N, v, c, hb, ha, scale, shift, offset, out, ldo, s)) {
+      NKA_CHECK_LAUNCH("nka_rbm_ntk");
+      return NKA_OK;
+    }
+  }
   const int64_t T = (ns + NT_T - 1) / NT_T;
   const int64_t tiles = T * (T + 1) / 2;
   NKA_CHECK_ARG(tiles < (1LL << 31), "nka_rbm_ntk: too large");
   const unsigned grid = (unsigned)(((tiles + 7) / 8) * 8);
-  hipLaunchKernelGGL(rbm_ntk_kernel, dim3(grid), dim3(256), 0, s, t64, M, sigma, N, ns, v, c,
-                     m->bias_off >= 0 ? 1.0 : 0.0, m->visible_off >= 0 ? 1.0 : 0.0, scale, shift,
-                     offset, out, ldo, tiles);
+  hipLaunchKernelGGL(rbm_ntk_kernel, dim3(grid), dim3(256), 0, s, t64, M, sigma, N, ns, v, c, hb, ha,
+                     scale, shift, offset, out, ldo, tiles);
   NKA_CHECK_LAUNCH("nka_rbm_ntk");
   return NKA_OK;
 }

@@ -323,14 +323,6 @@ __global__ void __launch_bounds__(256) contract_reduce(const double* __restrict_
 // ---------------------------------------------------------------------------
 // Centred Jacobian writer.
 // ---------------------------------------------------------------------------
-__device__ inline uint16_t bf16_rne(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-__device__ inline float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-
 __device__ inline void store_out(int kind, void* a, void* b, int64_t idx, double val) {
   if (kind == NKA_OUT_F64) {
     ((double*)a)[idx] = val;

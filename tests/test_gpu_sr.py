@@ -97,17 +97,19 @@ def test_qgt_dense(gpu, dtype):
         assert np.all(err <= bound)
 
 
-@pytest.mark.parametrize("dtype,ns,biases", [(np.float64, 300, (True, True)),
-                                             (np.float64, 64, (False, True)),
-                                             (np.float64, 129, (True, False)),
-                                             (np.float32, 300, (True, True)),
-                                             (np.float64, 1, (True, True))])
-def test_rbm_ntk_structured(gpu, dtype, ns, biases):
+@pytest.mark.parametrize("dtype,ns,biases,N", [(np.float64, 300, (True, True), 14),
+                                               (np.float64, 64, (False, True), 14),
+                                               (np.float64, 129, (True, False), 14),
+                                               (np.float32, 300, (True, True), 14),
+                                               (np.float32, 129, (False, True), 70),
+                                               (np.float32, 260, (True, False), 97),
+                                               (np.float64, 1, (True, True), 14)])
+def test_rbm_ntk_structured(gpu, dtype, ns, biases, N):
     # T = (O - mean)(O - mean)^T / ns + shift I + offset 11^T from the RBM
     # structure equals the explicit product of the oracle's FP64 Jacobian
     # (netket/_src/ngd/srt.py:55-66 with proj_reg giving the offset)
-    print(f"case rbm_ntk {np.dtype(dtype)} ns={ns} biases={biases}", flush=True)
-    N, alpha = 14, 3
+    print(f"case rbm_ntk {np.dtype(dtype)} ns={ns} biases={biases} N={N}", flush=True)
+    alpha = 3
     M = alpha * N
     model = nk.models.RBM(alpha=alpha, param_dtype=dtype, use_hidden_bias=biases[0],
                           use_visible_bias=biases[1])
@@ -133,7 +135,11 @@ def test_rbm_ntk_structured(gpu, dtype, ns, biases):
     dc = np.sqrt(np.sum(Oc * Oc, axis=1) / ns)
     du = np.sqrt(np.sum(Ou * Ou, axis=1) / ns)
     tol = 1e-12 if dtype == np.float64 else 2e-6
-    bound = (tol * np.outer(dc, dc) + 1e-14 * np.outer(du, du))[lo] + 1e-15
+    # FP32 models: t.t on bf16x3 keeps ~16 mantissa bits of t per product, so
+    # |dTheta_st| <= c 2^-16 |t_s||t_t| and |dT_st| <= scale (hb + N) |dTheta_st|
+    # <= c 2^-16 du_s du_t; sigma.sigma and the centring are exact / FP64
+    tol_u = 1e-14 if dtype == np.float64 else 3e-5
+    bound = (tol * np.outer(dc, dc) + tol_u * np.outer(du, du))[lo] + 1e-15
     err = np.abs(T[lo] - ref[lo])
     print(f"max |dT| / bound = {(err / bound).max():.2e}", flush=True)
     assert np.all(err <= bound)
