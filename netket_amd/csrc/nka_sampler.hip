@@ -231,6 +231,141 @@ __global__ void __launch_bounds__(256) rbm_metropolis_kernel(nka_rbm m, SweepArg
 }
 
 // ---------------------------------------------------------------------------
+// Local-rule sweeps of FP32 models whose W does not fit in LDS.  One
+// wavefront per chain, lane l owning the units j = l + 64 jj, as in
+// rbm_metropolis_kernel, with two changes for a W that streams from the
+// caches.  The site of a local-rule proposal does not depend on the state, so
+// the W row of the next proposal is loaded one proposal ahead and its latency
+// overlaps the current proposal.  And the change of log psi uses
+//   sum_j [lc(t_j) - lc(theta_j)] = (1/kappa) sum_j (|t'_j| - |theta'_j|)
+//                                  + ln2 log2( prod_j (1 + 2^-|t'_j|) / P0 )
+// (lc(t) = |t| + log(1 + e^{-2|t|}) - log 2, t' = kappa t, kappa = 2/ln 2):
+// one exp2 per unit instead of an exp and a log.  Each lane keeps theta' of
+// its units and 1/P0 of its product (< 2^64 over at most 64 units); an
+// accepted move replaces both by the proposal's.  Same random streams and
+// acceptance rule as rbm_metropolis_kernel.
+// ---------------------------------------------------------------------------
+constexpr float kKappaS = 2.8853900817779268f;    // 2 / ln 2
+constexpr float kLn2S = 0.69314718055994531f;
+
+template <int JL>
+__global__ void __launch_bounds__(256) rbm_metropolis_wide_kernel(nka_rbm m, SweepArgs sa, int8_t* __restrict__ sigma,
+                                                                  float* __restrict__ log_prob,
+                                                                  int8_t* __restrict__ samples,
+                                                                  float* __restrict__ lp_samples,
+                                                                  int64_t* __restrict__ n_accepted) {
+  extern __shared__ unsigned char smem[];
+  const RbmView<float> v = rbm_view<float>(m);
+  const int N = v.N, M = v.M;
+  const int wid = threadIdx.x / NKA_WAVE, waves = blockDim.x / NKA_WAVE;
+  const int lane = lane_id();
+  int8_t* xs = (int8_t*)smem + wid * pad16(N);
+  const int c = blockIdx.x * waves + wid;
+  if (c >= sa.C) return;
+  for (int i = lane; i < N; i += NKA_WAVE) xs[i] = sigma[(int64_t)c * N + i];
+  wave_lds_sync();
+
+  // theta of this lane's units: all JL loads of a row in flight per site
+  float th[JL];
+#pragma unroll
+  for (int jj = 0; jj < JL; ++jj) {
+    const int j = lane + NKA_WAVE * jj;
+    th[jj] = (j < M && v.b) ? v.b[j] : 0.0f;
+  }
+  for (int i = 0; i < N; ++i) {
+    const bool up = xs[i] > 0;
+    const float* Wr = v.W + (size_t)i * M;
+#pragma unroll
+    for (int jj = 0; jj < JL; ++jj) {
+      const int j = lane + NKA_WAVE * jj;
+      const float w = j < M ? Wr[j] : 0.0f;
+      th[jj] += up ? w : -w;
+    }
+  }
+  const float pw = (float)sa.machine_pow;
+  float lcs = 0.0f, P0 = 1.0f;
+#pragma unroll
+  for (int jj = 0; jj < JL; ++jj) {
+    lcs += lane + NKA_WAVE * jj < M ? log_cosh<float>(th[jj]) : 0.0f;
+    th[jj] *= kKappaS;
+    P0 = fmaf(P0, __builtin_amdgcn_exp2f(-fabsf(th[jj])), P0);
+  }
+  float rP0 = 1.0f / P0;
+  float lp = pw * (wave_sum(lcs) + visible_term<float>(xs, v.a, N));
+
+  int64_t nacc = 0;
+  uint64_t q = sa.counter;
+  if (sa.counter_dev) {
+    const uint64_t cd = *sa.counter_dev;
+    q = ((uint64_t)uniform_value<uint32_t>((uint32_t)(cd >> 32)) << 32) |
+        (uint64_t)uniform_value<uint32_t>((uint32_t)cd);
+  }
+  u32x4 rnd_next = philox4x32_10({(uint32_t)q, (uint32_t)(q >> 32), (uint32_t)c, TAG_METROPOLIS},
+                                 sa.k0, sa.k1);
+  int site_next = (int)bounded(uniform_value<uint32_t>(rnd_next.x), (uint32_t)N);
+  float wn[JL];
+#pragma unroll
+  for (int jj = 0; jj < JL; ++jj) {
+    const int j = lane + NKA_WAVE * jj;
+    wn[jj] = j < M ? v.W[(size_t)site_next * M + j] : 0.0f;
+  }
+  const int n_sweeps = sa.n_discard + sa.chain_length;
+  for (int sw = 0; sw < n_sweeps; ++sw) {
+    for (int p = 0; p < sa.sweep_size; ++p, ++q) {
+      const u32x4 rnd = rnd_next;
+      const int site = site_next;
+      float w[JL];
+#pragma unroll
+      for (int jj = 0; jj < JL; ++jj) w[jj] = wn[jj];
+      {
+        const uint64_t qn = q + 1;
+        rnd_next = philox4x32_10({(uint32_t)qn, (uint32_t)(qn >> 32), (uint32_t)c, TAG_METROPOLIS},
+                                 sa.k0, sa.k1);
+        site_next = (int)bounded(uniform_value<uint32_t>(rnd_next.x), (uint32_t)N);
+#pragma unroll
+        for (int jj = 0; jj < JL; ++jj) {
+          const int j = lane + NKA_WAVE * jj;
+          wn[jj] = j < M ? v.W[(size_t)site_next * M + j] : 0.0f;
+        }
+      }
+      const bool up = xs[site] > 0;
+      const float g = up ? -2.0f * kKappaS : 2.0f * kKappaS;
+      float A = 0.0f, P = 1.0f;
+#pragma unroll
+      for (int jj = 0; jj < JL; ++jj) {
+        const float t = fmaf(g, w[jj], th[jj]);
+        A += fabsf(t) - fabsf(th[jj]);
+        P = fmaf(P, __builtin_amdgcn_exp2f(-fabsf(t)), P);
+      }
+      float delta = wave_sum(A * (1.0f / kKappaS) + kLn2S * __builtin_amdgcn_logf(P * rP0));
+      if (v.a) delta += up ? -2.0f * v.a[site] : 2.0f * v.a[site];
+      delta = uniform_value<float>(delta);
+      const float u = uniform_value<float>(uniform01<float>(rnd));
+      if (u < exp_t<float>(pw * delta)) {
+#pragma unroll
+        for (int jj = 0; jj < JL; ++jj) th[jj] = fmaf(g, w[jj], th[jj]);
+        rP0 = 1.0f / P;
+        if (lane == 0) xs[site] = (int8_t)(-xs[site]);
+        lp += pw * delta;
+        ++nacc;
+        wave_lds_sync();
+      }
+    }
+    if (sw >= sa.n_discard) {
+      const int t = sw - sa.n_discard;
+      int8_t* dst = samples + ((int64_t)c * sa.chain_length + t) * N;
+      for (int i = lane; i < N; i += NKA_WAVE) dst[i] = xs[i];
+      if (lp_samples && lane == 0) lp_samples[(int64_t)c * sa.chain_length + t] = lp;
+    }
+  }
+  for (int i = lane; i < N; i += NKA_WAVE) sigma[(int64_t)c * N + i] = xs[i];
+  if (lane == 0) {
+    log_prob[c] = lp;
+    n_accepted[c] += nacc;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Several chains per wavefront.  For small RBMs (M <= 8 * 64/G hidden units)
 // a proposal's arithmetic is a few hundred instructions that the per-proposal
 // overhead (random draw, cluster bookkeeping, reduction, acceptance) would
@@ -564,6 +699,22 @@ static void launch_sweep(const nka_rbm& m, const SweepArgs& sa, int8_t* sigma, T
   const size_t lds = (stage ? pad16((size_t)m.n_visible * m.n_hidden * sizeof(T)) : 0) +
                      waves * pad16(m.n_visible);
   const unsigned g = (unsigned)((sa.C + waves - 1) / waves);
+  if constexpr (sizeof(T) == 4) {
+    if (!stage && sa.rule == NKA_RULE_LOCAL) {
+      // units per lane rounded up to a multiple of 8 (registers: theta' and
+      // two W rows per unit)
+      const int jlw = (m.n_hidden + 8 * NKA_WAVE - 1) / (8 * NKA_WAVE) * 8;
+#define NKA_WIDE_CASE(J)                                                                              \
+      if (jlw == J) {                                                                                 \
+        hipLaunchKernelGGL((rbm_metropolis_wide_kernel<J>), dim3(g), dim3(64 * waves), lds, s, m, sa, \
+                           sigma, (float*)lp, smp, (float*)lps, nacc);                                \
+        return;                                                                                       \
+      }
+      NKA_WIDE_CASE(8) NKA_WIDE_CASE(16) NKA_WIDE_CASE(24) NKA_WIDE_CASE(32)
+      NKA_WIDE_CASE(40) NKA_WIDE_CASE(48) NKA_WIDE_CASE(56) NKA_WIDE_CASE(64)
+#undef NKA_WIDE_CASE
+    }
+  }
   if (stage)
     hipLaunchKernelGGL((rbm_metropolis_kernel<T, JL, true>), dim3(g), dim3(64 * waves), lds, s, m,
                        sa, sigma, lp, smp, lps, nacc);

@@ -102,6 +102,36 @@ def test_distribution_matches_exact(gpu, rule, dtype):
         assert np.all((s < 0).sum(axis=1) == N // 2)
 
 
+@pytest.mark.parametrize("alpha", [300, 500])
+def test_distribution_matches_exact_wide_model(gpu, alpha):
+    # FP32 models whose W exceeds the LDS budget take the local-rule kernel
+    # that loads the next proposal's W row ahead and uses the exp2 product
+    # form of the log-amplitude change (M = 2400: 40 units per lane,
+    # M = 4000: 64 units per lane)
+    print(f"case distribution wide local alpha={alpha}", flush=True)
+    N = 8
+    model, flat, p = _model(N, alpha, np.float32, 11, 0.05)
+    hi = nk.hilbert.Spin(0.5, N=N)
+    sa = nk.sampler.MetropolisLocal(hi, n_chains=1024)
+    states = O.states_all(N)
+    st = sa.init_state(model, flat, seed=6)
+    samples, st = sa.sample_fused(model, flat, st, 100, 200)
+    s = samples.cpu().numpy().reshape(-1, N)
+    keys = O.state_index(s)
+    order = np.argsort(O.state_index(states))
+    skeys = O.state_index(states)[order]
+    pos = np.searchsorted(skeys, keys)
+    assert np.all(skeys[pos] == keys), "sample outside the Hilbert space"
+    counts = np.bincount(pos, minlength=len(states)).astype(float)
+    probs = orbm.exact_probability(p, states[order])
+    kl = _kl(counts, probs)
+    print(f"KL = {kl:.3e} over {len(states)} states, {s.shape[0]} samples", flush=True)
+    assert kl < 4e-3
+    # the chain's log_prob stays 2 log psi of its state (FP32 running sum)
+    lp_ref = 2 * orbm.log_psi(p, st.sigma.cpu().numpy())
+    np.testing.assert_allclose(st.log_prob.cpu().numpy(), lp_ref, rtol=1e-4, atol=1e-3)
+
+
 def test_mcstate_sample_shapes_and_reset(gpu):
     N = 12
     hi = nk.hilbert.Spin(0.5, N=N)
