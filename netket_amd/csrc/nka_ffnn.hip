@@ -39,6 +39,7 @@ constexpr int FF_GROUPS = FF_THREADS / FF_ROWS;  // threads per row in the first
 constexpr int FF_HPT = 128;      // hidden units per thread: H1 <= FF_GROUPS * FF_HPT
 constexpr int FF_MAXH = FF_GROUPS * FF_HPT;      // 1024
 constexpr int FF_MAXBLK = FF_MAXH / 32;          // column blocks of the second layer
+constexpr int FF_MAXD = 4;                        // differing sites per connected row, fast path
 
 template <class T>
 struct FfnnView {
@@ -74,9 +75,9 @@ constexpr int64_t FF_LDS_MAX = 160 * 1024;
 // rowsum[r] (0 <= r < 32) in a fixed order.  part is LDS scratch of
 // 32 * FF_MAXBLK values.  All threads of the workgroup must call it.
 // ---------------------------------------------------------------------------
-template <class BLoad, class Epi>
-__device__ void rows_gemm(const float* As, int lda, int K, int ncols, BLoad bload, Epi epi,
-                          float* part, float* rowsum) {
+template <class Epi>
+__device__ void rows_gemm(const float* As, int lda, int K, int ncols, const float* __restrict__ Bp, int sk,
+                          int sc, Epi epi, float* part, float* rowsum) {
   const int wid = threadIdx.x >> 6, lane = lane_id();
   const int nblk = (ncols + 31) >> 5;
   const int kh = lane >> 5, lc = lane & 31;
@@ -86,22 +87,55 @@ __device__ void rows_gemm(const float* As, int lda, int K, int ncols, BLoad bloa
     for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
     const int col = cb * 32 + lc;
     const bool cok = col < ncols;
+    // B loads of a lane past ncols use the last column (in bounds; its
+    // output column is discarded), so they need no branch; the loads of the
+    // next 16 k-rows are issued before the current 8 products
+    const int colc = cok ? col : ncols - 1;
+    // B(k, c) = Bp[k sk + c sc], 32-bit element offsets
+    const uint32_t boff = (uint32_t)colc * (uint32_t)sc + (uint32_t)kh * (uint32_t)sk;
+    const uint32_t s2 = 2u * (uint32_t)sk;
+    constexpr int KB = 16;
+    const int kfull = K / KB * KB;
     int k = 0;
-    for (; k + 8 <= K; k += 8) {
-      float a[4], b[4];
+    // two register sets in turn: the loads of the next 16 k-rows are issued
+    // before the current 8 products (the last group reloads its own rows,
+    // unused, so that every load is unconditional)
+    auto group = [&](int kk0, const float* bv) {
+      float a[KB / 2];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int kk = k + 2 * q + kh;
-        a[q] = As[lc * lda + kk];
-        b[q] = cok ? bload(kk, col) : 0.0f;
+      for (int q = 0; q < KB / 2; ++q) a[q] = As[lc * lda + kk0 + 2 * q + kh];
+#pragma unroll
+      for (int q = 0; q < KB / 2; ++q) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q], bv[q], acc, 0, 0, 0);
+    };
+    if (kfull > 0) {
+      float b0[KB / 2], b1[KB / 2];
+#pragma unroll
+      for (int q = 0; q < KB / 2; ++q) b0[q] = Bp[boff + (uint32_t)q * s2];
+      while (true) {
+        {
+          const uint32_t kn = (uint32_t)(k + KB < kfull ? k + KB : k) * (uint32_t)sk + boff;
+#pragma unroll
+          for (int q = 0; q < KB / 2; ++q) b1[q] = Bp[kn + (uint32_t)q * s2];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        group(k, b0);
+        k += KB;
+        if (k >= kfull) break;
+        {
+          const uint32_t kn = (uint32_t)(k + KB < kfull ? k + KB : k) * (uint32_t)sk + boff;
+#pragma unroll
+          for (int q = 0; q < KB / 2; ++q) b0[q] = Bp[kn + (uint32_t)q * s2];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        group(k, b1);
+        k += KB;
+        if (k >= kfull) break;
       }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q], b[q], acc, 0, 0, 0);
     }
     for (; k < K; k += 2) {
       const int kk = k + kh;
       const float a = kk < K ? As[lc * lda + kk] : 0.0f;
-      const float b = (kk < K && cok) ? bload(kk, col) : 0.0f;
+      const float b = (kk < K && cok) ? Bp[(uint32_t)kk * (uint32_t)sk + (uint32_t)col * (uint32_t)sc] : 0.0f;
       acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
     }
     // element e of lane l is (row (e/4)*8 + (l/32)*4 + e%4, col l%32)
@@ -123,14 +157,14 @@ __device__ void rows_gemm(const float* As, int lda, int K, int ncols, BLoad bloa
   __syncthreads();
 }
 
-template <class BLoad, class Epi>
-__device__ void rows_gemm(const double* As, int lda, int K, int ncols, BLoad bload, Epi epi,
-                          double* part, double* rowsum) {
+template <class Epi>
+__device__ void rows_gemm(const double* As, int lda, int K, int ncols, const double* __restrict__ Bp, int sk,
+                          int sc, Epi epi, double* part, double* rowsum) {
   const int r = threadIdx.x & 31, g = threadIdx.x >> 5;
   double s = 0.0;
   for (int c = g; c < ncols; c += FF_GROUPS) {
     double z = 0.0;
-    for (int k = 0; k < K; ++k) z += As[r * lda + k] * bload(k, c);
+    for (int k = 0; k < K; ++k) z += As[r * lda + k] * Bp[(int64_t)k * sk + (int64_t)c * sc];
     s += epi(r, c, z);
   }
   part[r * FF_MAXBLK + g] = s;
@@ -172,7 +206,8 @@ __global__ void __launch_bounds__(FF_THREADS) ffnn_forward_kernel(nka_ffnn m, co
   T* hA = (T*)smem;                                   // 32 x lda
   T* part = hA + FF_ROWS * lda;                        // 32 x FF_MAXBLK
   T* rowsum = part + FF_ROWS * FF_MAXBLK;              // 32
-  int8_t* xl = (int8_t*)(rowsum + FF_ROWS);            // 32 x N (+ 32 x N base rows)
+  int* dsite = (int*)(rowsum + FF_ROWS);               // 32 x (FF_MAXD + 1): count, sites
+  int8_t* xl = (int8_t*)(dsite + FF_ROWS * (FF_MAXD + 1));  // 32 x N (+ 32 x N base rows)
   int8_t* xb = xl + FF_ROWS * N;
   const int64_t r0 = (int64_t)blockIdx.x * FF_ROWS;
   for (int idx = threadIdx.x; idx < FF_ROWS * N; idx += FF_THREADS) {
@@ -182,8 +217,33 @@ __global__ void __launch_bounds__(FF_THREADS) ffnn_forward_kernel(nka_ffnn m, co
     if (MODE == 1) xb[idx] = gr < rows ? xs[(gr / Kc) * N + i] : (int8_t)1;
   }
   __syncthreads();
+  if (MODE == 1) {
+    // the sites where row r differs from its sample, ascending; a count of
+    // -1 marks a row with more than FF_MAXD of them (full scan below)
+    if (threadIdx.x < FF_ROWS) {
+      const int rr = threadIdx.x;
+      int cnt = 0;
+      for (int i = 0; i < N; ++i)
+        if (xl[rr * N + i] != xb[rr * N + i]) {
+          if (cnt < FF_MAXD) dsite[rr * (FF_MAXD + 1) + 1 + cnt] = i;
+          ++cnt;
+        }
+      dsite[rr * (FF_MAXD + 1)] = cnt <= FF_MAXD ? cnt : -1;
+    }
+    __syncthreads();
+  }
   const int r = threadIdx.x / FF_GROUPS, g = threadIdx.x % FF_GROUPS;
   const int64_t gr = r0 + r;
+  int nd = 0, ds[FF_MAXD];
+  T sg[FF_MAXD];
+  if (MODE == 1) {
+    nd = dsite[r * (FF_MAXD + 1)];
+#pragma unroll
+    for (int d = 0; d < FF_MAXD; ++d) {
+      ds[d] = d < nd ? dsite[r * (FF_MAXD + 1) + 1 + d] : 0;
+      sg[d] = xl[r * N + ds[d]] > 0 ? (T)2 : (T)-2;
+    }
+  }
   for (int a = g; a < H1; a += FF_GROUPS) {
     T t;
     if (MODE == 0) {
@@ -191,23 +251,29 @@ __global__ void __launch_bounds__(FF_THREADS) ffnn_forward_kernel(nka_ffnn m, co
       if (th_out && gr < rows) th_out[gr * H1 + a] = t;
     } else {
       t = gr < rows ? th_base[(gr / Kc) * H1 + a] : (T)0;
-      const int8_t* x1 = xl + r * N;
-      const int8_t* x0 = xb + r * N;
-      for (int i = 0; i < N; ++i)
-        if (x1[i] != x0[i]) t += (x1[i] > 0 ? (T)2 : (T)-2) * v.W1[(int64_t)i * H1 + a];
+      if (nd >= 0) {
+        // same terms in the same (ascending) order as the full scan
+#pragma unroll
+        for (int d = 0; d < FF_MAXD; ++d)
+          if (d < nd) t += sg[d] * v.W1[(int64_t)ds[d] * H1 + a];
+      } else {
+        const int8_t* x1 = xl + r * N;
+        const int8_t* x0 = xb + r * N;
+        for (int i = 0; i < N; ++i)
+          if (x1[i] != x0[i]) t += (x1[i] > 0 ? (T)2 : (T)-2) * v.W1[(int64_t)i * H1 + a];
+      }
     }
     hA[r * lda + a] = log_cosh<T>(t);
   }
   __syncthreads();
   const T* W2 = v.W2;
   const T* b2 = v.b2;
-  auto bload = [=](int k, int c) { return W2[(int64_t)k * H2 + c]; };
   auto epi = [=](int rr, int c, T z) {
     const T zz = z + (b2 ? b2[c] : (T)0);
     if (g2_out && r0 + rr < rows) g2_out[(r0 + rr) * H2 + c] = tanh_t<T>(zz);
     return log_cosh<T>(zz);
   };
-  rows_gemm(hA, lda, H1, H2, bload, epi, part, rowsum);
+  rows_gemm(hA, lda, H1, H2, W2, H2, 1, epi, part, rowsum);
   if (threadIdx.x < FF_ROWS && r0 + threadIdx.x < rows) logpsi[r0 + threadIdx.x] = rowsum[threadIdx.x];
 }
 
@@ -215,7 +281,7 @@ template <class T>
 inline size_t forward_lds(const nka_ffnn& m) {
   return ffpad((size_t)FF_ROWS * lda_of(m.n_hidden1) * sizeof(T)) +
          ffpad((size_t)FF_ROWS * FF_MAXBLK * sizeof(T)) + ffpad(FF_ROWS * sizeof(T)) +
-         ffpad((size_t)2 * FF_ROWS * m.n_visible);
+         ffpad((size_t)FF_ROWS * (FF_MAXD + 1) * sizeof(int)) + ffpad((size_t)2 * FF_ROWS * m.n_visible);
 }
 
 // eloc[s] = sum_k mel[s,k] exp(lp[s,k] - lp0[s]) (FP64); one wavefront per sample
@@ -262,12 +328,11 @@ __global__ void __launch_bounds__(FF_THREADS) ffnn_backprop_kernel(nka_ffnn m, c
   }
   __syncthreads();
   const T* W2 = v.W2;
-  auto bload = [=](int k, int a) { return W2[(int64_t)a * H2 + k]; };
   auto epi = [=](int rr, int a, T u) {
     if (r0 + rr < rows) d1_out[(r0 + rr) * H1 + a] = u * tanh_t<T>(th1[(r0 + rr) * H1 + a]);
     return (T)0;
   };
-  rows_gemm(gA, ldg, H2, H1, bload, epi, part, rowsum);
+  rows_gemm(gA, ldg, H2, H1, W2, 1, H2, epi, part, rowsum);
 }
 
 template <class T>
@@ -602,9 +667,8 @@ __global__ void __launch_bounds__(FF_THREADS) ffnn_metropolis_kernel(nka_ffnn m,
   __syncthreads();
   const T* W2 = v.W2;
   const T* b2 = v.b2;
-  auto bload = [=](int k, int c) { return W2[(int64_t)k * H2 + c]; };
   auto epi = [=](int, int c, T z) { return log_cosh<T>(z + (b2 ? b2[c] : (T)0)); };
-  rows_gemm(hA, lda, H1, H2, bload, epi, part, rowsum);
+  rows_gemm(hA, lda, H1, H2, W2, H2, 1, epi, part, rowsum);
   const T pw = (T)sa.machine_pow;
   if (tid < FF_ROWS) lp[tid] = pw * rowsum[tid];
 
@@ -714,7 +778,7 @@ __global__ void __launch_bounds__(FF_THREADS) ffnn_metropolis_kernel(nka_ffnn m,
         }
       }
       __syncthreads();
-      rows_gemm(hA, lda, H1, H2, bload, epi, part, rowsum);
+      rows_gemm(hA, lda, H1, H2, W2, H2, 1, epi, part, rowsum);
       if (tid < FF_ROWS) {
         const int r = tid;
         int acc = 0;
