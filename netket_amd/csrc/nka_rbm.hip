@@ -178,125 +178,162 @@ __global__ void __launch_bounds__(256) rbm_prepare_kernel(nka_rbm m, const int8_
 // Weighted contraction over samples, split over sample chunks:
 //   part[c, v, p] = sum_{s in chunk c} O[s,p] w[v,s]
 // O[s, b_j] = t_sj, O[s, W_ij] = s_i t_sj, O[s, a_i] = s_i.
-// A workgroup owns a chunk of 64 samples and a tile of 64 hidden units; the
-// chunk's weights, spins and tanh values are staged in LDS (every global load
-// of the stage in flight at once), then lane j of wave w accumulates the W
-// entries of rows i = i0 + w + 4r of each 64-row slab over the 64 samples.
+// A workgroup owns a chunk of nsub x 64 samples and a tile of 64 hidden
+// units.  For each 64-row slab of visible units it walks the chunk in 64-sample
+// steps: the step's weights, spins and tanh values are staged in LDS (every
+// global load of the stage in flight at once), then the four waves form the
+// W entries of the slab on FP64 matrix cores (K = samples).
+// nsub > 1 for models with many hidden units, so that the partials written and
+// re-read by the reduction stay a small multiple of the parameter count.
 // ---------------------------------------------------------------------------
-constexpr int kContractChunk = 64;    // samples per partial
+typedef double ddx4 __attribute__((ext_vector_type(4)));
+constexpr int kContractChunk = 64;    // samples per staged step
 constexpr int kContractMaxW = 2;      // weight vectors per call
+
+// 64-sample steps per chunk: about 2048 workgroups over chunks x hidden tiles
+static int contract_nsub(const nka_rbm& m, int64_t ns) {
+  const int64_t steps = (ns + kContractChunk - 1) / kContractChunk;
+  const int64_t jt = (m.n_hidden + 63) / 64;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(steps, steps * jt / 2048));
+}
+static int64_t contract_chunks(const nka_rbm& m, int64_t ns) {
+  const int64_t cs = (int64_t)kContractChunk * contract_nsub(m, ns);
+  return (ns + cs - 1) / cs;
+}
 
 template <class T, int NW>
 __global__ void __launch_bounds__(256) rbm_contract_partial(nka_rbm m, const int8_t* __restrict__ sigma,
                                                             const T* __restrict__ t, int64_t ns,
                                                             const double* __restrict__ w,
-                                                            double* __restrict__ part, int64_t np) {
+                                                            double* __restrict__ part, int64_t np, int nsub) {
   __shared__ double ws[NW][kContractChunk];
   __shared__ int8_t sg[kContractChunk][64];
   __shared__ T tl[kContractChunk][64];
   const int N = m.n_visible, M = m.n_hidden;
-  const int64_t s0 = (int64_t)blockIdx.x * kContractChunk;
-  const int cn = (int)min((int64_t)kContractChunk, ns - s0);
+  const int64_t c0 = (int64_t)blockIdx.x * kContractChunk * nsub;
   const int lane = lane_id();
   const int j = blockIdx.y * 64 + lane;
   const int wid = threadIdx.x / 64;
-  {
-    // tanh tile: wave w loads samples w + 4q (q < 16), lanes along j
-    T tv[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int s = wid + 4 * q;
-      tv[q] = (s < cn && j < M) ? t[(s0 + s) * M + j] : (T)0;
-    }
-    if (threadIdx.x < NW * kContractChunk) {
-      const int vv = threadIdx.x / kContractChunk, s = threadIdx.x % kContractChunk;
-      ws[vv][s] = s < cn ? w[(int64_t)vv * ns + s0 + s] : 0.0;
-    }
-#pragma unroll
-    for (int q = 0; q < 16; ++q) tl[wid + 4 * q][lane] = tv[q];
-  }
   // partial layout: [chunk][v][np + 1]; slot np holds the chunk's sum of w[v]
   const int64_t ld = np + 1;
   double* out = part + (int64_t)blockIdx.x * NW * ld;
+  double b2[NW];
+#pragma unroll
+  for (int vv = 0; vv < NW; ++vv) b2[vv] = 0.0;
   for (int i0 = 0; i0 < N; i0 += 64) {
     const int ni = min(64, N - i0);
-    __syncthreads();
-    {
-      // spins of the slab: wave w loads samples w + 4q, lanes along i
-      int8_t sv[16];
+    // W entries of the slab on FP64 MFMA: wave (wm, wn) owns a 32 x 32 block
+    // of (visible row, hidden unit) as 2 x 2 tiles of 16 x 16;
+    // A[i][s] = sigma_si w_s, B[s][j] = t_sj, K = samples
+    ddx4 acc[NW][2][2];
+    double a2[NW];
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const int s = wid + 4 * q;
-        sv[q] = (s < cn && lane < ni) ? sigma[(s0 + s) * N + i0 + lane] : (int8_t)0;
-      }
+    for (int vv = 0; vv < NW; ++vv) {
+      a2[vv] = 0.0;
 #pragma unroll
-      for (int q = 0; q < 16; ++q) sg[wid + 4 * q][lane] = sv[q];
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[vv][a][b] = (ddx4){0.0, 0.0, 0.0, 0.0};
     }
-    __syncthreads();
-    double acc[NW][16];
+    const int wm = (wid >> 1) * 32, wn = (wid & 1) * 32;
+    for (int sb = 0; sb < nsub; ++sb) {
+      const int64_t s0 = c0 + (int64_t)sb * kContractChunk;
+      if (s0 >= ns) break;
+      const int cn = (int)min((int64_t)kContractChunk, ns - s0);
+      __syncthreads();
+      {
+        // wave w loads samples w + 4q (q < 16): tanh values along j, spins along i
+        T tv[16];
+        int8_t sv[16];
 #pragma unroll
-    for (int vv = 0; vv < NW; ++vv)
+        for (int q = 0; q < 16; ++q) {
+          const int s = wid + 4 * q;
+          tv[q] = (s < cn && j < M) ? t[(s0 + s) * M + j] : (T)0;
+          sv[q] = (s < cn && lane < ni) ? sigma[(s0 + s) * N + i0 + lane] : (int8_t)0;
+        }
+        if (threadIdx.x < NW * kContractChunk) {
+          const int vv = threadIdx.x / kContractChunk, s = threadIdx.x % kContractChunk;
+          ws[vv][s] = s < cn ? w[(int64_t)vv * ns + s0 + s] : 0.0;
+        }
 #pragma unroll
-      for (int r = 0; r < 16; ++r) acc[vv][r] = 0.0;
+        for (int q = 0; q < 16; ++q) {
+          tl[wid + 4 * q][lane] = tv[q];
+          sg[wid + 4 * q][lane] = sv[q];
+        }
+      }
+      __syncthreads();
 #pragma unroll 4
-    for (int s = 0; s < kContractChunk; ++s) {
-      const double tvs = (double)tl[s][lane];
-      double tw[NW];
+      for (int ks = 0; ks < kContractChunk; ks += 4) {
+        // operand element of lane l: A[row l%16][k l/16], B[k l/16][col l%16]
+        const int sk = ks + (lane >> 4);
+        double bv[2];
 #pragma unroll
-      for (int vv = 0; vv < NW; ++vv) tw[vv] = tvs * ws[vv][s];
+        for (int b = 0; b < 2; ++b) bv[b] = (double)tl[sk][wn + b * 16 + (lane & 15)];
+        int8_t sv[2];
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int8_t sv = sg[s][wid + 4 * r];
+        for (int a = 0; a < 2; ++a) sv[a] = sg[sk][wm + a * 16 + (lane & 15)];
 #pragma unroll
-        for (int vv = 0; vv < NW; ++vv) acc[vv][r] += sv > 0 ? tw[vv] : (sv < 0 ? -tw[vv] : 0.0);
+        for (int vv = 0; vv < NW; ++vv) {
+          const double wv = ws[vv][sk];
+#pragma unroll
+          for (int a = 0; a < 2; ++a) {
+            const double av = sv[a] > 0 ? wv : (sv[a] < 0 ? -wv : 0.0);
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+              acc[vv][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[b], acc[vv][a][b], 0, 0, 0);
+          }
+        }
       }
-    }
-    if (j < M) {
+      // visible-bias entries of this slab (tile 0), hidden-bias entries (first slab)
+      if (m.visible_off >= 0 && blockIdx.y == 0 && threadIdx.x < ni) {
+        const int ii = threadIdx.x;
+        for (int s = 0; s < cn; ++s) {
+          const double sv = sg[s][ii] > 0 ? 1.0 : -1.0;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int ii = wid + 4 * r;
-        if (ii < ni) {
-          const int64_t p = m.kernel_off + (int64_t)(i0 + ii) * M + j;
+          for (int vv = 0; vv < NW; ++vv) a2[vv] += sv * ws[vv][s];
+        }
+      }
+      if (i0 == 0 && m.bias_off >= 0 && wid == 0 && j < M) {
+        for (int s = 0; s < cn; ++s) {
+          const double tvs = (double)tl[s][lane];
 #pragma unroll
-          for (int vv = 0; vv < NW; ++vv) out[(int64_t)vv * ld + p] = acc[vv][r];
+          for (int vv = 0; vv < NW; ++vv) b2[vv] += tvs * ws[vv][s];
         }
       }
     }
-    // visible-bias entries of this slab (tile 0 only)
-    if (m.visible_off >= 0 && blockIdx.y == 0 && threadIdx.x < ni) {
-      const int ii = threadIdx.x;
-      double a2[NW];
+    // element e of lane l in a 16x16 tile is (row l/16 + 4e, col l%16)
 #pragma unroll
-      for (int vv = 0; vv < NW; ++vv) a2[vv] = 0.0;
-      for (int s = 0; s < cn; ++s) {
-        const double sv = sg[s][ii] > 0 ? 1.0 : -1.0;
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int vv = 0; vv < NW; ++vv) a2[vv] += sv * ws[vv][s];
+      for (int b = 0; b < 2; ++b) {
+        const int jj = blockIdx.y * 64 + wn + b * 16 + (lane & 15);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int ii = wm + a * 16 + (lane >> 4) + 4 * e;
+          if (ii < ni && jj < M) {
+            const int64_t p = m.kernel_off + (int64_t)(i0 + ii) * M + jj;
+#pragma unroll
+            for (int vv = 0; vv < NW; ++vv) out[(int64_t)vv * ld + p] = acc[vv][a][b][e];
+          }
+        }
       }
+    if (m.visible_off >= 0 && blockIdx.y == 0 && threadIdx.x < ni) {
 #pragma unroll
-      for (int vv = 0; vv < NW; ++vv) out[(int64_t)vv * ld + m.visible_off + i0 + ii] = a2[vv];
+      for (int vv = 0; vv < NW; ++vv) out[(int64_t)vv * ld + m.visible_off + i0 + threadIdx.x] = a2[vv];
     }
   }
   // per-chunk weight sums (used to centre the contraction)
   if (blockIdx.y == 0 && wid == 1) {
+    const int64_t cend = min(ns, c0 + (int64_t)kContractChunk * nsub);
 #pragma unroll
     for (int vv = 0; vv < NW; ++vv) {
       double sw = 0.0;
-      for (int s = lane; s < cn; s += 64) sw += ws[vv][s];
+      for (int64_t s = c0 + lane; s < cend; s += 64) sw += w[(int64_t)vv * ns + s];
       sw = wave_sum(sw);
       if (lane == 0) out[(int64_t)vv * ld + np] = sw;
     }
   }
-  // hidden-bias entries
   if (m.bias_off >= 0 && wid == 0 && j < M) {
-    double b2[NW];
-#pragma unroll
-    for (int vv = 0; vv < NW; ++vv) b2[vv] = 0.0;
-    for (int s = 0; s < cn; ++s) {
-      const double tvs = (double)tl[s][lane];
-#pragma unroll
-      for (int vv = 0; vv < NW; ++vv) b2[vv] += tvs * ws[vv][s];
-    }
 #pragma unroll
     for (int vv = 0; vv < NW; ++vv) out[(int64_t)vv * ld + m.bias_off + j] = b2[vv];
   }
@@ -555,7 +592,6 @@ static void launch_eloc_f64(const nka_rbm& m, const OpArgs& op, const int8_t* sg
 // block per wave), FP64 on v_mfma_f64_16x16x4f64 (2x2 blocks per wave).
 // The epilogue writes t = tanh(theta) to t (ns, M) and/or tT (M, ns).
 typedef float ffx16 __attribute__((ext_vector_type(16)));
-typedef double ddx4 __attribute__((ext_vector_type(4)));
 constexpr int PG_T = 64, PG_K = 16;
 
 template <class T, bool RAW>
@@ -975,8 +1011,7 @@ int nka_rbm_prepare(const nka_rbm* m, const int8_t* sigma, int64_t ns, void* t, 
 
 int64_t nka_rbm_contract_workspace(const nka_rbm* m, int64_t ns, int n_w) {
   if (!m || ns <= 0 || n_w <= 0) return 0;
-  const int64_t chunks = (ns + kContractChunk - 1) / kContractChunk;
-  return chunks * n_w * (n_params(*m) + 1) * (int64_t)sizeof(double);
+  return contract_chunks(*m, ns) * n_w * (n_params(*m) + 1) * (int64_t)sizeof(double);
 }
 
 int nka_rbm_contract(const nka_rbm* m, const int8_t* sigma, const void* t, int64_t ns,
@@ -988,23 +1023,24 @@ int nka_rbm_contract(const nka_rbm* m, const int8_t* sigma, const void* t, int64
   NKA_CHECK_ARG(ns > 0 && sigma && t && w && out && workspace, "nka_rbm_contract: bad buffers");
   hipStream_t s = (hipStream_t)stream;
   const int64_t np = n_params(*m);
-  const int chunks = (int)((ns + kContractChunk - 1) / kContractChunk);
+  const int nsub = contract_nsub(*m, ns);
+  const int chunks = (int)contract_chunks(*m, ns);
   dim3 grid(chunks, (m->n_hidden + 63) / 64);
   double* ws = (double*)workspace;
   if (m->dtype == NKA_F32) {
     if (n_w == 1)
       hipLaunchKernelGGL((rbm_contract_partial<float, 1>), grid, dim3(256), 0, s, *m, sigma,
-                         (const float*)t, ns, w, ws, np);
+                         (const float*)t, ns, w, ws, np, nsub);
     else
       hipLaunchKernelGGL((rbm_contract_partial<float, 2>), grid, dim3(256), 0, s, *m, sigma,
-                         (const float*)t, ns, w, ws, np);
+                         (const float*)t, ns, w, ws, np, nsub);
   } else {
     if (n_w == 1)
       hipLaunchKernelGGL((rbm_contract_partial<double, 1>), grid, dim3(256), 0, s, *m, sigma,
-                         (const double*)t, ns, w, ws, np);
+                         (const double*)t, ns, w, ws, np, nsub);
     else
       hipLaunchKernelGGL((rbm_contract_partial<double, 2>), grid, dim3(256), 0, s, *m, sigma,
-                         (const double*)t, ns, w, ws, np);
+                         (const double*)t, ns, w, ws, np, nsub);
   }
   const int64_t len = (int64_t)n_w * np;
   unsigned g = (unsigned)std::min<int64_t>((len + 255) / 256, 4096);
