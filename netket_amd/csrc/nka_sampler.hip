@@ -342,8 +342,12 @@ __global__ void __launch_bounds__(256) rbm_metropolis_wide_kernel(nka_rbm m, Swe
       delta = uniform_value<float>(delta);
       const float u = uniform_value<float>(uniform01<float>(rnd));
       if (u < exp_t<float>(pw * delta)) {
+        // recompute t rather than keep it live through the reduction (an
+        // opaque copy of g stops the compiler from reusing the values)
+        float gr = g;
+        asm volatile("" : "+v"(gr));
 #pragma unroll
-        for (int jj = 0; jj < JL; ++jj) th[jj] = fmaf(g, w[jj], th[jj]);
+        for (int jj = 0; jj < JL; ++jj) th[jj] = fmaf(gr, w[jj], th[jj]);
         rP0 = 1.0f / P;
         if (lane == 0) xs[site] = (int8_t)(-xs[site]);
         lp += pw * delta;
