@@ -188,17 +188,19 @@ __device__ inline T theta1_full(const FfnnView<T>& v, const int8_t* x, int a) {
 // ---------------------------------------------------------------------------
 // Batched forward pass.  MODE 0: rows are configurations x[r]; theta1 from
 // scratch (optionally stored to th_out).  MODE 1: rows r = (s, k) of
-// connected configurations xp[r] of sample s = r / K; theta1 is the sample's
-// theta1 (th_base[s]) plus 2 x'_i W1[i] for every site where x' differs from
-// x[s].  Outputs log psi per row; g2_out (rows x H2), when set, receives
-// tanh(z2).
+// connected configurations xp[r] of sample s = r / K, taken from the list
+// ridx[0 .. *rcount) (the rows with a nonzero matrix element); theta1 is the
+// sample's theta1 (th_base[s]) plus 2 x'_i W1[i] for every site where x'
+// differs from x[s].  Outputs log psi per row (MODE 1: at the row's own
+// index); g2_out (rows x H2), when set, receives tanh(z2).
 // ---------------------------------------------------------------------------
 template <class T, int MODE>
 __global__ void __launch_bounds__(FF_THREADS) ffnn_forward_kernel(nka_ffnn m, const int8_t* __restrict__ xr,
                                                                  const int8_t* __restrict__ xs, int Kc,
                                                                  const T* __restrict__ th_base, int64_t rows,
                                                                  T* __restrict__ logpsi, T* __restrict__ th_out,
-                                                                 T* __restrict__ g2_out) {
+                                                                 T* __restrict__ g2_out, const int* __restrict__ ridx,
+                                                                 const int* __restrict__ rcount) {
   extern __shared__ unsigned char smem[];
   const FfnnView<T> v = ffnn_view<T>(m);
   const int N = v.N, H1 = v.H1, H2 = v.H2;
@@ -210,11 +212,20 @@ __global__ void __launch_bounds__(FF_THREADS) ffnn_forward_kernel(nka_ffnn m, co
   int8_t* xl = (int8_t*)(dsite + FF_ROWS * (FF_MAXD + 1));  // 32 x N (+ 32 x N base rows)
   int8_t* xb = xl + FF_ROWS * N;
   const int64_t r0 = (int64_t)blockIdx.x * FF_ROWS;
+  // MODE 1: tile rows r0 .. r0+31 of the row list; the grid covers the
+  // longest possible list, so tiles past its end leave at once
+  const int64_t nrows = MODE == 1 ? (int64_t)*rcount : rows;
+  if (r0 >= nrows) return;
+  auto row_of = [&](int r) -> int64_t {
+    const int64_t cr = r0 + r;
+    if (cr >= nrows) return -1;
+    return MODE == 1 ? (int64_t)ridx[cr] : cr;
+  };
   for (int idx = threadIdx.x; idx < FF_ROWS * N; idx += FF_THREADS) {
     const int r = idx / N, i = idx % N;
-    const int64_t gr = r0 + r;
-    xl[idx] = gr < rows ? xr[gr * N + i] : (int8_t)1;
-    if (MODE == 1) xb[idx] = gr < rows ? xs[(gr / Kc) * N + i] : (int8_t)1;
+    const int64_t gr = row_of(r);
+    xl[idx] = gr >= 0 ? xr[gr * N + i] : (int8_t)1;
+    if (MODE == 1) xb[idx] = gr >= 0 ? xs[(gr / Kc) * N + i] : (int8_t)1;
   }
   __syncthreads();
   if (MODE == 1) {
@@ -233,7 +244,7 @@ __global__ void __launch_bounds__(FF_THREADS) ffnn_forward_kernel(nka_ffnn m, co
     __syncthreads();
   }
   const int r = threadIdx.x / FF_GROUPS, g = threadIdx.x % FF_GROUPS;
-  const int64_t gr = r0 + r;
+  const int64_t gr = row_of(r);
   int nd = 0, ds[FF_MAXD];
   T sg[FF_MAXD];
   if (MODE == 1) {
@@ -248,9 +259,9 @@ __global__ void __launch_bounds__(FF_THREADS) ffnn_forward_kernel(nka_ffnn m, co
     T t;
     if (MODE == 0) {
       t = theta1_full<T>(v, xl + r * N, a);
-      if (th_out && gr < rows) th_out[gr * H1 + a] = t;
+      if (th_out && gr >= 0) th_out[gr * H1 + a] = t;
     } else {
-      t = gr < rows ? th_base[(gr / Kc) * H1 + a] : (T)0;
+      t = gr >= 0 ? th_base[(gr / Kc) * H1 + a] : (T)0;
       if (nd >= 0) {
         // same terms in the same (ascending) order as the full scan
 #pragma unroll
@@ -274,7 +285,10 @@ __global__ void __launch_bounds__(FF_THREADS) ffnn_forward_kernel(nka_ffnn m, co
     return log_cosh<T>(zz);
   };
   rows_gemm(hA, lda, H1, H2, W2, H2, 1, epi, part, rowsum);
-  if (threadIdx.x < FF_ROWS && r0 + threadIdx.x < rows) logpsi[r0 + threadIdx.x] = rowsum[threadIdx.x];
+  if (threadIdx.x < FF_ROWS) {
+    const int64_t go = row_of(threadIdx.x);
+    if (go >= 0) logpsi[go] = rowsum[threadIdx.x];
+  }
 }
 
 template <class T>
@@ -282,6 +296,29 @@ inline size_t forward_lds(const nka_ffnn& m) {
   return ffpad((size_t)FF_ROWS * lda_of(m.n_hidden1) * sizeof(T)) +
          ffpad((size_t)FF_ROWS * FF_MAXBLK * sizeof(T)) + ffpad(FF_ROWS * sizeof(T)) +
          ffpad((size_t)FF_ROWS * (FF_MAXD + 1) * sizeof(int)) + ffpad((size_t)2 * FF_ROWS * m.n_visible);
+}
+
+// List of the connected rows with a nonzero matrix element (only those enter
+// the local energy; the reduction below skips the others).  Slots are taken by
+// one atomic add per wavefront, so the list order varies between runs, but
+// every row's log psi is formed from that row alone, so the results do not.
+__global__ void ffnn_zero_count_kernel(int* __restrict__ count) {
+  if (threadIdx.x == 0) count[0] = 0;
+}
+
+template <class M>
+__global__ void __launch_bounds__(256) ffnn_compact_rows_kernel(const M* __restrict__ mels, int64_t rows,
+                                                                int* __restrict__ ridx, int* __restrict__ count) {
+  const int lane = lane_id();
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < rows; base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = base + threadIdx.x;
+    const bool nz = r < rows && mels[r] != (M)0;
+    const uint64_t bal = __ballot(nz);
+    int slot0 = 0;
+    if (lane == 0 && bal) slot0 = atomicAdd(count, (int)__popcll(bal));
+    slot0 = __shfl(slot0, 0, 64);
+    if (nz) ridx[slot0 + (int)__popcll(bal & ((1ull << lane) - 1ull))] = (int)r;
+  }
 }
 
 // eloc[s] = sum_k mel[s,k] exp(lp[s,k] - lp0[s]) (FP64); one wavefront per sample
@@ -885,7 +922,8 @@ static void launch_forward0(const nka_ffnn& m, const int8_t* x, int64_t rows, T*
   const unsigned g = (unsigned)((rows + FF_ROWS - 1) / FF_ROWS);
   allow_lds(ffnn_forward_kernel<T, 0>, (int64_t)forward_lds<T>(m));
   hipLaunchKernelGGL((ffnn_forward_kernel<T, 0>), dim3(g), dim3(FF_THREADS), forward_lds<T>(m), s, m, x,
-                     (const int8_t*)nullptr, 1, (const T*)nullptr, rows, lp, th, g2);
+                     (const int8_t*)nullptr, 1, (const T*)nullptr, rows, lp, th, g2, (const int*)nullptr,
+                     (const int*)nullptr);
 }
 
 static int op_rows(int op_kind, const void* op, int N, int* K, int* mel_dtype) {
@@ -906,7 +944,8 @@ static int op_rows(int op_kind, const void* op, int N, int* K, int* mel_dtype) {
 // bytes per sample of the local-energy workspace
 static int64_t eloc_bytes_per_sample(const nka_ffnn& m, int K) {
   const int64_t ts = m.dtype == NKA_F32 ? 4 : 8;
-  return ffpad((int64_t)K * m.n_visible) + (int64_t)K * 8 + (int64_t)K * ts + (int64_t)m.n_hidden1 * ts + ts + 4;
+  return ffpad((int64_t)K * m.n_visible) + (int64_t)K * 8 + (int64_t)K * ts + (int64_t)m.n_hidden1 * ts + ts +
+         (int64_t)K * 4 + 4;
 }
 
 template <class T>
@@ -932,6 +971,9 @@ static int ffnn_eloc_impl(const nka_ffnn& m, int op_kind, const void* op, const 
   T* lp = (T*)take(chunk * K * sizeof(T));
   T* th = (T*)take(chunk * H1 * sizeof(T));
   T* lp0own = (T*)take(chunk * sizeof(T));
+  int* ridx = (int*)take(chunk * K * (int64_t)sizeof(int));
+  int* rcount = (int*)take(16);
+  NKA_CHECK_ARG(chunk * (int64_t)K < (1LL << 31), "nka_ffnn_local_energy: too many connected rows per chunk");
   for (int64_t s0 = 0; s0 < n; s0 += chunk) {
     const int64_t cs = std::min<int64_t>(chunk, n - s0);
     const int8_t* x = sigma + s0 * N;
@@ -947,9 +989,17 @@ static int ffnn_eloc_impl(const nka_ffnn& m, int op_kind, const void* op, const 
     launch_forward0<T>(m, x, cs, lp0, th, nullptr, s);
     const int64_t rows = cs * K;
     const unsigned g = (unsigned)((rows + FF_ROWS - 1) / FF_ROWS);
+    hipLaunchKernelGGL(ffnn_zero_count_kernel, dim3(1), dim3(64), 0, s, rcount);
+    const unsigned gc = (unsigned)std::min<int64_t>((rows + 255) / 256, 8192);
+    if (md == NKA_F32)
+      hipLaunchKernelGGL((ffnn_compact_rows_kernel<float>), dim3(gc), dim3(256), 0, s, (const float*)mels, rows,
+                         ridx, rcount);
+    else
+      hipLaunchKernelGGL((ffnn_compact_rows_kernel<double>), dim3(gc), dim3(256), 0, s, (const double*)mels, rows,
+                         ridx, rcount);
     allow_lds(ffnn_forward_kernel<T, 1>, (int64_t)forward_lds<T>(m));
     hipLaunchKernelGGL((ffnn_forward_kernel<T, 1>), dim3(g), dim3(FF_THREADS), forward_lds<T>(m), s, m, xp, x, K,
-                       (const T*)th, rows, lp, (T*)nullptr, (T*)nullptr);
+                       (const T*)th, rows, lp, (T*)nullptr, (T*)nullptr, (const int*)ridx, (const int*)rcount);
     const unsigned gr = (unsigned)((cs + 3) / 4);
     if (md == NKA_F32)
       hipLaunchKernelGGL((ffnn_eloc_reduce_kernel<T, float>), dim3(gr), dim3(256), 0, s, (const float*)mels,
