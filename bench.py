@@ -314,7 +314,14 @@ def measured_traffic(kernel: str):
     try:
         with open(path) as f:
             data = json.load(f)
-        return data["kernels"][kernel]["hbm_bytes_per_launch"]
+        kernels = data["kernels"]
+        if kernel in kernels:
+            return kernels[kernel]["hbm_bytes_per_launch"]
+        # template instances are listed as name<args>
+        for name, rec in kernels.items():
+            if name.split("<")[0] == kernel:
+                return rec["hbm_bytes_per_launch"]
+        return None
     except (OSError, KeyError, ValueError):
         return None
 
