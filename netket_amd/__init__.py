@@ -6,10 +6,10 @@ kernels in ``libnetket_amd.so`` (``include/netket_amd.h``).  PyTorch provides
 device memory and the current stream only.
 """
 
-from . import graph, hilbert, logging, models, operator, optimizer, sampler, stats, vqs
+from . import graph, hilbert, logging, models, operator, optimizer, sampler, serialization, stats, vqs
 from . import driver
 from .driver import VMC, VMC_SR
 
-__all__ = ["graph", "hilbert", "logging", "models", "operator", "optimizer", "sampler", "stats",
+__all__ = ["graph", "hilbert", "logging", "models", "operator", "optimizer", "sampler", "serialization", "stats",
            "vqs", "driver", "VMC", "VMC_SR"]
 __version__ = "0.1.0"

@@ -78,3 +78,35 @@ def test_graph_replay_matches_eager(gpu, ntk):
     torch.cuda.synchronize()
     np.testing.assert_array_equal(v1.parameters_flat.cpu().numpy(), v2.parameters_flat.cpu().numpy())
     np.testing.assert_array_equal(np.array(e1[2:]), np.array(e2))
+
+
+def test_mcstate_serialization_round_trip_and_json_log(gpu, tmp_path):
+    # serialize_MCState / deserialize_MCState (netket/vqs/mc/mc_state/state.py:963-1016):
+    # a restored state continues the chains exactly; JsonLog writes the .log history
+    # and the variables as .mpack (netket/_src/logging/json_log.py:28-237)
+    import json
+
+    H, vs = _tfim_state(10, 8, 256, 2, seed=3, dtype=np.float32)
+    drv = nk.VMC(H, nk.optimizer.Sgd(0.02), variational_state=vs,
+                 preconditioner=nk.optimizer.SR(nk.optimizer.QGTJacobianDense, diag_shift=0.01))
+    prefix = tmp_path / "run"
+    drv.run(3, out=nk.logging.JsonLog(prefix, save_params_every=1, write_every=1))
+    data = json.load(open(str(prefix) + ".log"))
+    assert data["Energy"]["iters"] == [0, 1, 2] and len(data["Energy"]["Mean"]) == 3
+    saved = nk.serialization.msgpack_restore(open(str(prefix) + ".mpack", "rb").read())
+    for k, v in vs.parameters.items():
+        if isinstance(v, dict):
+            for kk, vv in v.items():
+                np.testing.assert_array_equal(saved["params"][k][kk], vv.cpu().numpy())
+        else:
+            np.testing.assert_array_equal(saved["params"][k], v.cpu().numpy())
+
+    blob = nk.serialization.to_bytes(vs)
+    _, other = _tfim_state(10, 8, 128, 5, seed=99, dtype=np.float32)
+    nk.serialization.from_bytes(other, blob)
+    np.testing.assert_array_equal(other.parameters_flat.cpu().numpy(), vs.parameters_flat.cpu().numpy())
+    assert other.n_samples == vs.n_samples and other.n_discard_per_chain == vs.n_discard_per_chain
+    np.testing.assert_array_equal(other.sampler_state.sigma.cpu().numpy(), vs.sampler_state.sigma.cpu().numpy())
+    s1 = vs.sample().cpu().numpy()
+    s2 = other.sample().cpu().numpy()
+    np.testing.assert_array_equal(s1, s2)
