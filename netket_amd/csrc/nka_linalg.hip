@@ -1358,7 +1358,12 @@ int nka_gemm_nt_f64(const double* a, int64_t lda, const double* b, int64_t ldb, 
                     void* stream) {
   NKA_CHECK_ARG(m >= 0 && n >= 0 && k >= 0 && lda >= k && ldb >= k && ldc >= n,
                 "nka_gemm_nt_f64: bad sizes");
-  gemm_launch(a, lda, b, ldb, c, ldc, m, n, k, alpha, beta, 0.0, 0, nullptr, (hipStream_t)stream);
+  if (k == 0) {
+    gemm_launch(a, lda, b, ldb, c, ldc, m, n, k, alpha, beta, 0.0, 0, nullptr, (hipStream_t)stream);
+  } else {
+    // 128x128 tiles when K is a multiple of their K step, else 64x64 tiles
+    gemm_big_launch(a, lda, b, ldb, c, ldc, m, n, k, alpha, beta, 0, nullptr, (hipStream_t)stream);
+  }
   NKA_CHECK_LAUNCH("nka_gemm_nt_f64");
   return NKA_OK;
 }

@@ -68,6 +68,7 @@ class MCState:
             chain_length = compute_chain_length(sampler.n_chains, 1000 if n_samples is None else n_samples)
         self._chain_length = int(chain_length)
         self.n_discard_per_chain = n_discard_per_chain
+        self._chunk_size = None
         self.chunk_size = chunk_size
         self._samples = None
         self._jac_cache = None
@@ -100,6 +101,25 @@ class MCState:
         if v is not None and v < 0:
             raise ValueError(f"Invalid number of discarded samples: n_discard_per_chain={v}")
         self._n_discard = 5 if v is None else int(v)
+
+    @property
+    def chunk_size(self) -> int | None:
+        """Largest number of samples whose local energies are evaluated in one
+        launch (netket/vqs/mc/mc_state/state.py:471-512); None evaluates all at
+        once.  The values do not depend on it."""
+        return self._chunk_size
+
+    @chunk_size.setter
+    def chunk_size(self, chunk_size):
+        if chunk_size is None:
+            self._chunk_size = None
+            return
+        if not isinstance(chunk_size, (int, np.integer)) or isinstance(chunk_size, bool) or chunk_size <= 0:
+            raise ValueError(f"Chunk size must be a positive INTEGER (got {chunk_size} instead).")
+        if chunk_size & (chunk_size - 1):
+            warnings.warn("For performance reasons, we suggest to use a power-of-two chunk size.",
+                          stacklevel=2)
+        self._chunk_size = int(chunk_size)
 
     @property
     def n_parameters(self) -> int:
@@ -166,10 +186,18 @@ class MCState:
 
     # -- estimators ----------------------------------------------------------------
     def local_estimators(self, op, *, chunk_size: int | None = None) -> torch.Tensor:
-        """E_loc of the cached samples, shape (n_chains, chain_length), FP64."""
+        """E_loc of the cached samples, shape (n_chains, chain_length), FP64, in
+        launches of at most ``chunk_size`` (default: the state's) samples."""
         s = self.samples
         C, L, N = s.shape
-        e = eng.local_energies(self.model, self._flat, op, s.reshape(C * L, N))
+        flat_s = s.reshape(C * L, N)
+        cs = self._chunk_size if chunk_size is None else int(chunk_size)
+        if cs is None or cs >= C * L:
+            e = eng.local_energies(self.model, self._flat, op, flat_s)
+        else:
+            e = torch.empty(C * L, dtype=torch.float64, device=s.device)
+            for i in range(0, C * L, cs):
+                e[i:i + cs] = eng.local_energies(self.model, self._flat, op, flat_s[i:i + cs])
         return e.reshape(C, L)
 
     def expect(self, op) -> Stats:

@@ -110,3 +110,20 @@ def test_mcstate_serialization_round_trip_and_json_log(gpu, tmp_path):
     s1 = vs.sample().cpu().numpy()
     s2 = other.sample().cpu().numpy()
     np.testing.assert_array_equal(s1, s2)
+
+
+def test_chunk_size_local_energies_identical(gpu):
+    # MCState.chunk_size (netket/vqs/mc/mc_state/state.py:471-512) bounds the
+    # samples per launch; the local energies do not depend on it
+    H, vs = _tfim_state(12, 16, 1024, 2, seed=4, dtype=np.float32)
+    e_all = vs.local_estimators(H).cpu().numpy()
+    with pytest.warns(UserWarning):
+        vs.chunk_size = 100
+    e_chunk = vs.local_estimators(H).cpu().numpy()
+    np.testing.assert_array_equal(e_chunk, e_all)
+    np.testing.assert_array_equal(vs.local_estimators(H, chunk_size=64).cpu().numpy(), e_all)
+    for bad in (0, -4, 2.5):
+        with pytest.raises(ValueError):
+            vs.chunk_size = bad
+    vs.chunk_size = None
+    assert vs.chunk_size is None
