@@ -797,18 +797,25 @@ __global__ void __launch_bounds__(FF_THREADS) ffnn_metropolis_kernel(nka_ffnn m,
       // theta1' and h1' of every chain
       T thn[HPT];
       {
+        // at most two flipped sites; their W1 rows are read with every load
+        // of the thread's units in flight (no loop over the flips per unit)
         const int nf = pnf[cr];
         const int8_t* x = xs + cr * N;
+        const int s0 = nf > 0 ? psite[2 * cr] : 0;
+        const int s1 = nf > 1 ? psite[2 * cr + 1] : 0;
+        // x'_site = -x_site: theta1 gains (x' - x) W1 = -2 x W1
+        const T c0 = x[s0] > 0 ? (T)-2 : (T)2;
+        const T c1 = x[s1] > 0 ? (T)-2 : (T)2;
+        const T* w0 = v.W1 + (int64_t)s0 * H1;
+        const T* w1 = v.W1 + (int64_t)s1 * H1;
 #pragma unroll
         for (int j = 0; j < HPT; ++j) {
           const int a = g + FF_GROUPS * j;
           T t = th[j];
           if (a < H1) {
-            for (int f = 0; f < nf; ++f) {
-              const int site = psite[2 * cr + f];
-              // x'_site = -x_site: theta1 gains (x' - x) W1 = -2 x W1
-              t += (x[site] > 0 ? (T)-2 : (T)2) * v.W1[(int64_t)site * H1 + a];
-            }
+            const T u0 = w0[a], u1 = w1[a];
+            if (nf > 0) t += c0 * u0;
+            if (nf > 1) t += c1 * u1;
             hA[cr * lda + a] = log_cosh<T>(t);
           }
           thn[j] = t;
@@ -891,6 +898,7 @@ static void dispatch_ffnn_sweep(const nka_ffnn& m, const FfSweepArgs& sa, int8_t
   if (need <= 8) launch_ffnn_sweep<T, 8>(m, sa, sigma, lp, smp, lps, nacc, s);
   else if (need <= 16) launch_ffnn_sweep<T, 16>(m, sa, sigma, lp, smp, lps, nacc, s);
   else if (need <= 32) launch_ffnn_sweep<T, 32>(m, sa, sigma, lp, smp, lps, nacc, s);
+  else if (need <= 56) launch_ffnn_sweep<T, 56>(m, sa, sigma, lp, smp, lps, nacc, s);
   else if (need <= 64) launch_ffnn_sweep<T, 64>(m, sa, sigma, lp, smp, lps, nacc, s);
   else launch_ffnn_sweep<T, 128>(m, sa, sigma, lp, smp, lps, nacc, s);
 }
