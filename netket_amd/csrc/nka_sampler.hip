@@ -634,6 +634,20 @@ __global__ void advance_counter_kernel(uint64_t* __restrict__ c, uint64_t inc) {
   if (threadIdx.x == 0 && blockIdx.x == 0) c[0] = c[0] + inc;
 }
 
+// compute units of the current device (queried once per process)
+static int device_cus() {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+      cus = n;
+    else
+      cus = 256;
+  }
+  return cus;
+}
+
 template <class T>
 static bool stage_fits_s(const nka_rbm& m, int waves) {
   return pad16((size_t)m.n_visible * m.n_hidden * sizeof(T)) + waves * pad16(m.n_visible) <=
@@ -647,11 +661,17 @@ static size_t group_lds(const nka_rbm& m, const SweepArgs& sa, int waves, int G)
 }
 
 // Chains per wavefront for the grouped kernel (0: one wavefront per chain):
-// 4 chains of 16 lanes when M <= 128, 2 of 32 lanes when M <= 256, provided
-// the clusters fit the per-lane slots and W is staged in LDS.
+// 4 chains of 16 lanes when M <= 128 and there are chains enough for two such
+// wavefronts per SIMD, else 2 of 32 lanes when M <= 256, provided the clusters
+// fit the per-lane slots and W is staged in LDS.
 template <class T>
 static int pick_group(const nka_rbm& m, const SweepArgs& sa, int waves) {
+  // four chains per wavefront only when that still leaves two wavefronts per
+  // SIMD (the chains are latency-bound): measured on MI355X with 4096 chains,
+  // M = 128: two chains per wavefront 0.79 ms, four 1.05 ms
+  const bool four_ok = (int64_t)sa.C >= (int64_t)4 * 8 * device_cus();
   for (int G : {4, 2}) {
+    if (G == 4 && !four_ok) continue;
     const int lpc = NKA_WAVE / G;
     if (m.n_hidden > 8 * lpc) continue;
     if (sa.rule == NKA_RULE_EXCHANGE && sa.n_clusters > kMaxClusterSlots * lpc) continue;

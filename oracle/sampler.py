@@ -72,9 +72,12 @@ def local_moves(x: np.ndarray):
 
 def metropolis(params, sigma0: np.ndarray, *, rule: str, clusters=None, n_discard: int,
                chain_length: int, sweep_size: int, seed: int, counter: int,
-               machine_pow: float = 2.0, log_psi_fn=None):
+               machine_pow: float = 2.0, log_psi_fn=None, chain_ids=None):
     """Returns (samples (C, L, N), final sigma, n_accepted per chain).  The
-    ansatz is the RBM of ``params`` unless ``log_psi_fn(params, x)`` is given."""
+    ansatz is the RBM of ``params`` unless ``log_psi_fn(params, x)`` is given.
+    ``chain_ids`` (default 0..C-1) are the chain indices of the rows of
+    ``sigma0`` in the random streams, so a subset of a larger batch of chains
+    can be followed."""
     log_psi_ = log_psi if log_psi_fn is None else log_psi_fn
     sigma = np.array(sigma0, dtype=np.int8, copy=True)
     C, N = sigma.shape
@@ -82,8 +85,10 @@ def metropolis(params, sigma0: np.ndarray, *, rule: str, clusters=None, n_discar
     nacc = np.zeros(C, dtype=np.int64)
     if clusters is not None:
         clusters = np.asarray(clusters, dtype=np.int64)
-    for c in range(C):
-        x = sigma[c].astype(np.int8).copy()
+    ids = np.arange(C) if chain_ids is None else np.asarray(chain_ids)
+    for row in range(C):
+        c = int(ids[row])
+        x = sigma[row].astype(np.int8).copy()
         lp = machine_pow * log_psi_(params, x[None])[0]
         q = counter
         for sw in range(n_discard + chain_length):
@@ -100,8 +105,8 @@ def metropolis(params, sigma0: np.ndarray, *, rule: str, clusters=None, n_discar
                 if u < np.exp(lpp - lp + corr):
                     x = xp
                     lp = lpp
-                    nacc[c] += 1
+                    nacc[row] += 1
             if sw >= n_discard:
-                samples[c, sw - n_discard] = x
-        sigma[c] = x
+                samples[row, sw - n_discard] = x
+        sigma[row] = x
     return samples, sigma, nacc

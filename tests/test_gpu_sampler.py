@@ -34,31 +34,37 @@ def test_random_states_match_oracle(gpu, N, n_down):
         assert np.all((st.sigma.cpu().numpy() < 0).sum(axis=1) == n_down)
 
 
-@pytest.mark.parametrize("rule", ["local", "exchange"])
-def test_chains_identical_to_oracle_fp64(gpu, rule):
-    print(f"case chains {rule}", flush=True)
+@pytest.mark.parametrize("rule,n_chains", [("local", 6), ("exchange", 6), ("local", 8192),
+                                           ("exchange", 8192)])
+def test_chains_identical_to_oracle_fp64(gpu, rule, n_chains):
+    # 8192 chains of a small model run four chains per wavefront, fewer run two
+    print(f"case chains {rule} n_chains={n_chains}", flush=True)
     N = 10
     model, flat, p = _model(N, 2, np.float64, 3, 0.4)
     g = nk.graph.Chain(N)
     if rule == "local":
         hi = nk.hilbert.Spin(0.5, N=N)
-        sa = nk.sampler.MetropolisLocal(hi, n_chains=6, sweep_size=7)
+        sa = nk.sampler.MetropolisLocal(hi, n_chains=n_chains, sweep_size=7)
         clusters = None
     else:
         hi = nk.hilbert.Spin(0.5, N=N, total_sz=0)
-        sa = nk.sampler.MetropolisExchange(hi, graph=g, n_chains=6, sweep_size=7)
+        sa = nk.sampler.MetropolisExchange(hi, graph=g, n_chains=n_chains, sweep_size=7)
         clusters = sa.rule.clusters
     st = sa.init_state(model, flat, seed=99)
     sigma0 = st.sigma.cpu().numpy().copy()
     counter0 = st.counter
     samples, st = sa.sample_fused(model, flat, st, 3, 9)
-    ref, ref_sigma, ref_acc = osamp.metropolis(p, sigma0, rule=rule, clusters=clusters, n_discard=3,
-                                               chain_length=9, sweep_size=7, seed=99, counter=counter0)
-    np.testing.assert_array_equal(samples.cpu().numpy(), ref)
-    np.testing.assert_array_equal(st.sigma.cpu().numpy(), ref_sigma)
-    np.testing.assert_array_equal(st.n_accepted_proc.cpu().numpy(), ref_acc)
+    # the oracle follows every chain of a small batch, and a spread of chains
+    # (both ends, the first and last of a wavefront's group) of a large one
+    ids = np.arange(n_chains) if n_chains <= 64 else np.array([0, 1, 2, 3, 4, 4095, 4096, 8190, 8191])
+    ref, ref_sigma, ref_acc = osamp.metropolis(p, sigma0[ids], rule=rule, clusters=clusters, n_discard=3,
+                                               chain_length=9, sweep_size=7, seed=99, counter=counter0,
+                                               chain_ids=ids)
+    np.testing.assert_array_equal(samples.cpu().numpy()[ids], ref)
+    np.testing.assert_array_equal(st.sigma.cpu().numpy()[ids], ref_sigma)
+    np.testing.assert_array_equal(st.n_accepted_proc.cpu().numpy()[ids], ref_acc)
     # log_prob of the chain state is 2 Re log psi of the final configurations
-    np.testing.assert_allclose(st.log_prob.cpu().numpy(), 2 * orbm.log_psi(p, ref_sigma), rtol=1e-12)
+    np.testing.assert_allclose(st.log_prob.cpu().numpy()[ids], 2 * orbm.log_psi(p, ref_sigma), rtol=1e-12)
 
 
 def _kl(counts, probs):
