@@ -947,7 +947,9 @@ __global__ void __launch_bounds__(256) trsv_fwd_lookback(const double* __restric
   const double bk = (wid == 0 && r < nb) ? b[k0 + r] : 0.0;
   // rows past nb re-read the last row; their sums are never used
   const double* arow = A + (k0 + min(r, nb - 1)) * lda;
-  double acc = 0.0;
+  // four interleaved partial sums: the dependent chain of each 64-term dot
+  // product is 16 FMAs instead of 64 (it is on the critical path of the solve)
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (int jb = wid; jb < kb; jb += 4) {
     const int64_t j0 = (int64_t)jb * CH_NB;
     double Lrow[CH_NB];
@@ -960,18 +962,19 @@ __global__ void __launch_bounds__(256) trsv_fwd_lookback(const double* __restric
     xs[wid][r] = y[j0 + r];
     wave_lds_sync();
 #pragma unroll
-    for (int c = 0; c < CH_NB; ++c) acc += Lrow[c] * xs[wid][c];
+    for (int c = 0; c < CH_NB; ++c) acc[c & 3] += Lrow[c] * xs[wid][c];
     wave_lds_sync();
   }
-  part[wid][r] = acc;
+  part[wid][r] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   __syncthreads();
   if (wid == 0) {
     t[r] = (r < nb) ? bk - (part[0][r] + part[1][r] + part[2][r] + part[3][r]) : 0.0;
     wave_lds_sync();
     // inv_kk is lower triangular (zeros above the diagonal)
-    double s = 0.0;
+    double sp[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int c = 0; c < CH_NB; ++c) s += Xs[r * (CH_NB + 1) + c] * t[c];
+    for (int c = 0; c < CH_NB; ++c) sp[c & 3] += Xs[r * (CH_NB + 1) + c] * t[c];
+    const double s = (sp[0] + sp[1]) + (sp[2] + sp[3]);
     if (r < nb) y[k0 + r] = s;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __builtin_amdgcn_wave_barrier();
@@ -1006,7 +1009,7 @@ __global__ void __launch_bounds__(256) trsv_bwd_lookback(const double* __restric
   const double yk = (wid == 0 && r < nb) ? y[k0 + r] : 0.0;
   // columns past nb re-read the last column; their sums are never used
   const int64_t col = k0 + min(r, nb - 1);
-  double acc = 0.0;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};   // interleaved partial sums, as in the forward solve
   for (int jb = nblk - 1 - wid; jb > kb; jb -= 4) {
     const int64_t j0 = (int64_t)jb * CH_NB;
     const int mb = (int)min<int64_t>(CH_NB, n - j0);
@@ -1021,18 +1024,19 @@ __global__ void __launch_bounds__(256) trsv_bwd_lookback(const double* __restric
     xs[wid][r] = (r < mb) ? x[j0 + min(r, mb - 1)] : 0.0;
     wave_lds_sync();
 #pragma unroll
-    for (int c = 0; c < CH_NB; ++c) acc += Lcol[c] * xs[wid][c];
+    for (int c = 0; c < CH_NB; ++c) acc[c & 3] += Lcol[c] * xs[wid][c];
     wave_lds_sync();
   }
-  part[wid][r] = acc;
+  part[wid][r] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   __syncthreads();
   if (wid == 0) {
     t[r] = (r < nb) ? yk - (part[0][r] + part[1][r] + part[2][r] + part[3][r]) : 0.0;
     wave_lds_sync();
     // column r of inv_kk (zeros above the diagonal)
-    double s = 0.0;
+    double sp[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int c = 0; c < CH_NB; ++c) s += Xs[c * (CH_NB + 1) + r] * t[c];
+    for (int c = 0; c < CH_NB; ++c) sp[c & 3] += Xs[c * (CH_NB + 1) + r] * t[c];
+    const double s = (sp[0] + sp[1]) + (sp[2] + sp[3]);
     if (r < nb) x[k0 + r] = s;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __builtin_amdgcn_wave_barrier();
