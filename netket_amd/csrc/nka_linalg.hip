@@ -764,17 +764,15 @@ __global__ void __launch_bounds__(256) panel_factor_kernel(double* __restrict__ 
 // Rows and columns past the edge re-read the last valid one and are not stored.
 constexpr int PU_LD = 33;  // doubles per row of a 64x32 LDS half tile
 
-__global__ void __launch_bounds__(256) panel_update_kernel(double* __restrict__ A, int64_t lda, int64_t k0,
-                                                           int64_t rest, int64_t cols, int64_t tiles_n,
-                                                           int64_t n_tiles, const int32_t* __restrict__ info) {
-  if (*info != 0) return;
-  __shared__ double sA[CH_NB * PU_LD];
-  __shared__ double sB[CH_NB * PU_LD];
-  const int64_t bid = xcd_remap(blockIdx.x, gridDim.x);
-  if (bid >= n_tiles) return;
-  const int64_t i0 = (bid / tiles_n) * 64, j0 = (bid % tiles_n) * 64;
-  const double* P = A + (k0 + CH_NB) * lda + k0;
-  double* C = A + (k0 + CH_NB) * lda + k0 + CH_NB;
+// One 64x64 tile of C - Pa Pb^T with K = 64: rows i0.. of Pa (clamped to ra
+// rows) against rows j0.. of Pb (clamped to rb rows) as the columns, C read at
+// C[(i0 + r) * lda + j0 + c] with the same clamps.  out[x][y][e] is element
+// (wm + 16x + l/16 + 4e, wn + 16y + l%16) of the tile, wave (wm, wn) of the
+// 2 x 2 wave grid.  All threads of the workgroup call it.
+__device__ __forceinline__ void pu_tile(const double* __restrict__ Pa, const double* __restrict__ Pb,
+                                        const double* __restrict__ C, int64_t lda, int64_t i0, int64_t j0,
+                                        int64_t ra, int64_t rb, double* sA, double* sB,
+                                        double out[2][2][4]) {
   const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
   const int wm = (wid >> 1) * 32, wn = (wid & 1) * 32;
   // operand loads: wave w reads rows w, w+4, ... with lanes along k
@@ -782,8 +780,8 @@ __global__ void __launch_bounds__(256) panel_update_kernel(double* __restrict__ 
 #pragma unroll
   for (int it = 0; it < 16; ++it) {
     const int r = wid + 4 * it;
-    va[it] = P[min<int64_t>(i0 + r, rest - 1) * lda + lane];
-    vb[it] = P[min<int64_t>(j0 + r, cols - 1) * lda + lane];
+    va[it] = Pa[min<int64_t>(i0 + r, ra - 1) * lda + lane];
+    vb[it] = Pb[min<int64_t>(j0 + r, rb - 1) * lda + lane];
   }
   // C in the MFMA output layout: element e of lane l in a 16x16 block is
   // (row l/16 + 4e, col l%16)
@@ -793,8 +791,8 @@ __global__ void __launch_bounds__(256) panel_update_kernel(double* __restrict__ 
     for (int y = 0; y < 2; ++y)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int64_t gi = min<int64_t>(i0 + wm + x * 16 + (lane >> 4) + 4 * e, rest - 1);
-        const int64_t gj = min<int64_t>(j0 + wn + y * 16 + (lane & 15), cols - 1);
+        const int64_t gi = min<int64_t>(i0 + wm + x * 16 + (lane >> 4) + 4 * e, ra - 1);
+        const int64_t gj = min<int64_t>(j0 + wn + y * 16 + (lane & 15), rb - 1);
         vc[x][y][e] = C[gi * lda + gj];
       }
   doublex4 acc[2][2];
@@ -802,6 +800,7 @@ __global__ void __launch_bounds__(256) panel_update_kernel(double* __restrict__ 
   for (int x = 0; x < 2; ++x)
 #pragma unroll
     for (int y = 0; y < 2; ++y) acc[x][y] = (doublex4){0.0, 0.0, 0.0, 0.0};
+  __syncthreads();  // the staging buffers may still be read by an earlier tile
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     if (h) __syncthreads();
@@ -834,11 +833,111 @@ __global__ void __launch_bounds__(256) panel_update_kernel(double* __restrict__ 
 #pragma unroll
     for (int y = 0; y < 2; ++y)
 #pragma unroll
+      for (int e = 0; e < 4; ++e) out[x][y][e] = vc[x][y][e] - acc[x][y][e];
+}
+
+// C[i][j] -= sum_{k<64} Pa[i][k] Pb[j][k] over a grid of 64x64 tiles, stored
+// for i < rest, j < cols.
+__device__ __forceinline__ void pu_store_tile(const double* Pa, const double* Pb, double* C, int64_t lda,
+                                              int64_t tile, int64_t tiles_n, int64_t rest, int64_t cols,
+                                              double* sA, double* sB) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int wm = (wid >> 1) * 32, wn = (wid & 1) * 32;
+  const int64_t i0 = (tile / tiles_n) * 64, j0 = (tile % tiles_n) * 64;
+  double out[2][2][4];
+  pu_tile(Pa, Pb, C, lda, i0, j0, rest, cols, sA, sB, out);
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int64_t gi = i0 + wm + x * 16 + (lane >> 4) + 4 * e;
         const int64_t gj = j0 + wn + y * 16 + (lane & 15);
-        if (gi < rest && gj < cols) C[gi * lda + gj] = vc[x][y][e] - acc[x][y][e];
+        if (gi < rest && gj < cols) C[gi * lda + gj] = out[x][y][e];
       }
+}
+
+__global__ void __launch_bounds__(256) panel_update_kernel(double* __restrict__ A, int64_t lda, int64_t k0,
+                                                           int64_t rest, int64_t cols, int64_t tiles_n,
+                                                           int64_t n_tiles, const int32_t* __restrict__ info) {
+  if (*info != 0) return;
+  __shared__ double sA[CH_NB * PU_LD];
+  __shared__ double sB[CH_NB * PU_LD];
+  const int64_t bid = xcd_remap(blockIdx.x, gridDim.x);
+  if (bid >= n_tiles) return;
+  const double* P = A + (k0 + CH_NB) * lda + k0;
+  double* C = A + (k0 + CH_NB) * lda + k0 + CH_NB;
+  pu_store_tile(P, P, C, lda, bid, tiles_n, rest, cols, sA, sB);
+}
+
+// One 64-column step k0 of a panel after its first, in one launch with two
+// roles.  Workgroups [0, n_factor) work as panel_factor_kernel, except that
+// the previous step's rank-64 update of this step's columns is applied here,
+// to the diagonal block and to each workgroup's own 64 rows (the operands
+// and C tiles go through the same tile code as the panel update), before the
+// factorisation.  The other workgroups apply the previous step's update to
+// the panel's columns right of this step (panel_update_kernel's tiles, shifted
+// by 64 columns).  The roles write disjoint columns and read only what earlier
+// launches wrote, so the panel update no longer sits between two steps.
+__global__ void __launch_bounds__(256) panel_step_kernel(double* __restrict__ A, int64_t lda, int64_t n,
+                                                         int64_t k0, int64_t n_factor, int64_t cols_b,
+                                                         int64_t tiles_n_b, double* __restrict__ slot,
+                                                         int32_t* __restrict__ info) {
+  if (*info != 0) return;
+  __shared__ double Ls[CH_NB * TS_LD];           // the diagonal block, then L11 (row-major)
+  __shared__ double colbuf[CH_NB];
+  __shared__ double rdiag[CH_NB];
+  __shared__ double Xs[4][PF_ROWS * TS_LD];      // 16 rows per wavefront
+  double* sA = Ls;                               // staging of the update tiles (before Ls is filled)
+  double* sB = &Xs[0][0];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int64_t kp = k0 - CH_NB;                 // the previous step
+  const double* P = A + k0 * lda + kp;           // rows k0.. of the previous step's columns
+  const int64_t rest_p = n - k0;                 // rows of P
+  if ((int64_t)blockIdx.x >= n_factor) {
+    const int64_t tile = (int64_t)blockIdx.x - n_factor;
+    // columns k0 + 64 .. of the panel: B operand rows k0 + 64 ..
+    pu_store_tile(P, P + CH_NB * lda, A + k0 * lda + k0 + CH_NB, lda, tile, tiles_n_b, rest_p, cols_b, sA, sB);
+    return;
+  }
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  double dg[2][2][4], ow[2][2][4];
+  // diagonal block (rows and columns k0 .. k0+63) and this workgroup's rows
+  pu_tile(P, P, A + k0 * lda + k0, lda, 0, 0, rest_p, CH_NB, sA, sB, dg);
+  const int64_t own0 = CH_NB + (int64_t)blockIdx.x * (4 * PF_ROWS);   // relative to k0
+  pu_tile(P, P, A + k0 * lda + k0, lda, own0, 0, rest_p, CH_NB, sA, sB, ow);
+  __syncthreads();  // staging done: Ls and Xs take the updated tiles
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int R = wm + x * 16 + (lane >> 4) + 4 * e, c = wn + y * 16 + (lane & 15);
+        Ls[R * TS_LD + c] = c <= R ? dg[x][y][e] : 0.0;
+        Xs[R >> 4][(R & 15) * TS_LD + c] = ow[x][y][e];
+      }
+  __syncthreads();
+  double* X = Xs[w];
+  const int64_t r0 = k0 + CH_NB + (int64_t)blockIdx.x * (4 * PF_ROWS) + PF_ROWS * w;
+  if (w == 0) {
+    const int bad = factor_block_wave(Ls, colbuf, rdiag, lane, k0);
+    if (blockIdx.x == 0 && lane == 0 && bad) atomicCAS(info, 0, bad);
+  }
+  __syncthreads();
+  if (blockIdx.x == 0) {
+#pragma unroll
+    for (int r = 0; r < PF_ROWS; ++r) {
+      const int R = PF_ROWS * w + r;
+      slot[R * CH_NB + lane] = Ls[R * TS_LD + lane];
+    }
+  }
+  solve_rows16_wave(Ls, rdiag, X, lane);
+#pragma unroll
+  for (int r = 0; r < PF_ROWS; ++r) {
+    if (r0 + r < n) A[(r0 + r) * lda + k0 + lane] = X[r * TS_LD + lane];
+  }
 }
 
 // inv_b = L_bb^{-1} for every 64x64 diagonal block b of the factor; one
@@ -1428,23 +1527,37 @@ int nka_potrf_f64(double* a, int64_t n, int64_t lda, void* workspace, int32_t* i
     const int64_t n0 = std::min<int64_t>(NB0, n - K0);
     for (int64_t K1 = K0; K1 < K0 + n0; K1 += NB1) {
       const int64_t n1 = std::min<int64_t>(NB1, K0 + n0 - K1);
+      // 64-column steps.  The first step of the panel factors its columns as
+      // the trailing updates left them.  A later step runs as one two-role
+      // launch: the previous step's update of this step's columns fused into
+      // the factorisation, and its update of the columns further right beside
+      // it.  The last block of the matrix (no rows below) takes the separate
+      // update and the one-wavefront factor.
       for (int64_t k0 = K1; k0 < K1 + n1; k0 += CH_NB) {
         const int64_t nb = std::min<int64_t>(CH_NB, n - k0);
         const int64_t rest = n - k0 - nb;
         double* slot = inv + (k0 / CH_NB) * CH_NB * CH_NB;
         if (rest <= 0) {
+          if (k0 > K1) {
+            const int64_t kp = k0 - CH_NB, rest_p = n - k0, cols_p = K1 + n1 - k0;
+            const int64_t tm = (rest_p + 63) / 64, tn = (cols_p + 63) / 64;
+            hipLaunchKernelGGL(panel_update_kernel, dim3(grid_mult8(tm * tn)), dim3(256), 0, s, a, lda, kp,
+                               rest_p, cols_p, tn, tm * tn, info);
+          }
           hipLaunchKernelGGL(potf2_kernel, dim3(1), dim3(64), 0, s, a, lda, n, k0, slot, info);
           continue;
         }
         // rows below imply nb == 64
         const unsigned grid = (unsigned)((rest + 4 * PF_ROWS - 1) / (4 * PF_ROWS));
-        hipLaunchKernelGGL(panel_factor_kernel, dim3(grid), dim3(256), 0, s, a, lda, n, k0, slot, info);
-        // update of the remaining columns of this 256-column panel
-        const int64_t cols = K1 + n1 - (k0 + nb);
-        if (cols > 0) {
-          const int64_t tm = (rest + 63) / 64, tn = (cols + 63) / 64;
-          hipLaunchKernelGGL(panel_update_kernel, dim3(grid_mult8(tm * tn)), dim3(256), 0, s, a, lda, k0,
-                             rest, cols, tn, tm * tn, info);
+        if (k0 == K1) {
+          hipLaunchKernelGGL(panel_factor_kernel, dim3(grid), dim3(256), 0, s, a, lda, n, k0, slot, info);
+        } else {
+          // the previous step's update of the columns right of this step
+          const int64_t cols_b = K1 + n1 - (k0 + CH_NB);
+          const int64_t tm = (n - k0 + 63) / 64, tn = cols_b > 0 ? (cols_b + 63) / 64 : 0;
+          const int64_t tiles_b = cols_b > 0 ? tm * tn : 0;
+          hipLaunchKernelGGL(panel_step_kernel, dim3((unsigned)(grid + tiles_b)), dim3(256), 0, s, a, lda, n,
+                             k0, (int64_t)grid, cols_b, tn, slot, info);
         }
       }
       // update of the remaining columns of this 1024-column panel (K = 256):
