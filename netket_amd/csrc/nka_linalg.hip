@@ -880,7 +880,7 @@ __global__ void __launch_bounds__(256) panel_update_kernel(double* __restrict__ 
 // the panel's columns right of this step (panel_update_kernel's tiles, shifted
 // by 64 columns).  The roles write disjoint columns and read only what earlier
 // launches wrote, so the panel update no longer sits between two steps.
-__global__ void __launch_bounds__(256) panel_step_kernel(double* __restrict__ A, int64_t lda, int64_t n,
+__global__ void __launch_bounds__(256, 2) panel_step_kernel(double* __restrict__ A, int64_t lda, int64_t n,
                                                          int64_t k0, int64_t n_factor, int64_t cols_b,
                                                          int64_t tiles_n_b, double* __restrict__ slot,
                                                          int32_t* __restrict__ info) {
