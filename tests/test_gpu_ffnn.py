@@ -156,11 +156,13 @@ def test_vmc_sr_update_matches_oracle(gpu, ntk_method):
     np.testing.assert_allclose(out[True], ref, rtol=0, atol=1e-8 * scale)
 
 
-@pytest.mark.parametrize("rule", ["local", "exchange"])
-def test_chains_identical_to_oracle_fp64(gpu, rule):
-    print(f"case ffnn chains {rule}", flush=True)
+@pytest.mark.parametrize("rule,H1,H2,scale", [("local", 9, 7, 0.4), ("exchange", 9, 7, 0.4),
+                                               ("exchange", 420, 30, 0.05)])
+def test_chains_identical_to_oracle_fp64(gpu, rule, H1, H2, scale):
+    # H1 = 420: 53 first-layer units per thread, the 56-unit sampler instance
+    print(f"case ffnn chains {rule} H1={H1}", flush=True)
     N = 10
-    model, flat, p = _model(N, 9, 7, np.float64, 3, 0.4)
+    model, flat, p = _model(N, H1, H2, np.float64, 3, scale)
     g = nk.graph.Chain(N)
     if rule == "local":
         hi = nk.hilbert.Spin(0.5, N=N)

@@ -108,12 +108,12 @@ def test_distribution_matches_exact(gpu, rule, dtype):
         assert np.all((s < 0).sum(axis=1) == N // 2)
 
 
-@pytest.mark.parametrize("alpha", [300, 500])
+@pytest.mark.parametrize("alpha", [300, 430, 500])
 def test_distribution_matches_exact_wide_model(gpu, alpha):
     # FP32 models whose W exceeds the LDS budget take the local-rule kernel
     # that loads the next proposal's W row ahead and uses the exp2 product
     # form of the log-amplitude change (M = 2400: 40 units per lane,
-    # M = 4000: 64 units per lane)
+    # M = 3440: 56, M = 4000: 64)
     print(f"case distribution wide local alpha={alpha}", flush=True)
     N = 8
     model, flat, p = _model(N, alpha, np.float32, 11, 0.05)
