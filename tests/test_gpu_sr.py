@@ -103,6 +103,7 @@ def test_qgt_dense(gpu, dtype):
                                                (np.float32, 300, (True, True), 14),
                                                (np.float32, 129, (False, True), 70),
                                                (np.float32, 260, (True, False), 97),
+                                               (np.float32, 200, (True, True), 150),
                                                (np.float64, 1, (True, True), 14)])
 def test_rbm_ntk_structured(gpu, dtype, ns, biases, N):
     # T = (O - mean)(O - mean)^T / ns + shift I + offset 11^T from the RBM
