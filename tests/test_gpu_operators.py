@@ -155,6 +155,7 @@ def test_local_energy(gpu, kind, N, alpha, dtype):
     ("chain100", 4, 0.1, True, 300),       # M = 400: a partly empty last unit tile
     ("square10", 3, 0.3, True, 200),       # M = 300, large |theta|, 2D edges
     ("chain90", 3, 0.2, False, 131),       # no biases, N not a multiple of 8, ragged sample tile
+    ("square12", 2, 0.1, True, 70),        # 144 sites: five spin words per sample
 ])
 def test_local_energy_ising_tiled(gpu, lattice, alpha, scale, bias, n):
     # FP32 models whose W exceeds 64 KiB take the path tiled over samples and
@@ -162,7 +163,7 @@ def test_local_energy_ising_tiled(gpu, lattice, alpha, scale, bias, n):
     print(f"case eloc ising tiled {lattice} alpha={alpha} bias={bias} n={n}", flush=True)
     import ctypes as C
     from netket_amd import _engine as eng, _native as nat, _ops
-    g = nk.graph.Square(10) if lattice == "square10" else nk.graph.Chain(int(lattice[5:]))
+    g = nk.graph.Square(int(lattice[6:])) if lattice.startswith("square") else nk.graph.Chain(int(lattice[5:]))
     N = g.n_nodes
     op = nk.operator.Ising(nk.hilbert.Spin(0.5, N=N), g, h=0.8, J=0.9, dtype=np.float32)
     model = nk.models.RBM(alpha=alpha, param_dtype=np.float32, use_hidden_bias=bias,
