@@ -31,7 +31,7 @@ BF16_PEAK_TFLOPS = 2500.0
 FP64_PEAK_TFLOPS = 78.6
 
 
-def build_cfg2(seed=0):
+def build_heisenberg_l64(seed=0):
     import numpy as np
 
     import netket_amd as nk
@@ -181,7 +181,7 @@ def step(drv):
 
 
 def accuracy_run(n_steps=3000, n_config=300, window=100):
-    """Config 1 (TFIM L=16, FP64 RBM alpha=1, 16 chains, 1008 samples, 16 discarded
+    """The TFIM L=16 workload (FP64 RBM alpha=1, 16 chains, 1008 samples, 16 discarded
     sweeps, SR diag_shift 0.01 with Cholesky, SGD lr 0.01) against the exact energy:
     the relative error of the mean energy over the last `window` steps at the
     configured n_config steps, and after continuing the same run to n_steps, with
@@ -226,7 +226,7 @@ def accuracy_run(n_steps=3000, n_config=300, window=100):
     }
 
 
-def cpu_baseline_cfg2(budget_s: float = 25.0):
+def cpu_baseline_heisenberg_l64(budget_s: float = 25.0):
     """One CPU step of the reference algorithm (FP64) on the same workload."""
     import numpy as np
 
@@ -363,7 +363,7 @@ def main():
     if args.workload != "heisenberg-l64":
         run_workload(args.workload, args.steps, args.warmup)
         return
-    drv, H, vs = build_cfg2()
+    drv, H, vs = build_heisenberg_l64()
     for _ in range(args.warmup):
         step(drv)
     torch.cuda.synchronize()
@@ -402,7 +402,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cores = len(os.sched_getaffinity(0))
-        rate, n_cpu, tim = cpu_baseline_cfg2(args.cpu_budget)
+        rate, n_cpu, tim = cpu_baseline_heisenberg_l64(args.cpu_budget)
         threads = int(os.environ.get("OMP_NUM_THREADS", cores))
         cpu = {"value": rate, "unit": "SR steps/s", "cores": min(cores, threads), "kind": "port",
                "sample": f"{n_cpu} full step(s) of the same workload (L=64 Heisenberg, 16384 samples, "

@@ -22,7 +22,7 @@ def _tfim_state(L, n_chains, n_samples, n_discard, seed, alpha=1, dtype=np.float
 
 
 def test_vmc_sr_tfim_l16_converges(gpu):
-    # Config 1: TFIM L=16 PBC h=J=1, FP64 RBM alpha=1, MetropolisLocal 16 chains,
+    # TFIM L=16 PBC h=J=1, FP64 RBM alpha=1, MetropolisLocal 16 chains,
     # 1008 samples, 16 discarded sweeps, SR (QGTJacobianDense, diag_shift 0.01,
     # Cholesky), SGD; the learning rate is raised from 0.01 to 0.05 so that 300
     # steps reach the converged regime.
