@@ -465,6 +465,33 @@ __global__ void ffnn_wsum_kernel(const double* __restrict__ w, int64_t ns, doubl
 // ---------------------------------------------------------------------------
 constexpr int NF_T = 64, NF_K = 16, NF_LD = 17;
 
+// One K slab (64 rows x NF_K) of both operands of an NTK tile into
+// registers: thread tid holds elements tid + 256 it (row idx / 16, k idx % 16).
+// Spins (pass over x) become +-1; entries past the edges are zero.
+__device__ __forceinline__ void ntk_fetch(const double* __restrict__ src, const int8_t* __restrict__ x, int N,
+                                          int K, int64_t ns, int64_t i0, int64_t j0, int k0, bool spins,
+                                          double (&pa)[4], double (&pb)[4]) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int it = 0; it < 4; ++it) {
+    const int idx = tid + it * 256;
+    const int r = idx >> 4, kc = idx & 15;
+    const int kk = k0 + kc;
+    double va = 0.0, vb = 0.0;
+    if (kk < K) {
+      if (spins) {
+        if (i0 + r < ns) va = x[(i0 + r) * N + kk] > 0 ? 1.0 : -1.0;
+        if (j0 + r < ns) vb = x[(j0 + r) * N + kk] > 0 ? 1.0 : -1.0;
+      } else {
+        if (i0 + r < ns) va = src[(i0 + r) * K + kk];
+        if (j0 + r < ns) vb = src[(j0 + r) * K + kk];
+      }
+    }
+    pa[it] = va;
+    pb[it] = vb;
+  }
+}
+
 __global__ void __launch_bounds__(256) ffnn_ntk_kernel(const double* __restrict__ h1, const double* __restrict__ g2,
                                                        const int8_t* __restrict__ x, const double* __restrict__ d1,
                                                        int N, int H1, int H2, int64_t ns,
@@ -493,27 +520,20 @@ __global__ void __launch_bounds__(256) ffnn_ntk_kernel(const double* __restrict_
   for (int pass = 0; pass < 4; ++pass) {
     const int K = pass == 0 ? H1 : pass == 1 ? H2 : pass == 2 ? N : H1;
     const double* src = pass == 0 ? h1 : pass == 1 ? g2 : pass == 2 ? nullptr : d1;
+    // the next K slab is loaded into registers while the current one is used
+    double pa[4], pb[4];
+    ntk_fetch(src, x, N, K, ns, i0, j0, 0, pass == 2, pa, pb);
     for (int k0 = 0; k0 < K; k0 += NF_K) {
       __syncthreads();
 #pragma unroll
       for (int it = 0; it < 4; ++it) {
         const int idx = tid + it * 256;
         const int r = idx >> 4, kc = idx & 15;
-        const int kk = k0 + kc;
-        double va = 0.0, vb = 0.0;
-        if (kk < K) {
-          if (pass == 2) {
-            if (i0 + r < ns) va = x[(i0 + r) * N + kk] > 0 ? 1.0 : -1.0;
-            if (j0 + r < ns) vb = x[(j0 + r) * N + kk] > 0 ? 1.0 : -1.0;
-          } else {
-            if (i0 + r < ns) va = src[(i0 + r) * K + kk];
-            if (j0 + r < ns) vb = src[(j0 + r) * K + kk];
-          }
-        }
-        sA[r * NF_LD + kc] = va;
-        sB[r * NF_LD + kc] = vb;
+        sA[r * NF_LD + kc] = pa[it];
+        sB[r * NF_LD + kc] = pb[it];
       }
       __syncthreads();
+      if (k0 + NF_K < K) ntk_fetch(src, x, N, K, ns, i0, j0, k0 + NF_K, pass == 2, pa, pb);
 #pragma unroll
       for (int ks = 0; ks < NF_K; ks += 4) {
         const int kq = ks + (lane >> 4);
