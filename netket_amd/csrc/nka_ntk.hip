@@ -127,30 +127,44 @@ __global__ void __launch_bounds__(256) rbm_ntk_kernel(const double* __restrict__
       th[x][y] = (doublex4){0.0, 0.0, 0.0, 0.0};
       sg[x][y] = (doublex4){0.0, 0.0, 0.0, 0.0};
     }
-  // pass 0: Theta = t t^T (K = M); pass 1: Sigma = sigma sigma^T (K = N)
+  // pass 0: Theta = t t^T (K = M); pass 1: Sigma = sigma sigma^T (K = N); the
+  // next K slab is loaded into registers while the current one is used
+  auto fetch = [&](int pass, int K, int k0, double* pa, double* pb) {
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+      const int idx = tid + it * 256;
+      const int r = idx >> 4, kc = idx & 15;
+      const int kk = k0 + kc;
+      double va = 0.0, vb = 0.0;
+      if (kk < K) {
+        if (pass == 0) {
+          if (i0 + r < ns) va = t64[(i0 + r) * M + kk];
+          if (j0 + r < ns) vb = t64[(j0 + r) * M + kk];
+        } else {
+          if (i0 + r < ns) va = sigma[(i0 + r) * N + kk] > 0 ? 1.0 : -1.0;
+          if (j0 + r < ns) vb = sigma[(j0 + r) * N + kk] > 0 ? 1.0 : -1.0;
+        }
+      }
+      pa[it] = va;
+      pb[it] = vb;
+    }
+  };
+#pragma unroll
   for (int pass = 0; pass < 2; ++pass) {
     const int K = pass == 0 ? M : N;
+    double pa[4], pb[4];
+    fetch(pass, K, 0, pa, pb);
     for (int k0 = 0; k0 < K; k0 += NT_K) {
       __syncthreads();
 #pragma unroll
       for (int it = 0; it < 4; ++it) {
         const int idx = tid + it * 256;
         const int r = idx >> 4, kc = idx & 15;
-        const int kk = k0 + kc;
-        double va = 0.0, vb = 0.0;
-        if (kk < K) {
-          if (pass == 0) {
-            if (i0 + r < ns) va = t64[(i0 + r) * M + kk];
-            if (j0 + r < ns) vb = t64[(j0 + r) * M + kk];
-          } else {
-            if (i0 + r < ns) va = sigma[(i0 + r) * N + kk] > 0 ? 1.0 : -1.0;
-            if (j0 + r < ns) vb = sigma[(j0 + r) * N + kk] > 0 ? 1.0 : -1.0;
-          }
-        }
-        sA[r * NT_LD + kc] = va;
-        sB[r * NT_LD + kc] = vb;
+        sA[r * NT_LD + kc] = pa[it];
+        sB[r * NT_LD + kc] = pb[it];
       }
       __syncthreads();
+      if (k0 + NT_K < K) fetch(pass, K, k0 + NT_K, pa, pb);
 #pragma unroll
       for (int ks = 0; ks < NT_K; ks += 4) {
         const int kq = ks + (lane >> 4);
