@@ -495,15 +495,33 @@ __global__ void __launch_bounds__(256) gemm_big_reduce_kernel(const double* __re
     tj = (int)(tile % tiles_n);
   }
   const double* w = work + r * S * (int64_t)(G2_T * G2_T);
-  for (int idx = threadIdx.x; idx < 32 * G2_T; idx += 256) {
+  // 16 elements per thread, the loads of a slice (and of C) all in flight;
+  // each element sums its slices in slice order
+  constexpr int PER = 32 * G2_T / 256;
+  double acc[PER], cv[PER];
+  bool keep[PER];
+  int64_t off[PER];
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    const int idx = threadIdx.x + e * 256;
     const int li = band * 32 + idx / G2_T, lj = idx % G2_T;
     const int64_t gi = (int64_t)ti * G2_T + li, gj = (int64_t)tj * G2_T + lj;
-    if (gi < m && gj < n && (!lower_only || gj <= gi)) {
-      double acc = 0.0;
-      for (int p = 0; p < S; ++p) acc += w[(int64_t)p * G2_T * G2_T + li * G2_T + lj];
-      double v = alpha * acc;
-      if (beta != 0.0) v += beta * c[gi * ldc + gj];
-      c[gi * ldc + gj] = v;
+    keep[e] = gi < m && gj < n && (!lower_only || gj <= gi);
+    off[e] = keep[e] ? gi * ldc + gj : 0;
+    acc[e] = 0.0;
+    cv[e] = (beta != 0.0 && keep[e]) ? c[off[e]] : 0.0;
+  }
+  for (int p = 0; p < S; ++p) {
+    const double* wp = w + (int64_t)p * G2_T * G2_T + band * 32 * G2_T;
+#pragma unroll
+    for (int e = 0; e < PER; ++e) acc[e] += wp[threadIdx.x + e * 256];
+  }
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    if (keep[e]) {
+      double v = alpha * acc[e];
+      if (beta != 0.0) v += beta * cv[e];
+      c[off[e]] = v;
     }
   }
 }
