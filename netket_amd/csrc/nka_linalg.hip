@@ -854,6 +854,87 @@ __device__ __forceinline__ void pu_tile(const double* __restrict__ Pa, const dou
       for (int e = 0; e < 4; ++e) out[x][y][e] = vc[x][y][e] - acc[x][y][e];
 }
 
+// Two tiles of C - P P^T with K = 64 and one set of loads: the diagonal tile
+// (rows and columns 0..63, both operands P rows 0..63, which must all be
+// valid rows) and the tile of rows i0.. (P rows i0.. clamped to ra rows,
+// against P rows 0..63).  Both share the B operand, so the loads of the two
+// tiles are in flight together and one staging pass feeds both products.
+// dg and ow are laid out as pu_tile's out; the sums run in pu_tile's order,
+// so the results equal two pu_tile calls.  All threads of the workgroup call it.
+__device__ __forceinline__ void pu_tile_pair(const double* __restrict__ P, const double* __restrict__ C,
+                                             int64_t lda, int64_t i0, int64_t ra, double* sA, double* sB,
+                                             double dg[2][2][4], double ow[2][2][4]) {
+  const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
+  const int wm = (wid >> 1) * 32, wn = (wid & 1) * 32;
+  double va[16], vb[16], cd[2][2][4], co[2][2][4];
+#pragma unroll
+  for (int it = 0; it < 16; ++it) {
+    const int r = wid + 4 * it;
+    va[it] = P[min<int64_t>(i0 + r, ra - 1) * lda + lane];
+    vb[it] = P[(int64_t)r * lda + lane];
+  }
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int rr = wm + x * 16 + (lane >> 4) + 4 * e, cc = wn + y * 16 + (lane & 15);
+        cd[x][y][e] = C[(int64_t)rr * lda + cc];
+        co[x][y][e] = C[min<int64_t>(i0 + rr, ra - 1) * lda + cc];
+      }
+  doublex4 ad[2][2], ao[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) {
+      ad[x][y] = (doublex4){0.0, 0.0, 0.0, 0.0};
+      ao[x][y] = (doublex4){0.0, 0.0, 0.0, 0.0};
+    }
+  __syncthreads();  // the staging buffers may still be read by earlier code
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (h) __syncthreads();
+    if ((lane >> 5) == h) {
+#pragma unroll
+      for (int it = 0; it < 16; ++it) {
+        const int r = wid + 4 * it;
+        sA[r * PU_LD + (lane & 31)] = va[it];
+        sB[r * PU_LD + (lane & 31)] = vb[it];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < 32; ks += 4) {
+      const int kq = ks + (lane >> 4);
+      double avd[2], avo[2], bv[2];
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        avd[x] = sB[(wm + x * 16 + (lane & 15)) * PU_LD + kq];
+        avo[x] = sA[(wm + x * 16 + (lane & 15)) * PU_LD + kq];
+      }
+#pragma unroll
+      for (int y = 0; y < 2; ++y) bv[y] = sB[(wn + y * 16 + (lane & 15)) * PU_LD + kq];
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+          ad[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(avd[x], bv[y], ad[x][y], 0, 0, 0);
+          ao[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(avo[x], bv[y], ao[x][y], 0, 0, 0);
+        }
+    }
+  }
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        dg[x][y][e] = cd[x][y][e] - ad[x][y][e];
+        ow[x][y][e] = co[x][y][e] - ao[x][y][e];
+      }
+}
+
 // C[i][j] -= sum_{k<64} Pa[i][k] Pb[j][k] over a grid of 64x64 tiles, stored
 // for i < rest, j < cols.
 __device__ __forceinline__ void pu_store_tile(const double* Pa, const double* Pb, double* C, int64_t lda,
@@ -922,9 +1003,9 @@ __global__ void __launch_bounds__(256, 2) panel_step_kernel(double* __restrict__
   const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
   double dg[2][2][4], ow[2][2][4];
   // diagonal block (rows and columns k0 .. k0+63) and this workgroup's rows
-  pu_tile(P, P, A + k0 * lda + k0, lda, 0, 0, rest_p, CH_NB, sA, sB, dg);
+  // (the launch has rows below this step, so rest_p > 64: P rows 0..63 are valid)
   const int64_t own0 = CH_NB + (int64_t)blockIdx.x * (4 * PF_ROWS);   // relative to k0
-  pu_tile(P, P, A + k0 * lda + k0, lda, own0, 0, rest_p, CH_NB, sA, sB, ow);
+  pu_tile_pair(P, A + k0 * lda + k0, lda, own0, rest_p, sA, sB, dg, ow);
   __syncthreads();  // staging done: Ls and Xs take the updated tiles
 #pragma unroll
   for (int x = 0; x < 2; ++x)
