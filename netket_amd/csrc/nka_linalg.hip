@@ -574,13 +574,20 @@ __device__ __forceinline__ int factor_block_wave(double* Ab, double* colbuf, dou
     double x[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) x[q] = Ab[lane * TS_LD + 16 * p + q];
+    double ajj_next = readlane_f64(x[0], 16 * p);
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       const int jj = 16 * p + j;
-      // the pivot by v_readlane (on the critical path), the rest of the
-      // column through LDS while the reciprocal square root is formed
+      // the pivot comes from v_readlane values (the critical path), the
+      // rest of the column through LDS while the reciprocal square root is
+      // formed
       colbuf[lane] = x[j];
-      const double ajj = readlane_f64(x[j], jj);
+      const double ajj = ajj_next;
+      double a1 = 0.0, b1 = 0.0;
+      if (j < 15) {
+        a1 = readlane_f64(x[j], jj + 1);
+        b1 = readlane_f64(x[j + 1], jj + 1);
+      }
       wave_lds_sync();
       double cj[16];
 #pragma unroll
@@ -589,6 +596,10 @@ __device__ __forceinline__ int factor_block_wave(double* Ab, double* colbuf, dou
       if (!ok && bad == 0) bad = (int)(k0 + jj + 1);
       const double piv = ok ? ajj : 1.0;
       const double y = rsqrt_refined(piv);
+      // the next pivot: lane jj + 1 forms x[j+1] - (x[j] y^2) x[j] below with
+      // these same operations (one rounding for the multiply-subtract), so
+      // the value is the one that lane holds after this step
+      if (j < 15) ajj_next = fma(-(a1 * (y * y)), a1, b1);
       const double sij = (lane > jj) ? x[j] * (y * y) : 0.0;
       x[j] = (lane > jj) ? x[j] * y : (lane == jj ? piv * y : 0.0);
       if (lane == jj) yv[jj] = y;
