@@ -1103,30 +1103,51 @@ __global__ void zero_u32_kernel(uint32_t* __restrict__ p, int64_t n) {
 
 // ---------------------------------------------------------------------------
 // Triangular solves with the blocked factor, one launch each.  Workgroup k
-// owns the 64-row block k; it waits (bounded spin on a flag in global memory)
-// for every block it depends on, and only ever on blocks with a smaller
-// workgroup id, which the hardware dispatched earlier, so the wait always
-// ends.  A wait that exceeds the spin bound sets *err and the workgroup
-// proceeds (the result is then invalid and reported as such by the caller).
+// owns the 64-row block k; it waits (bounded spin in global memory) for every
+// block it depends on, and only ever on blocks with a smaller workgroup id,
+// which the hardware dispatched earlier, so the wait always ends.  A wait
+// that exceeds the spin bound sets *err and the workgroup proceeds (the
+// result is then invalid and reported as such by the caller).
+//
+// The solved entries are their own flags: the output vector is filled with
+// kUnsolved before the launch, each entry is written once, as a 64-bit
+// atomic store, and a reader polls the entries it needs with 64-bit atomic
+// loads until none holds kUnsolved.  One memory round trip per dependency,
+// and no ordering between different entries is needed.  kUnsolved is a
+// signalling NaN, which arithmetic never returns; a value with its bit
+// pattern would be published as the default NaN all the same.
 // ---------------------------------------------------------------------------
 constexpr int64_t kSpinLimit = 1LL << 26;
+constexpr uint64_t kUnsolved = 0x7FF4D15EA5E0F00DULL;
+constexpr uint64_t kDefaultNaN = 0x7FF8000000000000ULL;
 
-// Polls with relaxed agent-scope loads (they read the L2 copy without
-// invalidating the CU's L1, which other waves on the CU are streaming
-// through); the caller issues one acquire fence after the flag is seen.
-__device__ inline void wait_flag(const uint32_t* flag, uint32_t want, int32_t* err) {
+// The entry *p (lanes with active), polled with relaxed agent-scope loads
+// (they read the L2 copy); inactive lanes get 0.  Wave-uniform loop.
+__device__ inline double poll_entry(const double* p, bool active, int32_t* err) {
+  uint64_t u = 0;
   int64_t spins = 0;
-  while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != want) {
+  while (true) {
+    if (active) u = __hip_atomic_load((const uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!__any(active && u == kUnsolved)) break;
     __builtin_amdgcn_s_sleep(1);
     if (++spins > kSpinLimit) {
-      atomicOr(err, 1);
+      if (lane_id() == 0) atomicOr(err, 1);
       break;
     }
   }
+  return active ? __longlong_as_double((long long)u) : 0.0;
 }
 
-__device__ inline void set_flag(uint32_t* flag, uint32_t v) {
-  __hip_atomic_store(flag, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ inline void publish_entry(double* p, double v) {
+  uint64_t u = (uint64_t)__double_as_longlong(v);
+  if (u == kUnsolved) u = kDefaultNaN;
+  __hip_atomic_store((uint64_t*)p, u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void fill_unsolved_kernel(uint64_t* __restrict__ p, int64_t n) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * blockDim.x)
+    p[k] = kUnsolved;
 }
 
 // forward: y_k = inv_kk (b_k - sum_{j<k} L_kj y_j); block kb = blockIdx.x.
@@ -1136,8 +1157,7 @@ __device__ inline void set_flag(uint32_t* flag, uint32_t v) {
 __global__ void __launch_bounds__(256) trsv_fwd_lookback(const double* __restrict__ A, int64_t lda, int64_t n,
                                                          const double* __restrict__ inv,
                                                          const double* __restrict__ b, double* y,
-                                                         uint32_t* flags, const int32_t* __restrict__ info,
-                                                         int32_t* err) {
+                                                         const int32_t* __restrict__ info, int32_t* err) {
   if (*info != 0) return;
   __shared__ double part[4][CH_NB];
   __shared__ double t[CH_NB];
@@ -1164,11 +1184,8 @@ __global__ void __launch_bounds__(256) trsv_fwd_lookback(const double* __restric
     double Lrow[CH_NB];
 #pragma unroll
     for (int c = 0; c < CH_NB; ++c) Lrow[c] = arow[j0 + c];
-    if (r == 0) wait_flag(flags + jb, 1u, err);
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     // block jb < kb is a full block: y[j0 + lane] is in bounds
-    xs[wid][r] = y[j0 + r];
+    xs[wid][r] = poll_entry(y + j0 + r, true, err);
     wave_lds_sync();
 #pragma unroll
     for (int c = 0; c < CH_NB; ++c) acc[c & 3] += Lrow[c] * xs[wid][c];
@@ -1184,10 +1201,7 @@ __global__ void __launch_bounds__(256) trsv_fwd_lookback(const double* __restric
 #pragma unroll
     for (int c = 0; c < CH_NB; ++c) sp[c & 3] += Xs[r * (CH_NB + 1) + c] * t[c];
     const double s = (sp[0] + sp[1]) + (sp[2] + sp[3]);
-    if (r < nb) y[k0 + r] = s;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __builtin_amdgcn_wave_barrier();
-    if (r == 0) set_flag(flags + kb, 1u);
+    if (r < nb) publish_entry(y + k0 + r, s);
   }
 }
 
@@ -1197,8 +1211,7 @@ __global__ void __launch_bounds__(256) trsv_fwd_lookback(const double* __restric
 __global__ void __launch_bounds__(256) trsv_bwd_lookback(const double* __restrict__ A, int64_t lda, int64_t n,
                                                          const double* __restrict__ inv,
                                                          const double* __restrict__ y, double* x,
-                                                         uint32_t* flags, const int32_t* __restrict__ info,
-                                                         int32_t* err) {
+                                                         const int32_t* __restrict__ info, int32_t* err) {
   if (*info != 0) return;
   __shared__ double part[4][CH_NB];
   __shared__ double t[CH_NB];
@@ -1227,10 +1240,7 @@ __global__ void __launch_bounds__(256) trsv_bwd_lookback(const double* __restric
     double Lcol[CH_NB];
 #pragma unroll
     for (int c = 0; c < CH_NB; ++c) Lcol[c] = A[(j0 + min(c, mb - 1)) * lda + col];
-    if (r == 0) wait_flag(flags + (nblk - 1 - jb), 1u, err);
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    xs[wid][r] = (r < mb) ? x[j0 + min(r, mb - 1)] : 0.0;
+    xs[wid][r] = poll_entry(x + j0 + r, r < mb, err);
     wave_lds_sync();
 #pragma unroll
     for (int c = 0; c < CH_NB; ++c) acc[c & 3] += Lcol[c] * xs[wid][c];
@@ -1246,10 +1256,7 @@ __global__ void __launch_bounds__(256) trsv_bwd_lookback(const double* __restric
 #pragma unroll
     for (int c = 0; c < CH_NB; ++c) sp[c & 3] += Xs[c * (CH_NB + 1) + r] * t[c];
     const double s = (sp[0] + sp[1]) + (sp[2] + sp[3]);
-    if (r < nb) x[k0 + r] = s;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __builtin_amdgcn_wave_barrier();
-    if (r == 0) set_flag(flags + blockIdx.x, 1u);
+    if (r < nb) publish_entry(x + k0 + r, s);
   }
 }
 
@@ -1708,13 +1715,15 @@ int nka_potrs_f64(const double* a, int64_t n, int64_t lda, const void* workspace
   double* y = ws_vec(ws, nblk);
   uint32_t* flags = ws_flags(ws, n, nblk);
   int32_t* err = (int32_t*)(flags + nblk);
-  const unsigned gz = (unsigned)std::min<int64_t>((nblk + 1 + 255) / 256, 64);
-  hipLaunchKernelGGL(zero_u32_kernel, dim3(gz), dim3(256), 0, s, flags, nblk + 1);
+  const unsigned gf = (unsigned)std::min<int64_t>((n + 255) / 256, 256);
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, (uint32_t*)err, (int64_t)1);
+  hipLaunchKernelGGL(fill_unsolved_kernel, dim3(gf), dim3(256), 0, s, (uint64_t*)y, n);
   hipLaunchKernelGGL(trsv_fwd_lookback, dim3((unsigned)nblk), dim3(256), 0, s, a, lda, n, inv, b, y,
-                     flags, info, err);
-  hipLaunchKernelGGL(zero_u32_kernel, dim3(gz), dim3(256), 0, s, flags, nblk);
+                     info, err);
+  // b has been read: it now receives x, entry by entry
+  hipLaunchKernelGGL(fill_unsolved_kernel, dim3(gf), dim3(256), 0, s, (uint64_t*)b, n);
   hipLaunchKernelGGL(trsv_bwd_lookback, dim3((unsigned)nblk), dim3(256), 0, s, a, lda, n, inv, y, b,
-                     flags, info, err);
+                     info, err);
   hipLaunchKernelGGL(nan_on_info_kernel, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 1024)),
                      dim3(256), 0, s, b, n, info, err);
   NKA_CHECK_LAUNCH("nka_potrs_f64");
