@@ -635,14 +635,37 @@ __device__ __forceinline__ int factor_block_wave(double* Ab, double* colbuf, dou
   return bad;
 }
 
+constexpr int MI_LD = 17;               // row stride of a 16x16 inverse block (doubles)
+constexpr int MI_SZ = 16 * MI_LD;       // one inverse block
+
+// Mi = L_pp^{-1} for the 16x16 diagonal sub-block p of L (Ls, row-major LDS;
+// rdiag[j] = 1/L_jj), stored row-major with row stride MI_LD.  One wavefront:
+// lane k (< 16) forms column k by forward substitution; lanes 16.. repeat the
+// work of lane & 15 and do not store.
+__device__ __forceinline__ void invert_diag16_wave(const double* Ls, const double* rdiag, double* Mi, int p,
+                                                   int lane) {
+  const int k = lane & 15;
+  double m[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) m[i] = (i == k) ? 1.0 : 0.0;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    m[j] *= rdiag[16 * p + j];
+#pragma unroll
+    for (int i = j + 1; i < 16; ++i) m[i] -= Ls[(16 * p + i) * TS_LD + 16 * p + j] * m[j];
+  }
+  if (lane < 16) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) Mi[i * MI_LD + k] = m[i];
+  }
+}
+
 // Solve 16 rows of X (LDS, row-major) in place, X <- X L^{-T}, with one
-// wavefront by 16-column blocks: block p first receives
-// X_p -= X_{<p} L_{p,<p}^T on FP64 MFMA (v_mfma_f64_16x16x4, one 16x16
-// accumulator), then the 16x16 triangular solve X_p <- X_p L_pp^{-T} by
-// substitution, one row per lane with 16 values in registers (lanes 16..63
-// repeat the work of lane & 15 and do not store).  Ls = L (row-major, LDS),
-// rdiag[j] = 1/L_jj.
-__device__ __forceinline__ void solve_rows16_wave(const double* Ls, const double* rdiag, double* X, int lane) {
+// wavefront by 16-column blocks on FP64 MFMA (v_mfma_f64_16x16x4, one 16x16
+// accumulator): block p first receives X_p -= X_{<p} L_{p,<p}^T, then
+// X_p <- X_p (L_pp^{-1})^T with the inverse of the diagonal sub-block
+// (Mi + p * MI_SZ, from invert_diag16_wave).  Ls = L (row-major, LDS).
+__device__ __forceinline__ void solve_rows16_wave(const double* Ls, const double* Mi, double* X, int lane) {
   const int row = lane & 15;
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
@@ -661,20 +684,18 @@ __device__ __forceinline__ void solve_rows16_wave(const double* Ls, const double
       for (int e = 0; e < 4; ++e) X[((lane >> 4) + 4 * e) * TS_LD + 16 * p + row] -= acc[e];
       wave_lds_sync();
     }
-    // X[row, 16p..16p+16) <- X[row, 16p..) L_pp^{-T} (substitution)
-    double x[16];
+    // X_p <- X_p Mi_p^T: operand B[k][n] = Mi_p[n][k]
+    doublex4 acc = (doublex4){0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int q = 0; q < 16; ++q) x[q] = X[row * TS_LD + 16 * p + q];
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      x[j] *= rdiag[16 * p + j];
-#pragma unroll
-      for (int q = j + 1; q < 16; ++q) x[q] -= x[j] * Ls[(16 * p + q) * TS_LD + 16 * p + j];
+    for (int kb = 0; kb < 16; kb += 4) {
+      const int kk = kb + (lane >> 4);
+      const double av = X[row * TS_LD + 16 * p + kk];
+      const double bv = Mi[p * MI_SZ + row * MI_LD + kk];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
     }
-    if (lane < 16) {
+    wave_lds_sync();  // every lane's operand reads of X_p precede the writes
 #pragma unroll
-      for (int q = 0; q < 16; ++q) X[row * TS_LD + 16 * p + q] = x[q];
-    }
+    for (int e = 0; e < 4; ++e) X[((lane >> 4) + 4 * e) * TS_LD + 16 * p + row] = acc[e];
     wave_lds_sync();
   }
 }
@@ -741,6 +762,7 @@ __global__ void __launch_bounds__(256) panel_factor_kernel(double* __restrict__ 
   __shared__ double colbuf[CH_NB];
   __shared__ double rdiag[CH_NB];
   __shared__ double Xs[4][PF_ROWS * TS_LD];      // 16 rows per wavefront
+  __shared__ double Mi[4 * MI_SZ];               // inverses of L11's diagonal 16x16 blocks
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   double* X = Xs[w];
   const int64_t r0 = k0 + CH_NB + (int64_t)blockIdx.x * (4 * PF_ROWS) + PF_ROWS * w;
@@ -777,7 +799,11 @@ __global__ void __launch_bounds__(256) panel_factor_kernel(double* __restrict__ 
       slot[R * CH_NB + lane] = Ls[R * TS_LD + lane];
     }
   }
-  solve_rows16_wave(Ls, rdiag, X, lane);
+  // wavefront w inverts diagonal sub-block w of L11; then each wavefront
+  // solves its rows with all four
+  invert_diag16_wave(Ls, rdiag, Mi + w * MI_SZ, w, lane);
+  __syncthreads();
+  solve_rows16_wave(Ls, Mi, X, lane);
 #pragma unroll
   for (int r = 0; r < PF_ROWS; ++r) {
     if (r0 + r < n) A[(r0 + r) * lda + k0 + lane] = X[r * TS_LD + lane];
@@ -999,6 +1025,7 @@ __global__ void __launch_bounds__(256, 2) panel_step_kernel(double* __restrict__
   __shared__ double colbuf[CH_NB];
   __shared__ double rdiag[CH_NB];
   __shared__ double Xs[4][PF_ROWS * TS_LD];      // 16 rows per wavefront
+  __shared__ double Mi[4 * MI_SZ];               // inverses of L11's diagonal 16x16 blocks
   double* sA = Ls;                               // staging of the update tiles (before Ls is filled)
   double* sB = &Xs[0][0];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
@@ -1043,7 +1070,11 @@ __global__ void __launch_bounds__(256, 2) panel_step_kernel(double* __restrict__
       slot[R * CH_NB + lane] = Ls[R * TS_LD + lane];
     }
   }
-  solve_rows16_wave(Ls, rdiag, X, lane);
+  // wavefront w inverts diagonal sub-block w of L11; then each wavefront
+  // solves its rows with all four
+  invert_diag16_wave(Ls, rdiag, Mi + w * MI_SZ, w, lane);
+  __syncthreads();
+  solve_rows16_wave(Ls, Mi, X, lane);
 #pragma unroll
   for (int r = 0; r < PF_ROWS; ++r) {
     if (r0 + r < n) A[(r0 + r) * lda + k0 + lane] = X[r * TS_LD + lane];
