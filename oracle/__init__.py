@@ -17,7 +17,9 @@ How each part is pinned (see ``tests/test_oracle_*.py``):
                  vs the Kronecker Hamiltonian elementwise.
 * rbm          : log psi vs an independent dense evaluation; E_loc vs
                  (H psi)(s)/psi(s) from the dense H; derivatives vs central
-                 finite differences.
+                 finite differences; the energy gradient (forces_to_grad)
+                 and the SR right-hand side dv vs central differences of the
+                 exact energy <psi|H|psi>/<psi|psi> (test_oracle_gradient).
 * stats        : numpy identities and hand-computed branch cases.
 * ngd          : SR == minSR (Woodbury identity, test/driver/test_srt.py:98-211),
                  scipy cho_solve, CG residual bound, pinv_smooth vs lstsq.

@@ -19,17 +19,28 @@ import numpy as np
 import scipy.linalg as sla
 
 
-def centred_jacobian(O: np.ndarray) -> np.ndarray:
-    """(O - mean over samples) / sqrt(Ns)  (netket/jax/_jacobian/logic.py:365-373)."""
-    Ns = O.shape[0]
-    return (O - O.mean(axis=0, keepdims=True)) / np.sqrt(Ns)
+def _weights(n: int, weights):
+    return np.full(n, 1.0 / n) if weights is None else np.asarray(weights, dtype=np.float64)
 
 
-def prepare(O: np.ndarray, eloc: np.ndarray):
-    Ns = O.shape[0]
-    OL = centred_jacobian(O)
-    de = eloc - np.sum(eloc / Ns)
-    dv = 2.0 * de * np.sqrt(1.0 / Ns)
+def centred_jacobian(O: np.ndarray, weights=None) -> np.ndarray:
+    """sqrt(w_s) (O_s - sum_t w_t O_t): with w = 1/Ns this is
+    (O - mean over samples) / sqrt(Ns)  (netket/jax/_jacobian/logic.py:365-373)."""
+    w = _weights(O.shape[0], weights)
+    return np.sqrt(w)[:, None] * (O - (w @ O)[None, :])
+
+
+def prepare(O: np.ndarray, eloc: np.ndarray, weights=None):
+    """(O_L, dv) of _prepare_input (netket/_src/ngd/sr_srt_common.py:44-90):
+    dv = 2 sqrt(w) (E - sum_s w_s E_s), i.e. 2 (E - <E>) / sqrt(Ns) for the
+    uniform sample weights.  O_L^T dv is then the energy gradient of
+    ``rbm.forces_to_grad``; with the exact weights |psi|^2/Z over the full
+    basis it is the exact gradient (pinned by finite differences in the
+    oracle tests)."""
+    eloc = np.asarray(eloc, dtype=np.float64)
+    w = _weights(eloc.shape[0], weights)
+    OL = centred_jacobian(O, w)
+    dv = 2.0 * np.sqrt(w) * (eloc - np.sum(w * eloc))
     return OL, dv
 
 

@@ -10,9 +10,10 @@
                     Dense/bias, Dense/kernel, visible_bias
                     (netket/jax/_jacobian/logic.py:308-387), from the closed
                     form of the RBM derivative.
-* ``expect_and_grad`` forces_expect_hermitian + force_to_grad
-                    (netket/vqs/mc/mc_state/expect_forces.py:67-112,
-                    netket/vqs/mc/common.py:102-118).
+* ``forces_to_grad`` / ``expect_and_grad`` forces_expect_hermitian +
+                    force_to_grad (netket/vqs/mc/mc_state/expect_forces.py:67-112,
+                    netket/vqs/mc/common.py:102-118), one weighted formula
+                    whose uniform-weight case is the sample estimate.
 """
 
 from __future__ import annotations
@@ -88,16 +89,32 @@ def jacobian(p: RBMParams, x) -> np.ndarray:
     return np.concatenate(blocks, axis=1)
 
 
+def forces_to_grad(O, eloc, weights=None):
+    """grad = 2 Re sum_s w_s O_s^T conj(E_s - <E>_w), <E>_w = sum_s w_s E_s.
+
+    forces_expect_hermitian (netket/vqs/mc/mc_state/expect_forces.py:67-112)
+    is the Monte Carlo estimate with w_s = 1/Ns over the samples, and
+    force_to_grad (netket/vqs/mc/common.py:102-118) doubles the real part.
+    The same formula with the exact weights w_s = |psi(s)|^2 / Z over the full
+    basis is the gradient of the exact energy; the oracle tests pin it against
+    central finite differences of <psi|H|psi>/<psi|psi>.  Without weights the
+    uniform 1/Ns is used (the sample estimate)."""
+    O = np.asarray(O, dtype=np.float64)
+    eloc = np.asarray(eloc, dtype=np.float64)
+    w = np.full(eloc.shape[0], 1.0 / eloc.shape[0]) if weights is None else np.asarray(weights, np.float64)
+    ebar = np.sum(w * eloc)
+    return 2.0 * (O.T @ (w * (eloc - ebar))), ebar
+
+
 def expect_and_grad(p: RBMParams, samples, conn_fn):
-    """(Stats, grad) with grad = 2 Re[O^T conj(E - mean E) / Ns] on the chain-major samples (C, L, N)."""
+    """(Stats, grad, E_loc) on the chain-major samples (C, L, N), the gradient
+    by ``forces_to_grad`` with the uniform sample weights 1/Ns."""
     C, L, N = samples.shape
     x = samples.reshape(-1, N)
     e = local_energy(p, x, conn_fn)
     st = ostats.statistics(e.reshape(C, L))
-    ec = e - st["mean"]
-    O = jacobian(p, x)
-    F = O.T @ ec / x.shape[0]
-    return st, 2.0 * F, e
+    grad, _ = forces_to_grad(jacobian(p, x), e)
+    return st, grad, e
 
 
 def exact_probability(p: RBMParams, states) -> np.ndarray:
