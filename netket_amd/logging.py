@@ -3,12 +3,11 @@
 * RuntimeLog (runtime_log.py:30-190): ``log.data["Energy"]["Mean"]`` etc. in
   memory, one history per key with its ``iters``; ``serialize`` writes it as
   JSON and ``deserialize`` reads it back.
-* JsonLog (json_log.py:28-237): a RuntimeLog that writes ``<prefix>.log``
-  (the same JSON) every ``write_every`` steps and ``<prefix>.mpack`` (the
-  variational state's variables in flax msgpack encoding) every
-  ``save_params_every`` steps, both also when the time since the last write
-  keeps the writing cost under ``autoflush_cost`` of the run time, and at the
-  end of a run.
+* JsonLog (the files and arguments of json_log.py:28-237): a RuntimeLog
+  that keeps ``<prefix>.log`` (the same JSON) and ``<prefix>.mpack`` (the
+  variational state's variables in flax msgpack encoding) on disk.  When to
+  rewrite each file is decided by ``_WriteCadence`` below; both are written
+  at the end of a run.
 """
 
 from __future__ import annotations
@@ -114,74 +113,114 @@ class RuntimeLog:
         return f"RuntimeLog():\n keys = {list(self._data.keys())}"
 
 
+class _WriteCadence:
+    """When to rewrite one output file of a JsonLog.
+
+    A file is always rewritten once ``every`` logged steps are missing from
+    it.  Before that it is rewritten whenever the total time spent writing
+    it, plus one more write as long as the last one, stays within
+    ``cost_fraction`` of the wall time since the logger started.  A file that
+    is cheap to write is therefore kept current at every step, while one that
+    is slow to write (large parameters) falls back to every ``every`` steps,
+    and the writes never take more than about ``cost_fraction`` of a run that
+    is not limited by ``every``.  The first step is always written.
+    """
+
+    def __init__(self, every: int, cost_fraction: float):
+        self.every = max(int(every), 1)
+        self.cost_fraction = float(cost_fraction)
+        self.pending = 0
+        self.spent_s = 0.0
+        self.last_s = 0.0
+        self.started = time.perf_counter()
+
+    def note_step(self) -> bool:
+        """Count one logged step; True when the file should be written now."""
+        self.pending += 1
+        if self.pending >= self.every:
+            return True
+        budget = self.cost_fraction * (time.perf_counter() - self.started)
+        return self.spent_s + self.last_s <= budget
+
+    def timed(self, write) -> None:
+        t = time.perf_counter()
+        write()
+        self.last_s = time.perf_counter() - t
+        self.spent_s += self.last_s
+        self.pending = 0
+
+
+def _replace_file(path: str, payload, binary: bool) -> None:
+    """Write ``payload`` next to ``path`` and move it into place, so a reader
+    (or a crash during the write) never sees a half-written file."""
+    tmp = f"{path}.partial"
+    with open(tmp, "wb" if binary else "w") as f:
+        f.write(payload)
+    os.replace(tmp, path)
+
+
+_MODES = {"w": "write", "write": "write", "a": "append", "append": "append", "x": "fail",
+          "fail": "fail"}
+
+
 class JsonLog(RuntimeLog):
+    """RuntimeLog that mirrors its histories to ``<output_prefix>.log`` and the
+    variational state's variables to ``<output_prefix>.mpack``.
+
+    ``mode``: ``"write"``/``"w"`` replaces existing files, ``"append"``/``"a"``
+    continues the histories of an existing ``.log``, ``"fail"``/``"x"``
+    refuses a prefix whose files exist.  ``write_every`` and
+    ``save_params_every`` bound how many logged steps each file may lag
+    behind; ``autoflush_cost`` is the fraction of the run time the writes of
+    each file may take when they are made earlier than that."""
+
     def __init__(self, output_prefix, mode: str = "write", save_params_every: int = 50,
                  write_every: int = 50, save_params: bool = True, autoflush_cost: float = 0.005):
         super().__init__()
-        mode = {"w": "write", "a": "append", "x": "fail"}.get(mode, mode)
-        if mode not in ("write", "append", "fail"):
-            raise ValueError("Mode not recognized: should be one of `[w]rite`, `[a]ppend` or`[x]`(fail).")
-        output_prefix = str(output_prefix)
-        exists = os.path.exists(output_prefix + ".log") or os.path.exists(output_prefix + ".mpack")
-        if exists and mode == "fail":
-            raise ValueError("Output file already exists. Either delete it manually orchange `output_prefix`.")
-        dir_name = os.path.dirname(output_prefix)
-        if dir_name:
-            os.makedirs(dir_name, exist_ok=True)
-        self._prefix = output_prefix
-        self._file_mode = mode
-        self._write_every = write_every
-        self._save_params_every = save_params_every
-        self._save_params = save_params
-        self._steps_notflushed_write = 0
-        self._steps_notflushed_pars = 0
-        self._autoflush_cost = autoflush_cost
-        self._last_flush_time = time.time()
-        self._last_flush_runtime = 0.0
-        self._last_flush_pars_time = time.time()
-        self._last_flush_pars_runtime = 0.0
-        if mode == "append" and os.path.exists(output_prefix + ".log"):
-            existing = RuntimeLog.deserialize(output_prefix + ".log")
-            self._data = existing._data
-            self._old_step = existing._old_step
+        if mode not in _MODES:
+            raise ValueError(f"JsonLog mode {mode!r} is not one of 'write' ('w'), 'append' ('a') "
+                             f"or 'fail' ('x')")
+        self.mode = _MODES[mode]
+        self.output_prefix = str(output_prefix)
+        self.log_path = self.output_prefix + ".log"
+        self.params_path = self.output_prefix + ".mpack"
+        existing = [p for p in (self.log_path, self.params_path) if os.path.exists(p)]
+        if existing and self.mode == "fail":
+            raise ValueError(f"JsonLog in 'fail' mode: {', '.join(existing)} already exist(s); "
+                             f"remove the file(s) or choose another output_prefix")
+        folder = os.path.dirname(self.output_prefix)
+        if folder:
+            os.makedirs(folder, exist_ok=True)
+        self.save_params = bool(save_params)
+        self._log_cadence = _WriteCadence(write_every, autoflush_cost)
+        self._params_cadence = _WriteCadence(save_params_every, autoflush_cost)
+        if self.mode == "append" and os.path.exists(self.log_path):
+            previous = RuntimeLog.deserialize(self.log_path)
+            self._data, self._old_step = previous._data, previous._old_step
 
     def __call__(self, step: int, item: dict, variational_state=None):
-        old_step = self._old_step
         super().__call__(step, item, variational_state)
-        elapsed = time.time() - self._last_flush_time
-        flush_anyway = self._last_flush_runtime / (elapsed + 1e-7) < self._autoflush_cost
-        if self._steps_notflushed_write % self._write_every == 0 or step == old_step - 1 or flush_anyway:
-            self._flush_log()
-        elapsed = time.time() - self._last_flush_pars_time
-        flush_anyway = self._last_flush_pars_runtime / (elapsed + 1e-7) < self._autoflush_cost
-        if self._steps_notflushed_pars % self._save_params_every == 0 or step == old_step - 1 or flush_anyway:
-            self._flush_params(variational_state)
-        self._old_step = step
-        self._steps_notflushed_write += 1
-        self._steps_notflushed_pars += 1
+        if self._log_cadence.note_step():
+            self._write_log()
+        if self.save_params and variational_state is not None and self._params_cadence.note_step():
+            self._write_params(variational_state)
 
-    def _flush_log(self):
-        self._last_flush_time = time.time()
-        self.serialize(self._prefix + ".log")
-        self._last_flush_runtime = time.time() - self._last_flush_time
-        self._steps_notflushed_write = 0
+    def _write_log(self):
+        self._log_cadence.timed(
+            lambda: _replace_file(self.log_path, json.dumps(_jsonable(self._data)), binary=False))
 
-    def _flush_params(self, variational_state):
-        if not self._save_params or variational_state is None:
+    def _write_params(self, variational_state):
+        if not self.save_params or variational_state is None:
             return
         from . import serialization
 
-        self._last_flush_pars_time = time.time()
-        data = serialization.to_bytes(variational_state.variables)
-        with open(self._prefix + ".mpack", "wb") as f:
-            f.write(data)
-        self._last_flush_pars_runtime = time.time() - self._last_flush_pars_time
+        self._params_cadence.timed(lambda: _replace_file(
+            self.params_path, serialization.to_bytes(variational_state.variables), binary=True))
 
     def flush(self, variational_state=None):
-        """Write both files now (end of a run)."""
-        self._flush_log()
-        if variational_state is not None:
-            self._flush_params(variational_state)
+        """Write both files now (the end of a run)."""
+        self._write_log()
+        self._write_params(variational_state)
 
     def __repr__(self):
-        return f"JsonLog('{self._prefix}', mode={self._file_mode})"
+        return f"JsonLog('{self.output_prefix}', mode={self.mode})"

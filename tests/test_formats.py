@@ -79,3 +79,42 @@ def test_runtime_log_serialize_round_trip(tmp_path):
     np.testing.assert_array_equal(back["Energy"]["Mean"], log["Energy"]["Mean"])
     np.testing.assert_array_equal(back["Energy"]["iters"], [0, 1, 2, 3])
     assert back._old_step == 3
+
+
+class _StubState:
+    """Only what JsonLog reads from a variational state: its variables."""
+
+    def __init__(self):
+        self.step = -1
+
+    @property
+    def variables(self):
+        return {"params": {"step": np.array([self.step], dtype=np.int64)}}
+
+
+def test_json_log_write_cadence(tmp_path):
+    # with no time budget for early writes, each file is written at the first
+    # step and then whenever `every` logged steps are missing from it
+    prefix = str(tmp_path / "cad")
+    log = nklog.JsonLog(prefix, write_every=2, save_params_every=3, autoflush_cost=0.0)
+    vs = _StubState()
+    seen_log, seen_params = [], []
+    for step in range(10):
+        vs.step = step
+        log(step, {"Energy": _stats(float(step))}, vs)
+        seen_log.append(json.load(open(prefix + ".log"))["Energy"]["iters"][-1])
+        seen_params.append(int(ser.msgpack_restore(open(prefix + ".mpack", "rb").read())["params"]["step"][0]))
+    assert seen_log == [0, 0, 2, 2, 4, 4, 6, 6, 8, 8]
+    assert seen_params == [0, 0, 0, 3, 3, 3, 6, 6, 6, 9]
+    vs.step = 10
+    log.flush(vs)
+    assert json.load(open(prefix + ".log"))["Energy"]["iters"][-1] == 9
+    assert int(ser.msgpack_restore(open(prefix + ".mpack", "rb").read())["params"]["step"][0]) == 10
+    # a generous budget keeps both files current at every step
+    log2 = nklog.JsonLog(str(tmp_path / "eager"), write_every=1000, save_params_every=1000,
+                         autoflush_cost=1e9)
+    for step in range(4):
+        vs.step = step
+        log2(step, {"Energy": _stats(float(step))}, vs)
+        assert json.load(open(str(tmp_path / "eager") + ".log"))["Energy"]["iters"][-1] == step
+    assert not list(tmp_path.glob("*.partial"))
