@@ -196,6 +196,13 @@ int nka_ffnn_ntk(const nka_ffnn* m, const int8_t* sigma, const void* h1, const v
                  int64_t ns, const double* mean, double scale, double shift, double offset, double* out,
                  int64_t ldo, void* workspace, void* stream);
 
+/* Jacobian-vector product of the centred Jacobian, as nka_rbm_jvp, from the
+ * factors of nka_ffnn_prepare (netket/_src/ngd/srt.py:34-35). */
+int64_t nka_ffnn_jvp_workspace(const nka_ffnn* m, int64_t ns);
+int nka_ffnn_jvp(const nka_ffnn* m, const int8_t* sigma, const void* h1, const void* d1, const void* g2,
+                 int64_t ns, const double* v, const double* mean, double alpha, double beta, double* y,
+                 void* workspace, void* stream);
+
 /* Centred, rescaled Jacobian (layouts and output kinds of nka_rbm_jacobian). */
 int nka_ffnn_jacobian(const nka_ffnn* m, const int8_t* sigma, const void* h1, const void* d1,
                       const void* g2, int64_t ns, const double* mean, double scale, int layout,
@@ -310,6 +317,17 @@ int nka_rbm_ntk(const nka_rbm* m, const int8_t* sigma, const void* t, int64_t ns
                 const double* mean, double scale, double shift, double offset, double* out,
                 int64_t ldo, void* workspace, void* stream);
 
+/* Jacobian-vector product of the centred Jacobian without forming it:
+ * y = alpha (O - 1 mean^T) v + beta y (FP64, ns entries; beta == 0: y is not
+ * read), O as for nka_rbm_contract, t from nka_rbm_prepare, v of n_params
+ * entries in ravel order, mean == NULL meaning no centring.  With
+ * alpha = -mu / sqrt(ns), beta = 1 and y = dv this is the SPRING momentum term
+ * dv -= mu O_L old_updates of netket/_src/ngd/srt.py:34-35.  workspace holds
+ * nka_rbm_jvp_workspace(...) bytes. */
+int64_t nka_rbm_jvp_workspace(const nka_rbm* m, int64_t ns);
+int nka_rbm_jvp(const nka_rbm* m, const int8_t* sigma, const void* t, int64_t ns, const double* v,
+                const double* mean, double alpha, double beta, double* y, void* workspace, void* stream);
+
 /* C = beta C + alpha X X^T on the lower triangle (FP64 MFMA), X n x k row-major. */
 int nka_syrk_f64(const double* x, int64_t n, int64_t k, int64_t ldx, double alpha, double beta,
                  double shift, double* c, int64_t ldc, void* stream);
@@ -372,6 +390,9 @@ int nka_sgd_update(void* params, int dtype, const double* dp, int64_t n, double 
  * building blocks of the CG / dv preparation). */
 int nka_centre_scale(const double* e, int64_t n, const double* mean, double scale, double* out,
                      void* stream);
+/* y += alpha x (FP64): the momentum term updates += mu old_updates of
+ * netket/_src/ngd/srt.py:80-82. */
+int nka_axpy_f64(int64_t n, double alpha, const double* x, double* y, void* stream);
 
 #ifdef __cplusplus
 }

@@ -192,6 +192,33 @@ def ntk_structured(model, flat, sigma2d: torch.Tensor, t, mean: torch.Tensor, sc
 rbm_ntk = ntk_structured
 
 
+def jvp(model, flat, sigma2d: torch.Tensor, t, v: torch.Tensor, mean: torch.Tensor | None,
+        alpha: float, beta: float, y: torch.Tensor) -> torch.Tensor:
+    """y <- alpha (O - mean) v + beta y (FP64, one entry per sample) from the
+    model's layer structure, without a Jacobian."""
+    n, N = sigma2d.shape
+    m = model.native(flat, N)
+    with _ops.TIMER.region("jvp"):
+        if model.kind == "ffnn":
+            f = t
+            ws = _ops.WORKSPACE.get("ffnn_jvp", nat.lib().nka_ffnn_jvp_workspace(C.byref(m), n))
+            nat.call("nka_ffnn_jvp", C.byref(m), _ops.ptr(sigma2d), _ops.ptr(f.h1), _ops.ptr(f.d1),
+                     _ops.ptr(f.g2), n, _ops.ptr(v), _ops.ptr(mean), float(alpha), float(beta),
+                     _ops.ptr(y), _ops.ptr(ws), _ops.stream())
+        else:
+            ws = _ops.WORKSPACE.get("rbm_jvp", nat.lib().nka_rbm_jvp_workspace(C.byref(m), n))
+            nat.call("nka_rbm_jvp", C.byref(m), _ops.ptr(sigma2d), _ops.ptr(t), n, _ops.ptr(v),
+                     _ops.ptr(mean), float(alpha), float(beta), _ops.ptr(y), _ops.ptr(ws), _ops.stream())
+    return y
+
+
+def axpy_(y: torch.Tensor, alpha: float, x: torch.Tensor) -> torch.Tensor:
+    """y += alpha x (FP64 vectors)."""
+    with _ops.TIMER.region("axpy"):
+        nat.call("nka_axpy_f64", y.numel(), float(alpha), _ops.ptr(x), _ops.ptr(y), _ops.stream())
+    return y
+
+
 def centre_scale(e: torch.Tensor, mean_dev: torch.Tensor, scale: float) -> torch.Tensor:
     out = torch.empty_like(e)
     with _ops.TIMER.region("centre"):

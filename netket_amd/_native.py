@@ -134,6 +134,11 @@ SIGNATURES = {
     "nka_check_finite": (i32, [vp, i64, vp, vp]),
     "nka_sgd_update": (i32, [vp, i32, vp, i64, f64, vp]),
     "nka_centre_scale": (i32, [vp, i64, vp, f64, vp, vp]),
+    "nka_rbm_jvp_workspace": (i64, [C.POINTER(NkaRbm), i64]),
+    "nka_rbm_jvp": (i32, [C.POINTER(NkaRbm), vp, vp, i64, vp, vp, f64, f64, vp, vp, vp]),
+    "nka_ffnn_jvp_workspace": (i64, [C.POINTER(NkaFfnn), i64]),
+    "nka_ffnn_jvp": (i32, [C.POINTER(NkaFfnn), vp, vp, vp, vp, i64, vp, vp, f64, f64, vp, vp, vp]),
+    "nka_axpy_f64": (i32, [i64, f64, vp, vp, vp]),
     "nka_cg_workspace": (i64, [i64]),
     "nka_cg_start": (i32, [vp, i64, f64, f64, i64, vp, vp, vp]),
     "nka_cg_iterate": (i32, [vp, i64, i64, vp, vp, i32, vp]),

@@ -1071,6 +1071,10 @@ static void contract_one(const nka_ffnn& m, const int8_t* sigma, const T* h1, co
   }
 }
 
+// y = alpha (raw - mean.v) + beta y (nka_ntk.hip)
+void launch_jvp_finish(const double* raw, int64_t ns, const double* mean, const double* v, int64_t np,
+                       double* dot_slot, double alpha, double beta, double* y, hipStream_t s);
+
 }  // namespace nka
 
 using namespace nka;
@@ -1264,6 +1268,47 @@ int nka_ffnn_ntk(const nka_ffnn* m, const int8_t* sigma, const void* h1, const v
   hipLaunchKernelGGL(ffnn_ntk_kernel, dim3(grid), dim3(256), 0, s, h1d, g2d, sigma, d1d, N, H1, H2, ns, vv, cc,
                      m->b1_off >= 0 ? 1.0 : 0.0, m->b2_off >= 0 ? 1.0 : 0.0, scale, shift, offset, out, ldo, tiles);
   NKA_CHECK_LAUNCH("nka_ffnn_ntk");
+  return NKA_OK;
+}
+
+int64_t nka_ffnn_jvp_workspace(const nka_ffnn* m, int64_t ns) { return nka_ffnn_ntk_workspace(m, ns); }
+
+// (O v)_s = d1_s.(v_b1 + x_s v_W1) + g2_s.(v_b2 + h1_s v_W2): the products
+// D1 v_W1^T and G2 v_W2^T on FP64 matrix cores and the per-sample sums of the
+// NTK's v = O mu with v in place of mu.
+int nka_ffnn_jvp(const nka_ffnn* m, const int8_t* sigma, const void* h1, const void* d1, const void* g2,
+                 int64_t ns, const double* v, const double* mean, double alpha, double beta, double* y,
+                 void* workspace, void* stream) {
+  if (int st = check_model(m, "nka_ffnn_jvp")) return st;
+  NKA_CHECK_ARG(ns > 0 && sigma && h1 && d1 && g2 && v && y && workspace, "nka_ffnn_jvp: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  const int N = m->n_visible, H1 = m->n_hidden1, H2 = m->n_hidden2;
+  double* h1d = (double*)workspace;
+  double* g2d = h1d + ns * H1;
+  double* d1d = g2d + ns * H2;
+  double* U1 = d1d + ns * H1;
+  double* U2 = U1 + ns * N;
+  double* raw = U2 + ns * H1;
+  double* c = raw + ns;
+  const unsigned gc = 4096;
+  if (m->dtype == NKA_F32) {
+    hipLaunchKernelGGL((ffnn_to_f64_kernel<float>), dim3(gc), dim3(256), 0, s, (const float*)h1, ns * H1, h1d);
+    hipLaunchKernelGGL((ffnn_to_f64_kernel<float>), dim3(gc), dim3(256), 0, s, (const float*)g2, ns * H2, g2d);
+    hipLaunchKernelGGL((ffnn_to_f64_kernel<float>), dim3(gc), dim3(256), 0, s, (const float*)d1, ns * H1, d1d);
+  } else {
+    hipLaunchKernelGGL((ffnn_to_f64_kernel<double>), dim3(gc), dim3(256), 0, s, (const double*)h1, ns * H1, h1d);
+    hipLaunchKernelGGL((ffnn_to_f64_kernel<double>), dim3(gc), dim3(256), 0, s, (const double*)g2, ns * H2, g2d);
+    hipLaunchKernelGGL((ffnn_to_f64_kernel<double>), dim3(gc), dim3(256), 0, s, (const double*)d1, ns * H1, d1d);
+  }
+  int st = nka_gemm_nt_f64(d1d, H1, v + m->w1_off, H1, U1, N, ns, N, H1, 1.0, 0.0, stream);
+  if (st) return st;
+  st = nka_gemm_nt_f64(g2d, H2, v + m->w2_off, H2, U2, H1, ns, H1, H2, 1.0, 0.0, stream);
+  if (st) return st;
+  const int64_t np = ffnn_np(*m);
+  hipLaunchKernelGGL(ffnn_ntk_vec_kernel, dim3((unsigned)std::min<int64_t>((ns + 3) / 4, 4096)), dim3(256), 0, s,
+                     h1d, g2d, sigma, d1d, U1, U2, N, H1, H2, ns, v, m->b1_off, m->b2_off, np, raw, c);
+  launch_jvp_finish(raw, ns, mean, v, np, c + 1, alpha, beta, y, s);
+  NKA_CHECK_LAUNCH("nka_ffnn_jvp");
   return NKA_OK;
 }
 

@@ -1363,6 +1363,11 @@ __global__ void sgd_kernel(T* __restrict__ p, const double* __restrict__ dp, int
   }
 }
 
+__global__ void axpy_kernel(int64_t n, double alpha, const double* __restrict__ x, double* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] += alpha * x[i];
+}
+
 __global__ void centre_scale_kernel(const double* __restrict__ e, int64_t n, const double* __restrict__ mean,
                                     double scale, double* __restrict__ out) {
   const double mu = *mean;
@@ -1659,18 +1664,14 @@ static inline double* ws_split(void* ws, int64_t n, int64_t nblk) {
 // against the tile-count quantisation of the narrower updates.  The inverses
 // of the diagonal blocks, used by the triangular solves, are formed at the
 // end.
-int nka_potrf_f64(double* a, int64_t n, int64_t lda, void* workspace, int32_t* info,
-                  void* stream) {
-  NKA_CHECK_ARG(n >= 0 && lda >= n && a && workspace && info, "nka_potrf_f64: bad arguments");
-  if (n == 0) return NKA_OK;
-  hipStream_t s = (hipStream_t)stream;
+// The launches of the factorisation.  split / n_split are the K-slices of
+// the trailing products' split tail tiles (nullptr / 0: no split).
+static void potrf_run(double* a, int64_t n, int64_t lda, void* workspace, double* split, int64_t n_split,
+                      int32_t* info, hipStream_t s) {
   double* inv = ws_inv(workspace);
   const int64_t nblk = (n + CH_NB - 1) / CH_NB;
-  NKA_CHECK_ARG(nblk < (1LL << 31), "nka_potrf_f64: too large");
   // panel widths measured best on MI355X: n = 8384 -> (512, 512), n = 32768 -> (4096, 512)
   const int64_t NB0 = n >= 16384 ? 4096 : 512, NB1 = 512;
-  double* split = ws_split(workspace, n, nblk);
-  const int64_t n_split = split_units(n);
   for (int64_t K0 = 0; K0 < n; K0 += NB0) {
     const int64_t n0 = std::min<int64_t>(NB0, n - K0);
     for (int64_t K1 = K0; K1 < K0 + n0; K1 += NB1) {
@@ -1730,18 +1731,23 @@ int nka_potrf_f64(double* a, int64_t n, int64_t lda, void* workspace, int32_t* i
     }
   }
   hipLaunchKernelGGL(trtri_blocks_kernel, dim3((unsigned)nblk), dim3(64), 0, s, a, lda, n, inv, info);
+}
+
+int nka_potrf_f64(double* a, int64_t n, int64_t lda, void* workspace, int32_t* info,
+                  void* stream) {
+  NKA_CHECK_ARG(n >= 0 && lda >= n && a && workspace && info, "nka_potrf_f64: bad arguments");
+  if (n == 0) return NKA_OK;
+  const int64_t nblk = (n + CH_NB - 1) / CH_NB;
+  NKA_CHECK_ARG(nblk < (1LL << 31), "nka_potrf_f64: too large");
+  potrf_run(a, n, lda, workspace, ws_split(workspace, n, nblk), split_units(n), info, (hipStream_t)stream);
   NKA_CHECK_LAUNCH("nka_potrf_f64");
   return NKA_OK;
 }
 
-int nka_potrs_f64(const double* a, int64_t n, int64_t lda, const void* workspace, double* b,
-                  const int32_t* info, void* stream) {
-  NKA_CHECK_ARG(n >= 0 && lda >= n && a && workspace && b && info, "nka_potrs_f64: bad arguments");
-  if (n == 0) return NKA_OK;
-  hipStream_t s = (hipStream_t)stream;
-  void* ws = (void*)workspace;
+// The launches of the two triangular solves.
+static void potrs_run(const double* a, int64_t n, int64_t lda, void* ws, double* b, const int32_t* info,
+                      hipStream_t s) {
   const int64_t nblk = (n + CH_NB - 1) / CH_NB;
-  NKA_CHECK_ARG(nblk < (1LL << 31), "nka_potrs_f64: too large");
   const double* inv = ws_inv(ws);
   double* y = ws_vec(ws, nblk);
   uint32_t* flags = ws_flags(ws, n, nblk);
@@ -1757,6 +1763,14 @@ int nka_potrs_f64(const double* a, int64_t n, int64_t lda, const void* workspace
                      info, err);
   hipLaunchKernelGGL(nan_on_info_kernel, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 1024)),
                      dim3(256), 0, s, b, n, info, err);
+}
+
+int nka_potrs_f64(const double* a, int64_t n, int64_t lda, const void* workspace, double* b,
+                  const int32_t* info, void* stream) {
+  NKA_CHECK_ARG(n >= 0 && lda >= n && a && workspace && b && info, "nka_potrs_f64: bad arguments");
+  if (n == 0) return NKA_OK;
+  NKA_CHECK_ARG((n + CH_NB - 1) / CH_NB < (1LL << 31), "nka_potrs_f64: too large");
+  potrs_run(a, n, lda, (void*)workspace, b, info, (hipStream_t)stream);
   NKA_CHECK_LAUNCH("nka_potrs_f64");
   return NKA_OK;
 }
@@ -1812,6 +1826,15 @@ int nka_sgd_update(void* params, int dtype, const double* dp, int64_t n, double 
     return NKA_ERR_ARG;
   }
   NKA_CHECK_LAUNCH("nka_sgd_update");
+  return NKA_OK;
+}
+
+int nka_axpy_f64(int64_t n, double alpha, const double* x, double* y, void* stream) {
+  NKA_CHECK_ARG(n >= 0 && (n == 0 || (x && y)), "nka_axpy_f64: bad arguments");
+  if (n == 0) return NKA_OK;
+  const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(axpy_kernel, dim3(g), dim3(256), 0, (hipStream_t)stream, n, alpha, x, y);
+  NKA_CHECK_LAUNCH("nka_axpy_f64");
   return NKA_OK;
 }
 

@@ -92,6 +92,41 @@ __global__ void ntk_pack_spins_kernel(const int8_t* __restrict__ sigma, int64_t 
   }
 }
 
+// dot = a.b over n entries in one workgroup, summed in a fixed order.
+__global__ void __launch_bounds__(1024) dot_f64_kernel(const double* __restrict__ a, const double* __restrict__ b,
+                                                       int64_t n, double* __restrict__ dot) {
+  __shared__ double red[16];
+  double q = 0.0;
+  for (int64_t p = threadIdx.x; p < n; p += 1024) q += a[p] * b[p];
+  q = wave_sum(q);
+  if (lane_id() == 0) red[threadIdx.x >> 6] = q;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    for (int w = 0; w < 16; ++w) r += red[w];
+    dot[0] = r;
+  }
+}
+
+// y_s = alpha (raw_s - dot) + beta y_s (dot == NULL: 0; beta == 0: y is not read)
+__global__ void jvp_finish_kernel(const double* __restrict__ raw, int64_t ns, const double* __restrict__ dot,
+                                  double alpha, double beta, double* __restrict__ y) {
+  const double d = dot ? dot[0] : 0.0;
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += (int64_t)gridDim.x * blockDim.x) {
+    const double v = alpha * (raw[s] - d);
+    y[s] = beta == 0.0 ? v : v + beta * y[s];
+  }
+}
+
+// Shared tail of the Jacobian-vector products: y = alpha (raw - mean.v) + beta y.
+void launch_jvp_finish(const double* raw, int64_t ns, const double* mean, const double* v, int64_t np,
+                       double* dot_slot, double alpha, double beta, double* y, hipStream_t s) {
+  if (mean) hipLaunchKernelGGL(dot_f64_kernel, dim3(1), dim3(1024), 0, s, mean, v, np, dot_slot);
+  const unsigned g = (unsigned)std::min<int64_t>((ns + 255) / 256, 4096);
+  hipLaunchKernelGGL(jvp_finish_kernel, dim3(g), dim3(256), 0, s, raw, ns, mean ? dot_slot : nullptr, alpha,
+                     beta, y);
+}
+
 bool launch_gram_rbm(const uint16_t* xh, const uint16_t* xl, int64_t n, int64_t K, int64_t ldx,
                      const uint32_t* bits, int nw, int N, const double* v, const double* c, double hb,
                      double ha, double scale, double shift, double offset, double* out, int64_t ldo,
@@ -225,6 +260,40 @@ int64_t nka_rbm_ntk_workspace(const nka_rbm* m, int64_t ns) {
     b += ns * ntk_nw(*m) * (int64_t)sizeof(uint32_t);
   }
   return b;
+}
+
+int64_t nka_rbm_jvp_workspace(const nka_rbm* m, int64_t ns) {
+  if (!m || ns <= 0) return 0;
+  return ntk_base_bytes(*m, ns) + 256;
+}
+
+// (O v)_s = t_s.v_b + sum_i sigma_si (t_s v_W^T)_i + sigma_s.v_a: the product
+// t v_W^T on FP64 matrix cores and the per-sample sums of the NTK's v = O mu
+// with v in place of mu.
+int nka_rbm_jvp(const nka_rbm* m, const int8_t* sigma, const void* t, int64_t ns, const double* v,
+                const double* mean, double alpha, double beta, double* y, void* workspace, void* stream) {
+  NKA_CHECK_ARG(m && m->params && sigma && t && v && y && workspace && ns > 0, "nka_rbm_jvp: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  const int N = m->n_visible, M = m->n_hidden;
+  const int64_t np = (m->bias_off >= 0 ? M : 0) + (int64_t)N * M + (m->visible_off >= 0 ? N : 0);
+  double* t64 = (double*)workspace;
+  double* U = t64 + ns * M;
+  double* raw = U + ns * N;
+  double* c = raw + ns;  // |v|^2 (written by the shared kernel, not used) and mean.v
+  const int64_t total = ns * M;
+  const unsigned g1 = (unsigned)std::min<int64_t>((total + 255) / 256, 65536);
+  if (m->dtype == NKA_F32)
+    hipLaunchKernelGGL(to_f64_kernel<float>, dim3(g1), dim3(256), 0, s, (const float*)t, total, t64);
+  else
+    hipLaunchKernelGGL(to_f64_kernel<double>, dim3(g1), dim3(256), 0, s, (const double*)t, total, t64);
+  int st = nka_gemm_nt_f64(t64, M, v + m->kernel_off, M, U, N, ns, N, M, 1.0, 0.0, stream);
+  if (st) return st;
+  const unsigned g2 = (unsigned)std::min<int64_t>((ns + 3) / 4, 4096);
+  hipLaunchKernelGGL(ntk_vec_kernel, dim3(g2), dim3(256), 0, s, t64, sigma, U, ns, N, M, v, m->bias_off,
+                     m->visible_off, np, raw, c);
+  launch_jvp_finish(raw, ns, mean, v, np, c + 1, alpha, beta, y, s);
+  NKA_CHECK_LAUNCH("nka_rbm_jvp");
+  return NKA_OK;
 }
 
 int nka_rbm_ntk(const nka_rbm* m, const int8_t* sigma, const void* t, int64_t ns,

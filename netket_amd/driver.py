@@ -196,8 +196,6 @@ class VMC_SR(AbstractVariationalDriver):
             # with vjp/jvp products)
             if ntk_method != "structured":
                 raise ValueError("on_the_fly=True needs ntk_method='structured'")
-        if momentum is not None and not use_ntk:
-            pass
         self._ham = hamiltonian
         self.diag_shift = diag_shift
         self.proj_reg = proj_reg
@@ -247,16 +245,19 @@ class VMC_SR(AbstractVariationalDriver):
                 raise ValueError("proj_reg not implemented for SR")
             # F = O_L^T dv  (the contraction weights are dv / sqrt(Ns))
             F = eng.contract(model, flat, sig, t, w_dv.reshape(1, -1), mean)[0]
-            if momentum is not None and self._old_updates is not None:
-                F = F + diag_shift * momentum * self._old_updates
+            if momentum is not None:
+                # F + eps mu old_updates (netket/_src/ngd/sr.py:45-47)
+                eng.axpy_(F, diag_shift * float(momentum), self._momentum_buffer(flat))
             system = solvers.DenseSystem(builder=lambda: eng.gram(X, diag_shift))
             x, info = self._solve(system, F)
             if momentum is not None:
-                self._old_updates = x
+                self._old_updates.copy_(x)  # old_updates = updates (sr.py:67-68)
             dp = x
         else:
-            if momentum is not None and self._old_updates is not None:
-                raise NotImplementedError("SPRING momentum with the NTK is not supported")
+            if momentum is not None:
+                # dv -= mu O_L old_updates (netket/_src/ngd/srt.py:34-35)
+                eng.jvp(model, flat, sig, t, self._momentum_buffer(flat), mean,
+                        -float(momentum) / math.sqrt(ns), 1.0, dv)
             offset = 0.0 if proj_reg is None else float(proj_reg) / ns
             if structured:
                 system = solvers.DenseSystem(builder=lambda: eng.ntk_structured(
@@ -267,8 +268,19 @@ class VMC_SR(AbstractVariationalDriver):
             # updates = O_L^T y (srt.py:79), from the analytic Jacobian
             dp = eng.contract(model, flat, sig, t, eng.scale(y, 1.0 / math.sqrt(ns)).reshape(1, -1),
                               mean)[0]
+            if momentum is not None:
+                # updates += mu old_updates; old_updates = updates (srt.py:80-82)
+                eng.axpy_(dp, float(momentum), self._old_updates)
+                self._old_updates.copy_(dp)
         self.info = info
         return st, dp
+
+    def _momentum_buffer(self, flat):
+        """old_updates, zero before the first step (sr_srt_common.py:168-169);
+        one persistent buffer, so a captured step reads the previous replay's."""
+        if self._old_updates is None or self._old_updates.numel() != flat.numel():
+            self._old_updates = torch.zeros(flat.numel(), dtype=torch.float64, device=flat.device)
+        return self._old_updates
 
     def _solve(self, system, b):
         res = self.linear_solver(system, b)

@@ -154,6 +154,7 @@ class ServingReport:
     max_concurrent: int
     final_energies: dict = field(default_factory=dict)
     final_params: dict = field(default_factory=dict)
+    rejected: list = field(default_factory=list)
 
 
 class Server:
@@ -161,13 +162,19 @@ class Server:
                  max_active: int = 64, keep_params: bool = False):
         self.keep_params = keep_params
         self.budget = int(hbm_budget_gb * (1 << 30))
-        self.streams = [torch.cuda.Stream() for _ in range(n_streams)]
+        self.n_streams = n_streams
+        self.streams = None
         self.use_graph = use_graph
         self.max_active = max_active
 
     def run(self, specs, time_scale: float = 1.0, on_admit=None, on_progress=None,
             progress_every_s: float = 10.0) -> ServingReport:
-        pending = sorted(specs, key=lambda s: s.arrival)
+        # a job larger than the whole budget can never be admitted: it is
+        # reported as rejected instead of blocking the queue
+        rejected = [s.job_id for s in specs if footprint_bytes(s) > self.budget]
+        pending = sorted((s for s in specs if footprint_bytes(s) <= self.budget), key=lambda s: s.arrival)
+        if pending and self.streams is None:
+            self.streams = [torch.cuda.Stream() for _ in range(self.n_streams)]
         active = []
         used = 0
         t0 = time.perf_counter()
@@ -233,11 +240,10 @@ class Server:
         wall = time.perf_counter() - t0
         jl = np.array(list(finished_at.values()))
         sl = np.array(step_lat)
+        pct = lambda a, q: float(np.percentile(a, q)) if a.size else float("nan")
         return ServingReport(
-            n_jobs=len(specs), wall_s=wall, samples_per_s=total_samples / wall,
-            steps_per_s=total_steps / wall,
-            step_latency_ms_p50=float(np.percentile(sl, 50)),
-            step_latency_ms_p99=float(np.percentile(sl, 99)),
-            job_latency_s_p50=float(np.percentile(jl, 50)),
-            job_latency_s_p99=float(np.percentile(jl, 99)),
-            max_concurrent=max_conc, final_energies=finals, final_params=params)
+            n_jobs=len(specs), wall_s=wall, samples_per_s=total_samples / wall if wall > 0 else 0.0,
+            steps_per_s=total_steps / wall if wall > 0 else 0.0,
+            step_latency_ms_p50=pct(sl, 50), step_latency_ms_p99=pct(sl, 99),
+            job_latency_s_p50=pct(jl, 50), job_latency_s_p99=pct(jl, 99),
+            max_concurrent=max_conc, final_energies=finals, final_params=params, rejected=rejected)

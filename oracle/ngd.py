@@ -49,18 +49,29 @@ def cholesky_solve(A: np.ndarray, b: np.ndarray) -> np.ndarray:
     return sla.cho_solve((c, low), b)
 
 
-def sr_update(OL, dv, diag_shift):
+def sr_update(OL, dv, diag_shift, momentum=None, old_updates=None):
+    """SR update; with SPRING momentum F += eps mu old_updates (sr.py:43-47),
+    and the new old_updates is the returned x (sr.py:67-68)."""
     F = OL.T @ dv
+    if momentum is not None:
+        F = F + diag_shift * momentum * old_updates
     S = OL.T @ OL + diag_shift * np.eye(OL.shape[1])
     return cholesky_solve(S, F), S, F
 
 
-def srt_update(OL, dv, diag_shift, proj_reg=None):
+def srt_update(OL, dv, diag_shift, proj_reg=None, momentum=None, old_updates=None):
+    """minSR update; with SPRING momentum dv -= mu O_L old_updates before the
+    solve and updates += mu old_updates after it (srt.py:34-35, 80-82)."""
+    if momentum is not None:
+        dv = dv - momentum * (OL @ old_updates)
     T = OL @ OL.T + diag_shift * np.eye(OL.shape[0])
     if proj_reg is not None:
         T = T + proj_reg / T.shape[0]
     y = cholesky_solve(T, dv)
-    return OL.T @ y, T, y
+    upd = OL.T @ y
+    if momentum is not None:
+        upd = upd + momentum * old_updates
+    return upd, T, y
 
 
 def qgt_dense(OL, diag_shift):

@@ -156,6 +156,33 @@ def test_vmc_sr_update_matches_oracle(gpu, ntk_method):
     np.testing.assert_allclose(out[True], ref, rtol=0, atol=1e-8 * scale)
 
 
+def test_vmc_minsr_spring_momentum_two_steps(gpu):
+    # minSR with momentum 0.8 through the network's structured Jacobian-vector
+    # product, two chained steps against the oracle (netket/_src/ngd/srt.py:34-82)
+    print("case ffnn minsr spring momentum", flush=True)
+    g, hi, H, conn = _j1j2(4)
+    N = hi.size
+    sa = nk.sampler.MetropolisExchange(hi, graph=g, n_chains=16)
+    model = nk.models.FFNN(hidden_dims=(10, 8), param_dtype=np.float64)
+    vs = nk.vqs.MCState(sa, model, n_samples=64, n_discard_per_chain=4, seed=5)
+    vs.parameters = torch.as_tensor(np.random.default_rng(6).normal(scale=0.2, size=vs.n_parameters),
+                                    device="cuda")
+    drv = nk.VMC_SR(H, nk.optimizer.Sgd(0.05), diag_shift=1e-2, variational_state=vs, use_ntk=True,
+                    momentum=0.8, linear_solver=gsolver.cholesky)
+    old = np.zeros(vs.n_parameters)
+    for step in range(2):
+        drv.reset_step()
+        p = of.FFNNParams(vs.parameters_flat.cpu().numpy(), N, 10, 8)
+        x = vs.samples.cpu().numpy().reshape(-1, N)
+        st, dp = drv.compute_loss_and_update()
+        OL, dv = ongd.prepare(of.jacobian(p, x), of.local_energy(p, x, conn))
+        ref, _, _ = ongd.srt_update(OL, dv, 1e-2, momentum=0.8, old_updates=old)
+        np.testing.assert_allclose(dp.cpu().numpy(), ref, rtol=0, atol=1e-8 * np.abs(ref).max())
+        old = ref
+        drv.update_parameters(dp)
+        drv._step_count += 1
+
+
 @pytest.mark.parametrize("rule,H1,H2,scale", [("local", 9, 7, 0.4), ("exchange", 9, 7, 0.4),
                                                ("exchange", 420, 30, 0.05)])
 def test_chains_identical_to_oracle_fp64(gpu, rule, H1, H2, scale):

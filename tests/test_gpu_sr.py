@@ -265,6 +265,37 @@ def test_vmc_sr_update_matches_oracle_and_minsr(gpu, dtype, ntk_method):
     np.testing.assert_allclose(out[True], ref_sr, rtol=0, atol=tol * scale)
 
 
+@pytest.mark.parametrize("use_ntk,ntk_method", [(False, "structured"), (True, "structured"),
+                                                (True, "jacobian")])
+def test_vmc_sr_spring_momentum_two_steps(gpu, use_ntk, ntk_method):
+    # SPRING (momentum 0.8): two chained steps against the oracle's SR / minSR
+    # momentum formulas on the same samples (netket/_src/ngd/sr.py:43-68, srt.py:34-82)
+    print(f"case spring momentum use_ntk={use_ntk} {ntk_method}", flush=True)
+    vs, H, p, conn = _state("ising", np.float64, 16, 256, scale=0.1)
+    N, M = vs.hilbert.size, vs.model.n_hidden(vs.hilbert.size)
+    drv = nk.VMC_SR(H, nk.optimizer.Sgd(0.05), diag_shift=1e-2, variational_state=vs, use_ntk=use_ntk,
+                    momentum=0.8, linear_solver=gsolver.cholesky, ntk_method=ntk_method)
+    old = np.zeros(vs.n_parameters)
+    for step in range(2):
+        drv.reset_step()
+        flat = vs.parameters_flat.cpu().numpy()
+        x = vs.samples.cpu().numpy().reshape(-1, N)
+        st, dp = drv.compute_loss_and_update()
+        pp = orbm.RBMParams(flat, N, M)
+        OL, dv = ongd.prepare(orbm.jacobian(pp, x), orbm.local_energy(pp, x, conn))
+        if use_ntk:
+            ref, _, _ = ongd.srt_update(OL, dv, 1e-2, momentum=0.8, old_updates=old)
+        else:
+            ref, _, _ = ongd.sr_update(OL, dv, 1e-2, momentum=0.8, old_updates=old)
+        got = dp.cpu().numpy()
+        print(f"step {step}: max|dp - ref| / max|ref| = {np.abs(got - ref).max() / np.abs(ref).max():.2e}",
+              flush=True)
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-8 * np.abs(ref).max())
+        old = ref
+        drv.update_parameters(dp)
+        drv._step_count += 1
+
+
 def test_vmc_sr_preconditioner_step(gpu):
     # VMC + SR(QGTJacobianDense, cholesky): dp = (S + eps I)^-1 grad on the same samples
     vs, H, p, conn = _state("heis", np.float64, 16, 512, scale=0.1)

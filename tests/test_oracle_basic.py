@@ -201,6 +201,23 @@ def test_sr_equals_minsr_woodbury():
     np.testing.assert_allclose(x1, x2, rtol=1e-9, atol=1e-12)
 
 
+def test_sr_equals_minsr_with_spring_momentum():
+    # with momentum mu the minSR update O_L^T (T + eps)^-1 (dv - mu O_L u) + mu u equals the
+    # SR update (S + eps)^-1 (O_L^T dv + eps mu u) (Woodbury), over two chained steps
+    rng = np.random.default_rng(16)
+    u_sr = u_srt = np.zeros(25)
+    for step in range(2):
+        OL, dv = ongd.prepare(rng.normal(size=(40, 25)), rng.normal(size=40))
+        x1, _, _ = ongd.sr_update(OL, dv, 1e-2, momentum=0.8, old_updates=u_sr)
+        x2, _, _ = ongd.srt_update(OL, dv, 1e-2, momentum=0.8, old_updates=u_srt)
+        np.testing.assert_allclose(x1, x2, rtol=1e-9, atol=1e-12)
+        if step == 1:
+            # the momentum term changes the update
+            x0, _, _ = ongd.sr_update(OL, dv, 1e-2)
+            assert np.linalg.norm(x1 - x0) > 1e-3 * np.linalg.norm(x0)
+        u_sr, u_srt = x1, x2
+
+
 def test_cholesky_and_cg_and_pinv():
     rng = np.random.default_rng(7)
     A = rng.normal(size=(30, 30))
