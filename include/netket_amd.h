@@ -356,6 +356,23 @@ int nka_potrf_f64(double* a, int64_t n, int64_t lda, void* workspace, int32_t* i
 int nka_potrs_f64(const double* a, int64_t n, int64_t lda, const void* workspace, double* b,
                   const int32_t* info, void* stream);
 
+/* x <- src (or 0 when src is NULL) if *flag != 0, else unchanged: the
+ * device-side branch of the nan_fallback solver
+ * (netket/_src/solvers/nan_fallback.py:34-38, jax.lax.cond on the flag). */
+int nka_select_on_flag_f64(const int32_t* flag, const double* src, double* x, int64_t n, void* stream);
+
+/* Scale-invariant regularisation of the dense QGT, diag_scale of
+ * netket/optimizer/qgt/qgt_jacobian_common.py:21-60 (to_shift_offset, rescale)
+ * and qgt_jacobian_dense.py:93-150: scale_k = sqrt(S_kk + offset) from the
+ * unregularised S in the lower triangle of a; mode 0 replaces the lower
+ * triangle by S_ij / (scale_i scale_j) + shift delta_ij (the system that is
+ * solved), mode 1 adds shift scale_k^2 to the diagonal (to_dense). */
+int nka_qgt_rescale_f64(double* a, int64_t n, int64_t lda, double offset, double shift, int mode,
+                        double* scale, void* stream);
+/* out = x * d (divide = 0) or x / d (divide = 1), FP64 elementwise: the y / scale
+ * and x / scale of the rescaled solve (qgt_jacobian_dense.py:99-132). */
+int nka_mul_div_f64(const double* x, const double* d, int64_t n, int divide, double* out, void* stream);
+
 /* Mirror the lower triangle of a row-major FP64 matrix onto its upper triangle. */
 int nka_symmetrize_f64(double* a, int64_t n, int64_t lda, void* stream);
 
@@ -374,6 +391,11 @@ int nka_check_finite(const double* x, int64_t n, int32_t* flag, void* stream);
 int64_t nka_cg_workspace(int64_t n);
 int nka_cg_start(const double* b, int64_t n, double tol, double atol, int64_t maxiter, double* x,
                  void* workspace, void* stream);
+/* As nka_cg_start from a given x0 (r0 = b - A x0; the tolerance stays
+ * relative to b.b): the x0 that SR passes when solver_restart=True
+ * (netket/optimizer/preconditioner.py:118). */
+int nka_cg_start_x0(const double* a, int64_t lda, const double* b, int64_t n, double tol, double atol,
+                    int64_t maxiter, const double* x0, double* x, void* workspace, void* stream);
 int nka_cg_iterate(const double* a, int64_t lda, int64_t n, double* x, void* workspace, int iters,
                    void* stream);
 int nka_cg_status(const void* workspace, int64_t n, double* out8, void* stream);

@@ -132,6 +132,7 @@ SIGNATURES = {
     "nka_potrs_f64": (i32, [vp, i64, i64, vp, vp, vp, vp]),
     "nka_symmetrize_f64": (i32, [vp, i64, i64, vp]),
     "nka_check_finite": (i32, [vp, i64, vp, vp]),
+    "nka_select_on_flag_f64": (i32, [vp, vp, vp, i64, vp]),
     "nka_sgd_update": (i32, [vp, i32, vp, i64, f64, vp]),
     "nka_centre_scale": (i32, [vp, i64, vp, f64, vp, vp]),
     "nka_rbm_jvp_workspace": (i64, [C.POINTER(NkaRbm), i64]),
@@ -141,7 +142,10 @@ SIGNATURES = {
     "nka_axpy_f64": (i32, [i64, f64, vp, vp, vp]),
     "nka_cg_workspace": (i64, [i64]),
     "nka_cg_start": (i32, [vp, i64, f64, f64, i64, vp, vp, vp]),
+    "nka_cg_start_x0": (i32, [vp, i64, vp, i64, f64, f64, i64, vp, vp, vp, vp]),
     "nka_cg_iterate": (i32, [vp, i64, i64, vp, vp, i32, vp]),
+    "nka_qgt_rescale_f64": (i32, [vp, i64, i64, f64, f64, i32, vp, vp]),
+    "nka_mul_div_f64": (i32, [vp, vp, i64, i32, vp, vp]),
     "nka_cg_status": (i32, [vp, i64, vp, vp]),
 }
 

@@ -315,3 +315,22 @@ def gemv(a: torch.Tensor, x: torch.Tensor, trans: int = 0, alpha: float = 1.0, b
 
 def check_finite(x: torch.Tensor, flag: torch.Tensor):
     nat.call("nka_check_finite", ptr(x), x.numel(), ptr(flag), stream())
+
+
+def select_on_flag(flag: torch.Tensor, src: torch.Tensor | None, x: torch.Tensor):
+    """x <- src (or 0 when src is None) if flag[0] != 0, else unchanged; on the device."""
+    nat.call("nka_select_on_flag_f64", ptr(flag), ptr(src), ptr(x), x.numel(), stream())
+
+
+def qgt_rescale_(a: torch.Tensor, offset: float, shift: float, mode: int, scale: torch.Tensor):
+    """Scale-invariant regularisation of the lower triangle of ``a`` in place;
+    ``scale`` receives sqrt(S_kk + offset)."""
+    nat.call("nka_qgt_rescale_f64", ptr(a), a.shape[0], a.stride(0), float(offset), float(shift), int(mode),
+             ptr(scale), stream())
+
+
+def mul_div(x: torch.Tensor, d: torch.Tensor, divide: bool) -> torch.Tensor:
+    """x / d or x * d as a new FP64 vector."""
+    out = torch.empty_like(x)
+    nat.call("nka_mul_div_f64", ptr(x), ptr(d), x.numel(), 1 if divide else 0, ptr(out), stream())
+    return out

@@ -1,5 +1,6 @@
 // Conjugate gradient on the device, with jax.scipy.sparse.linalg.cg semantics
-// (the default SR solver, netket/optimizer/sr.py:121): x0 = 0, stop when
+// (the default SR solver, netket/optimizer/sr.py:121): x0 = 0 (or a given x0,
+// used when SR keeps the previous solution, netket/optimizer/preconditioner.py:118), stop when
 // r.r <= max(tol^2 b.b, atol^2) or after maxiter iterations.
 //
 // State lives in a small device array so that no iteration synchronises with
@@ -135,6 +136,35 @@ __global__ void __launch_bounds__(256) cg_matvec(const double* __restrict__ A, i
   }
 }
 
+// x = x0, r = b - A x0, p = r for a full row-major matrix (one wavefront per row).
+__global__ void __launch_bounds__(256) cg_residual(const double* __restrict__ A, int64_t lda, int64_t n,
+                                                   const double* __restrict__ b, const double* __restrict__ x0,
+                                                   double* __restrict__ x, double* __restrict__ r,
+                                                   double* __restrict__ p) {
+  const int wid = threadIdx.x / 64, lane = lane_id();
+  for (int64_t row = (int64_t)blockIdx.x * 4 + wid; row < n; row += (int64_t)gridDim.x * 4) {
+    double s = 0.0;
+    for (int64_t c = lane; c < n; c += 64) s += A[row * lda + c] * x0[c];
+    s = wave_sum(s);
+    if (lane == 0) {
+      const double ri = b[row] - s;
+      x[row] = x0[row];
+      r[row] = ri;
+      p[row] = ri;
+    }
+  }
+}
+
+// sc[0] = r0.r0 and sc[1] = b.b already; the tolerance is relative to b.b
+// as in jax.scipy.sparse.linalg.cg, the first test is made on r0.r0.
+__global__ void cg_init_scalars_x0(double* __restrict__ sc, double tol, double atol, double maxiter) {
+  const double a2 = fmax(tol * tol * sc[1], atol * atol);
+  sc[4] = a2;
+  sc[5] = 0.0;
+  sc[7] = maxiter;
+  sc[6] = (sc[0] <= a2 || maxiter <= 0.0) ? 1.0 : 0.0;
+}
+
 }  // namespace nka
 
 using namespace nka;
@@ -158,6 +188,26 @@ int nka_cg_start(const double* b, int64_t n, double tol, double atol, int64_t ma
   hipLaunchKernelGGL(dot_final, dim3(1), dim3(256), 0, s, part, CG_BLOCKS, sc, 0, 0);
   hipLaunchKernelGGL(cg_init_scalars, dim3(1), dim3(1), 0, s, sc, tol, atol, (double)maxiter);
   NKA_CHECK_LAUNCH("nka_cg_start");
+  return NKA_OK;
+}
+
+int nka_cg_start_x0(const double* a, int64_t lda, const double* b, int64_t n, double tol, double atol,
+                    int64_t maxiter, const double* x0, double* x, void* workspace, void* stream) {
+  NKA_CHECK_ARG(n > 0 && lda >= n && a && b && x0 && x && workspace, "nka_cg_start_x0: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  double* ws = (double*)workspace;
+  double* r = ws;
+  double* p = ws + n;
+  double* part = ws + 3 * n;
+  double* sc = ws + 3 * n + CG_BLOCKS;
+  hipLaunchKernelGGL(dot_partial, dim3(CG_BLOCKS), dim3(256), 0, s, b, b, n, part, nullptr);
+  hipLaunchKernelGGL(dot_final, dim3(1), dim3(256), 0, s, part, CG_BLOCKS, sc, 1, 0);
+  const unsigned gm = (unsigned)std::min<int64_t>((n + 3) / 4, 4096);
+  hipLaunchKernelGGL(cg_residual, dim3(gm), dim3(256), 0, s, a, lda, n, b, x0, x, r, p);
+  hipLaunchKernelGGL(dot_partial, dim3(CG_BLOCKS), dim3(256), 0, s, r, r, n, part, nullptr);
+  hipLaunchKernelGGL(dot_final, dim3(1), dim3(256), 0, s, part, CG_BLOCKS, sc, 0, 0);
+  hipLaunchKernelGGL(cg_init_scalars_x0, dim3(1), dim3(1), 0, s, sc, tol, atol, (double)maxiter);
+  NKA_CHECK_LAUNCH("nka_cg_start_x0");
   return NKA_OK;
 }
 

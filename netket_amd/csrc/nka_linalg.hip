@@ -1775,6 +1775,80 @@ int nka_potrs_f64(const double* a, int64_t n, int64_t lda, const void* workspace
   return NKA_OK;
 }
 
+}  // extern "C"
+
+// Scale-invariant regularisation of the QGT (netket/optimizer/qgt/qgt_jacobian_common.py:21-60):
+// d_k = sqrt(S_kk + offset) from the diagonal of the unregularised S.
+__global__ void qgt_diag_scale_kernel(const double* __restrict__ a, int64_t n, int64_t lda, double offset,
+                                      double* __restrict__ d) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+    d[k] = sqrt(a[k * lda + k] + offset);
+}
+// mode 0: lower triangle S_ij / (d_i d_j) + shift delta_ij (the rescaled system);
+// mode 1: S_kk + shift d_k^2 on the diagonal only (to_dense of the scaled QGT).
+__global__ void qgt_rescale_kernel(double* __restrict__ a, int64_t n, int64_t lda, const double* __restrict__ d,
+                                   double shift, int mode) {
+  if (mode == 1) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+      a[k * lda + k] += shift * d[k] * d[k];
+    return;
+  }
+  for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const double di = d[i];
+    for (int64_t j = threadIdx.x; j <= i; j += blockDim.x) {
+      double v = a[i * lda + j] / (di * d[j]);
+      if (j == i) v += shift;
+      a[i * lda + j] = v;
+    }
+  }
+}
+__global__ void mul_div_kernel(const double* __restrict__ x, const double* __restrict__ d, int64_t n, int divide,
+                               double* __restrict__ out) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+    out[k] = divide ? x[k] / d[k] : x[k] * d[k];
+}
+
+__global__ void select_on_flag_kernel(const int32_t* __restrict__ flag, const double* __restrict__ src,
+                                      double* __restrict__ x, int64_t n) {
+  if (*flag == 0) return;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+    x[k] = src ? src[k] : 0.0;
+}
+
+extern "C" {
+
+int nka_qgt_rescale_f64(double* a, int64_t n, int64_t lda, double offset, double shift, int mode,
+                        double* scale, void* stream) {
+  NKA_CHECK_ARG(n >= 0 && lda >= n && (mode == 0 || mode == 1) && (n == 0 || (a && scale)),
+                "nka_qgt_rescale_f64: bad arguments");
+  if (n == 0) return NKA_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 1024);
+  hipLaunchKernelGGL(qgt_diag_scale_kernel, dim3(g), dim3(256), 0, s, a, n, lda, offset, scale);
+  const unsigned g2 = mode == 1 ? g : (unsigned)std::min<int64_t>(n, 8192);
+  hipLaunchKernelGGL(qgt_rescale_kernel, dim3(g2), dim3(256), 0, s, a, n, lda, scale, shift, mode);
+  NKA_CHECK_LAUNCH("nka_qgt_rescale_f64");
+  return NKA_OK;
+}
+
+int nka_mul_div_f64(const double* x, const double* d, int64_t n, int divide, double* out, void* stream) {
+  NKA_CHECK_ARG(n >= 0 && (n == 0 || (x && d && out)), "nka_mul_div_f64: bad arguments");
+  if (n == 0) return NKA_OK;
+  const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 1024);
+  hipLaunchKernelGGL(mul_div_kernel, dim3(g), dim3(256), 0, (hipStream_t)stream, x, d, n, divide, out);
+  NKA_CHECK_LAUNCH("nka_mul_div_f64");
+  return NKA_OK;
+}
+
+int nka_select_on_flag_f64(const int32_t* flag, const double* src, double* x, int64_t n, void* stream) {
+  NKA_CHECK_ARG(n >= 0 && flag && (n == 0 || x), "nka_select_on_flag_f64: bad arguments");
+  if (n == 0) return NKA_OK;
+  const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 1024);
+  hipLaunchKernelGGL(select_on_flag_kernel, dim3(g), dim3(256), 0, (hipStream_t)stream, flag, src, x, n);
+  NKA_CHECK_LAUNCH("nka_select_on_flag_f64");
+  return NKA_OK;
+}
+
 int nka_gemv_f64(const double* a, int64_t lda, int64_t m, int64_t n, int trans, const double* x,
                  double alpha, double beta, double* y, void* workspace, void* stream) {
   NKA_CHECK_ARG(m >= 0 && n >= 0 && a && x && y, "nka_gemv_f64: bad arguments");

@@ -57,6 +57,29 @@ class AbstractVariationalDriver:
     def compute_loss_and_update(self):
         raise NotImplementedError
 
+    def resolve_fallback(self, dp):
+        """The update of the last compute_loss_and_update, with the fallback
+        solver's solution when the device-side nan_fallback test of
+        ``cholesky_with_fallback`` fired.
+
+        Reads the device flag: the one host synchronisation of an eager step
+        that uses that solver, made after the whole step has been enqueued.
+        Inside a captured step nothing is read: a flagged solution is zero
+        there (no update) and the flag stays on the device in the solver's
+        info."""
+        pend = getattr(self, "_pending_fallback", None)
+        self._pending_fallback = None
+        if pend is None or torch.cuda.is_current_stream_capturing():
+            return dp
+        fb, redo, note = pend
+        fired = bool(fb)
+        note(fired)
+        return redo() if fired else dp
+
+    def _set_pending_fallback(self, info, redo, note):
+        fb = info.get("solver_fallback") if isinstance(info, dict) else None
+        self._pending_fallback = (fb, redo, note) if isinstance(fb, solvers.DeviceFallback) else None
+
     def update_parameters(self, dp_flat: torch.Tensor):
         """p <- p + (-lr * (dtype)dp) on the flat parameter buffer."""
         eng.sgd_update_(self.state.parameters_flat, dp_flat, self.optimizer.lr_at(self._step_count))
@@ -86,6 +109,7 @@ class AbstractVariationalDriver:
                 for name, op in obs.items():
                     log_data[name] = self.state.expect(op)
             self._log_additional_data(log_data)
+            dp = self.resolve_fallback(dp)
             stop = False
             for cb in callbacks:
                 if cb(self._step_count, log_data, self) is False:
@@ -110,6 +134,7 @@ class AbstractVariationalDriver:
     def _eager_step(self):
         self.reset_step()
         st, dp = self.compute_loss_and_update()
+        dp = self.resolve_fallback(dp)
         self.update_parameters(dp)
         return st
 
@@ -128,10 +153,11 @@ class AbstractVariationalDriver:
         # scratch buffers owned by this driver for as long as the graph lives
         # (stream handles come from a pool and are shared between drivers)
         self._graph_ws = _ops.Workspace()
+        self.warmup_stats = []
         with _ops.workspace_scope(self._graph_ws):
             with torch.cuda.stream(self._graph_stream):
                 for _ in range(warmup):
-                    self._eager_step()
+                    self.warmup_stats.append(self._eager_step())
                     self._step_count += 1
             torch.cuda.current_stream().wait_stream(self._graph_stream)
             torch.cuda.synchronize()
@@ -167,6 +193,18 @@ class VMC(AbstractVariationalDriver):
             dp = self.preconditioner(self.state, g, self._step_count)
             if isinstance(dp, dict):
                 dp = self.state.flatten_tree(dp)
+            pc = self.preconditioner
+            info = getattr(pc, "info", None)
+
+            def redo():
+                x = info["solver_fallback"].solve()
+                pc.x0 = x
+                return self.state.flatten_tree(x) if isinstance(x, dict) else x
+
+            def note(fired):
+                pc.info = {**info, "solver_fallback": fired}
+
+            self._set_pending_fallback(info, redo, note)
         return st, dp
 
 
@@ -209,6 +247,9 @@ class VMC_SR(AbstractVariationalDriver):
         self._old_updates = None
         self._jac = None
         self.info = None
+        # when a dict, each step leaves its local energies, right-hand side,
+        # solution, update and linear system there (for error reports)
+        self.record = None
 
     def _hyper(self, v):
         return v(self._step_count) if callable(v) else v
@@ -240,6 +281,10 @@ class VMC_SR(AbstractVariationalDriver):
             self._jac = eng.jacobian(model, flat, sig, t, tT, mean, 1.0 / math.sqrt(ns), layout,
                                      kind, self._jac)
         X = self._jac
+        if momentum is not None:
+            # the previous update, kept so that a fallback solve can redo this step's momentum terms
+            self._momentum_buffer(flat)
+            self._old_saved = self._old_updates.clone()
         if not self.use_ntk:
             if proj_reg is not None:
                 raise ValueError("proj_reg not implemented for SR")
@@ -247,16 +292,18 @@ class VMC_SR(AbstractVariationalDriver):
             F = eng.contract(model, flat, sig, t, w_dv.reshape(1, -1), mean)[0]
             if momentum is not None:
                 # F + eps mu old_updates (netket/_src/ngd/sr.py:45-47)
-                eng.axpy_(F, diag_shift * float(momentum), self._momentum_buffer(flat))
+                eng.axpy_(F, diag_shift * float(momentum), self._old_updates)
             system = solvers.DenseSystem(builder=lambda: eng.gram(X, diag_shift))
-            x, info = self._solve(system, F)
-            if momentum is not None:
-                self._old_updates.copy_(x)  # old_updates = updates (sr.py:67-68)
-            dp = x
+            rhs = F
+
+            def finish(x):
+                if momentum is not None:
+                    self._old_updates.copy_(x)  # old_updates = updates (sr.py:67-68)
+                return x
         else:
             if momentum is not None:
                 # dv -= mu O_L old_updates (netket/_src/ngd/srt.py:34-35)
-                eng.jvp(model, flat, sig, t, self._momentum_buffer(flat), mean,
+                eng.jvp(model, flat, sig, t, self._old_updates, mean,
                         -float(momentum) / math.sqrt(ns), 1.0, dv)
             offset = 0.0 if proj_reg is None else float(proj_reg) / ns
             if structured:
@@ -264,15 +311,32 @@ class VMC_SR(AbstractVariationalDriver):
                     model, flat, sig, t, mean, 1.0 / ns, diag_shift, offset))
             else:
                 system = solvers.DenseSystem(builder=lambda: eng.gram(X, diag_shift, offset))
-            y, info = self._solve(system, dv)
-            # updates = O_L^T y (srt.py:79), from the analytic Jacobian
-            dp = eng.contract(model, flat, sig, t, eng.scale(y, 1.0 / math.sqrt(ns)).reshape(1, -1),
-                              mean)[0]
-            if momentum is not None:
-                # updates += mu old_updates; old_updates = updates (srt.py:80-82)
-                eng.axpy_(dp, float(momentum), self._old_updates)
-                self._old_updates.copy_(dp)
+            rhs = dv
+
+            def finish(y):
+                # updates = O_L^T y (srt.py:79), from the analytic Jacobian
+                dp = eng.contract(model, flat, sig, t, eng.scale(y, 1.0 / math.sqrt(ns)).reshape(1, -1),
+                                  mean)[0]
+                if momentum is not None:
+                    # updates += mu old_updates; old_updates = updates (srt.py:80-82)
+                    eng.axpy_(dp, float(momentum), self._old_updates)
+                    self._old_updates.copy_(dp)
+                return dp
+        y, info = self._solve(system, rhs)
+        dp = finish(y)
         self.info = info
+        if self.record is not None:
+            self.record.update(eloc=e, rhs=rhs, solution=y, dp=dp, system=system)
+
+        def redo():
+            if momentum is not None:
+                self._old_updates.copy_(self._old_saved)
+            return finish(info["solver_fallback"].solve())
+
+        def note(fired):
+            self.info = {**info, "solver_fallback": fired}
+
+        self._set_pending_fallback(info, redo, note)
         return st, dp
 
     def _momentum_buffer(self, flat):

@@ -35,12 +35,18 @@ class LinearOperator:
 class QGTJacobianDenseT(LinearOperator):
     def __init__(self, vstate, *, diag_shift: float = 0.0, diag_scale=None, mode=None,
                  holomorphic=None, chunk_size=None):
-        if diag_scale is not None and diag_scale != 0.0:
-            raise NotImplementedError("diag_scale (scale-invariant regularisation) is not supported")
         if mode not in (None, "real"):
             raise NotImplementedError("only real-parameter ('real' mode) QGTs are supported")
         self.vstate = vstate
-        self.diag_shift = float(diag_shift)
+        # to_shift_offset (netket/optimizer/qgt/qgt_jacobian_common.py:21-34): with a
+        # nonzero diag_scale the solved system is D^-1 S D^-1 + diag_scale I with
+        # D_kk = sqrt(S_kk + diag_shift / diag_scale)
+        diag_shift = 0.0 if diag_shift is None else float(diag_shift)
+        if diag_scale is None or float(diag_scale) == 0.0:
+            self.diag_shift, self._offset = diag_shift, None
+        else:
+            self.diag_shift, self._offset = float(diag_scale), diag_shift / float(diag_scale)
+        self.scale = None
         s = vstate.samples
         C, L, N = s.shape
         self.n_samples = C * L
@@ -58,15 +64,52 @@ class QGTJacobianDenseT(LinearOperator):
 
     # -- matrices ---------------------------------------------------------------
     def lower(self) -> torch.Tensor:
-        """A fresh FP64 matrix whose lower triangle is S + diag_shift I."""
-        return eng.gram(self.X, self.diag_shift)
+        """A fresh FP64 matrix whose lower triangle is the system that is
+        solved: S + diag_shift I, or D^-1 S D^-1 + diag_shift I with a
+        diag_scale (which also sets ``self.scale`` = diag(D))."""
+        if self._offset is None:
+            return eng.gram(self.X, self.diag_shift)
+        a = eng.gram(self.X, 0.0)
+        self.scale = torch.empty(a.shape[0], dtype=torch.float64, device=a.device)
+        _ops.qgt_rescale_(a, self._offset, self.diag_shift, 0, self.scale)
+        return a
 
     def to_dense(self) -> torch.Tensor:
+        """S + diag_shift I, or S + diag_shift D^2 with a diag_scale
+        (qgt_jacobian_dense.py:136-161)."""
         if self._dense is None:
-            a = self.lower()
+            if self._offset is None:
+                a = self.lower()
+            else:
+                a = eng.gram(self.X, 0.0)
+                self.scale = torch.empty(a.shape[0], dtype=torch.float64, device=a.device)
+                _ops.qgt_rescale_(a, self._offset, self.diag_shift, 1, self.scale)
             _ops.symmetrize_(a)
             self._dense = a
         return self._dense
+
+    def solve(self, solve_fun, y, *, x0=None, **kwargs):
+        """_solve of netket/optimizer/qgt/qgt_jacobian_dense.py:93-133: with a
+        diag_scale the solver sees the rescaled system, y / D and x0 D, and the
+        solution is divided by D."""
+        if self._offset is None:
+            return solve_fun(self, y, x0=x0, **kwargs)
+        from .solver import DenseSystem
+
+        vs = self.vstate
+        yf = vs.flatten_tree(y) if isinstance(y, dict) else y.to(torch.float64).contiguous()
+        a = self.lower()
+        sys_ = DenseSystem(a, builder=self.lower)
+        ys = _ops.mul_div(yf, self.scale, divide=True)
+        x0s = None
+        if x0 is not None:
+            x0f = vs.flatten_tree(x0) if isinstance(x0, dict) else x0.to(torch.float64).contiguous()
+            x0s = _ops.mul_div(x0f, self.scale, divide=False)
+        out, info = solve_fun(sys_, ys, x0=x0s, **kwargs)
+        out = _ops.mul_div(out, self.scale, divide=True)
+        if isinstance(y, dict):
+            out = vs.unflatten(out)
+        return out, info
 
     def __matmul__(self, vec):
         v = self.vstate.flatten_tree(vec) if not isinstance(vec, torch.Tensor) else vec.to(torch.float64)
@@ -76,6 +119,8 @@ class QGTJacobianDenseT(LinearOperator):
         return out
 
     def __repr__(self):
+        if self._offset is not None:
+            return f"QGTJacobianDense(diag_scale={self.diag_shift}, offset={self._offset}, mode=real)"
         return f"QGTJacobianDense(diag_shift={self.diag_shift}, mode=real)"
 
 

@@ -83,7 +83,13 @@ def cholesky(A, b, *, lower: bool = False, x0=None):
 def pinv_smooth(A, b, *, rtol: float = 1e-14, rtol_smooth: float = 1e-14, x0=None,
                 rcond=None, rcond_smooth=None):
     """Smoothed pseudo-inverse from the eigendecomposition
-    (netket/optimizer/solver/solvers.py:32-137)."""
+    (netket/optimizer/solver/solvers.py:32-137).
+
+    The eigendecomposition is ``torch.linalg.eigh`` (rocSOLVER), not a kernel
+    of this library: it is the rarely taken fallback of
+    ``cholesky_with_fallback`` and of explicit calls, never part of the
+    timed step.
+    """
     if rcond is not None:
         rtol = rcond
     if rcond_smooth is not None:
@@ -101,18 +107,49 @@ def pinv_smooth(A, b, *, rtol: float = 1e-14, rtol_smooth: float = 1e-14, x0=Non
     return unflat(x), info
 
 
+class DeviceFallback:
+    """The outcome of the nan_fallback test, decided on the device.
+
+    ``flag`` is a device int32 (nonzero: the right-hand side or the primary
+    solution held NaN/Inf, or the factorisation failed).  The primary
+    solution has already been replaced by zeros on the device in that case,
+    so nothing downstream consumes NaN; ``solve()`` runs the fallback solver
+    on the same system when the caller finds the flag set.  Reading the flag
+    is the caller's choice of synchronisation point (the reference decides
+    with ``jax.lax.cond`` on the device, netket/_src/solvers/nan_fallback.py:34-38).
+    """
+
+    def __init__(self, flag: torch.Tensor, fallback):
+        self.flag = flag
+        self._fallback = fallback
+
+    def __bool__(self) -> bool:
+        return bool(self.flag.item() != 0)
+
+    def solve(self):
+        return self._fallback()
+
+
 def cholesky_with_fallback(A, b, *, rtol: float = 1e-14, rtol_smooth: float = 1e-14, x0=None):
     """Cholesky, falling back to pinv_smooth on NaN/Inf
-    (netket/optimizer/solver/solvers.py:304-340, netket/_src/solvers/nan_fallback.py:22-45)."""
+    (netket/optimizer/solver/solvers.py:304-340, netket/_src/solvers/nan_fallback.py:22-45).
+
+    Nothing here synchronises with the host, so the call can be captured in a
+    HIP graph.  The test is made on the device: a non-finite b, a failed
+    factorisation (the solve then returns NaN) or a non-finite x set the flag,
+    and a flagged x is zeroed on the device.  ``info["solver_fallback"]`` is a
+    ``DeviceFallback``: ``bool()`` of it reads the flag, and its ``solve()``
+    returns the pinv_smooth solution of the same system.  The drivers resolve
+    it at the end of each eager step.
+    """
     sys_, bf, unflat = _system(A, b)
     flag = torch.zeros(1, dtype=torch.int32, device=bf.device)
     _ops.check_finite(bf, flag)
     x, info = _chol_solve(sys_, bf)
     _ops.check_finite(x, flag)
-    fallback = bool(flag.item() != 0)
-    if fallback:
-        x, _ = pinv_smooth(sys_, bf, rtol=rtol, rtol_smooth=rtol_smooth)
-    return unflat(x), {"solver_fallback": fallback}
+    _ops.select_on_flag(flag, None, x)
+    fb = DeviceFallback(flag, lambda: pinv_smooth(sys_, bf, rtol=rtol, rtol_smooth=rtol_smooth)[0])
+    return unflat(x), {"solver_fallback": fb}
 
 
 def solve(A, b, *, x0=None):
@@ -122,11 +159,11 @@ def solve(A, b, *, x0=None):
 
 def cg(A, b, x0=None, *, tol: float = 1e-5, atol: float = 0.0, maxiter: int | None = None, M=None,
        check_every: int = 8):
-    """Conjugate gradient on the device (jax.scipy.sparse.linalg.cg semantics)."""
+    """Conjugate gradient on the device (jax.scipy.sparse.linalg.cg semantics),
+    from x0 = 0 or from a given x0 (the previous solution that SR passes with
+    solver_restart=True, netket/optimizer/preconditioner.py:118)."""
     if M is not None:
         raise NotImplementedError("preconditioned CG is not supported")
-    if x0 is not None:
-        raise NotImplementedError("CG starts from x0 = 0 (solver_restart=False)")
     sys_, bf, unflat = _system(A, b)
     a = sys_.to_dense()
     n = bf.numel()
@@ -135,8 +172,15 @@ def cg(A, b, x0=None, *, tol: float = 1e-5, atol: float = 0.0, maxiter: int | No
     x = torch.empty_like(bf)
     ws = _ops.WORKSPACE.get("cg", nat.lib().nka_cg_workspace(n))
     status = torch.empty(8, dtype=torch.float64, device=bf.device)
-    nat.call("nka_cg_start", _ops.ptr(bf), n, float(tol), float(atol), int(maxiter), _ops.ptr(x),
-             _ops.ptr(ws), _ops.stream())
+    if x0 is None:
+        nat.call("nka_cg_start", _ops.ptr(bf), n, float(tol), float(atol), int(maxiter), _ops.ptr(x),
+                 _ops.ptr(ws), _ops.stream())
+    else:
+        if isinstance(x0, dict):
+            x0 = A.vstate.flatten_tree(x0)
+        x0f = x0.to(torch.float64).contiguous()
+        nat.call("nka_cg_start_x0", _ops.ptr(a), a.stride(0), _ops.ptr(bf), n, float(tol), float(atol),
+                 int(maxiter), _ops.ptr(x0f), _ops.ptr(x), _ops.ptr(ws), _ops.stream())
     done = 0
     while True:
         nat.call("nka_cg_status", _ops.ptr(ws), n, _ops.ptr(status), _ops.stream())

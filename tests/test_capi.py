@@ -51,3 +51,4 @@ def test_argument_errors_are_reported_without_gpu():
     assert b"bad sizes" in lib.nka_last_error()
     st = lib.nka_gram_bf16x3(None, None, 10, 12, 12, 0.0, 0.0, None, 10, None)
     assert st == -1
+

@@ -146,6 +146,71 @@ def test_rbm_ntk_structured(gpu, dtype, ns, biases, N):
     assert np.all(err <= bound)
 
 
+@pytest.mark.parametrize("solver_name", ["cholesky", "cg"])
+def test_qgt_diag_scale(gpu, solver_name):
+    # scale-invariant regularisation (netket/optimizer/qgt/qgt_jacobian_common.py:21-60,
+    # qgt_jacobian_dense.py:93-161): the solution satisfies
+    # (S + diag_scale diag(S) + diag_shift I) x = F, and to_dense is that matrix
+    print(f"case qgt diag_scale {solver_name}", flush=True)
+    vs, H, p, conn = _state("ising", np.float64, 16, 1008, scale=0.2)
+    shift, dscale = 0.01, 0.1
+    qgt = nk.optimizer.QGTJacobianDense(vs, diag_shift=shift, diag_scale=dscale)
+    x = vs.samples.cpu().numpy().reshape(-1, vs.hilbert.size)
+    OL = ongd.centred_jacobian(orbm.jacobian(p, x))
+    S = OL.T @ OL
+    A = S + dscale * np.diag(np.diag(S)) + shift * np.eye(len(S))
+    np.testing.assert_allclose(qgt.to_dense().cpu().numpy(), A, rtol=1e-12, atol=1e-14)
+    F = np.random.default_rng(7).normal(size=len(S))
+    fun = gsolver.cholesky if solver_name == "cholesky" else gsolver.cg
+    kw = {} if solver_name == "cholesky" else {"tol": 1e-12}
+    sol, _ = qgt.solve(lambda A_, b_, x0=None: fun(A_, b_, x0=x0, **kw), torch.as_tensor(F, device="cuda"))
+    ref = np.linalg.solve(A, F)
+    np.testing.assert_allclose(sol.cpu().numpy(), ref, rtol=1e-8, atol=1e-10 * np.abs(ref).max())
+    if solver_name == "cg":
+        # the CG sees the rescaled system: the iterations of the oracle's CG on
+        # D^-1 S D^-1 + scale I, not those on S + scale diag(S) + shift I (which
+        # needs many more); one either way for the last-bit differences of S
+        d = np.sqrt(np.diag(S) + shift / dscale)
+        Ss = S / np.outer(d, d) + dscale * np.eye(len(S))
+        ys, k = ongd.cg(lambda v: Ss @ v, F / d, tol=1e-6)
+        _, k_unscaled = ongd.cg(lambda v: A @ v, F, tol=1e-6)
+        _, info = qgt.solve(lambda A_, b_, x0=None: gsolver.cg(A_, b_, x0=x0, tol=1e-6),
+                            torch.as_tensor(F, device="cuda"))
+        print(f"CG iterations: {info['iterations']} (oracle rescaled {k}, unscaled {k_unscaled})", flush=True)
+        assert abs(info["iterations"] - k) <= 1
+
+
+def test_cg_from_x0(gpu):
+    # jax.scipy.sparse.linalg.cg from a given x0: r0 = b - A x0, tolerance relative to b.b;
+    # SR with solver_restart=True passes the previous solution
+    n = 150
+    rng = np.random.default_rng(12)
+    A = rng.normal(size=(n, n))
+    A = A @ A.T / n + 0.5 * np.eye(n)
+    b = rng.normal(size=n)
+    x0 = np.linalg.solve(A, b) + 1e-3 * rng.normal(size=n)
+    x, info = gsolver.cg(torch.as_tensor(A, device="cuda"), torch.as_tensor(b, device="cuda"),
+                         x0=torch.as_tensor(x0, device="cuda"))
+    ref, k = ongd.cg(lambda v: A @ v, b, x0=x0)
+    x_cold, k_cold = ongd.cg(lambda v: A @ v, b)
+    assert info["iterations"] == k and k < k_cold
+    np.testing.assert_allclose(x.cpu().numpy(), ref, rtol=1e-8, atol=1e-10)
+    # an x0 that already solves the system: no iteration
+    xs = np.linalg.solve(A, b)
+    _, info0 = gsolver.cg(torch.as_tensor(A, device="cuda"), torch.as_tensor(b, device="cuda"),
+                          x0=torch.as_tensor(xs, device="cuda"))
+    assert info0["iterations"] == 0
+    # through SR with solver_restart=True: the second solve starts from the first solution
+    vs, H, p, conn = _state("ising", np.float64, 16, 1008, scale=0.2)
+    sr = nk.optimizer.SR(nk.optimizer.QGTJacobianDense, solver=gsolver.cg, diag_shift=0.01,
+                         solver_restart=True)
+    st, g = vs.expect_and_grad(H)
+    x1 = sr(vs, g)
+    it1 = sr.info["iterations"]
+    sr(vs, g)
+    assert sr.info["iterations"] < it1
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_prepare_large_model(gpu, dtype):
     # W larger than the LDS budget takes the tiled matrix-product path
@@ -204,7 +269,9 @@ def test_cholesky_large_many_panel_workgroups(gpu):
 
 
 def test_cholesky_failure_and_fallback(gpu):
-    # netket/_src/solvers/nan_fallback.py: NaN output triggers pinv_smooth
+    # netket/_src/solvers/nan_fallback.py: NaN output triggers pinv_smooth.
+    # The test is made on the device: the primary solution comes back zeroed
+    # and the flag set, with no host synchronisation inside the solver.
     n = 96
     rng = np.random.default_rng(3)
     U = rng.normal(size=(n, 40))
@@ -215,13 +282,52 @@ def test_cholesky_failure_and_fallback(gpu):
     assert np.all(np.isnan(x.cpu().numpy()))
     x2, info = gsolver.cholesky_with_fallback(torch.as_tensor(A, device="cuda"),
                                               torch.as_tensor(b, device="cuda"))
-    assert info["solver_fallback"] is True
-    np.testing.assert_allclose(x2.cpu().numpy(), ongd.pinv_smooth(A, b), rtol=1e-7, atol=1e-8)
-    # a good system does not fall back
+    fb = info["solver_fallback"]
+    assert isinstance(fb, gsolver.DeviceFallback)
+    assert int(fb.flag.item()) != 0 and bool(fb)
+    assert np.all(x2.cpu().numpy() == 0.0)
+    np.testing.assert_allclose(fb.solve().cpu().numpy(), ongd.pinv_smooth(A, b), rtol=1e-7, atol=1e-8)
+    # a NaN right-hand side also flags
+    bn = b.copy()
+    bn[5] = np.nan
     B = U @ U.T + np.eye(n) * 5.0
+    _, info_n = gsolver.cholesky_with_fallback(torch.as_tensor(B, device="cuda"),
+                                               torch.as_tensor(bn, device="cuda"))
+    assert bool(info_n["solver_fallback"])
+    # a good system does not fall back and is solved by the Cholesky
     x3, info3 = gsolver.cholesky_with_fallback(torch.as_tensor(B, device="cuda"),
                                                torch.as_tensor(b, device="cuda"))
-    assert info3["solver_fallback"] is False
+    assert not bool(info3["solver_fallback"])
+    np.testing.assert_allclose(x3.cpu().numpy(), np.linalg.solve(B, b), rtol=1e-9, atol=1e-10)
+
+
+@pytest.mark.parametrize("use_ntk", [False, True])
+def test_vmc_sr_fallback_resolved_by_driver(gpu, use_ntk):
+    # A diag_shift that makes the system indefinite: the device flag fires,
+    # the driver's eager step replaces the zeroed solution by pinv_smooth's,
+    # so the update equals the oracle's pinv_smooth update; a captured step
+    # records the same solver with no host read and applies no update.
+    print(f"case vmc_sr fallback use_ntk={use_ntk}", flush=True)
+    vs, H, p, conn = _state("ising", np.float64, 8, 64, scale=0.1)
+    samples = vs.samples.cpu().numpy()
+    x = samples.reshape(-1, samples.shape[-1])
+    eps = -1e3  # S + eps I (T + eps I) is negative definite: the factorisation fails
+    drv = nk.VMC_SR(H, nk.optimizer.Sgd(0.01), diag_shift=eps, variational_state=vs,
+                    use_ntk=use_ntk, linear_solver=gsolver.cholesky_with_fallback)
+    st, dp = drv.compute_loss_and_update()
+    assert isinstance(drv.info["solver_fallback"], gsolver.DeviceFallback)
+    dp = drv.resolve_fallback(dp)
+    assert drv.info["solver_fallback"] is True
+    e = orbm.local_energy(p, x, conn)
+    OL, dv = ongd.prepare(orbm.jacobian(p, x), e)
+    if use_ntk:
+        T = OL @ OL.T + eps * np.eye(OL.shape[0])
+        ref = OL.T @ ongd.pinv_smooth(T, dv)
+    else:
+        S = OL.T @ OL + eps * np.eye(OL.shape[1])
+        ref = ongd.pinv_smooth(S, OL.T @ dv)
+    got = dp.cpu().numpy()
+    np.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-9 * np.abs(ref).max())
 
 
 def test_cg(gpu):
