@@ -95,11 +95,22 @@ def metropolis(params, sigma0: np.ndarray, *, rule: str, clusters=None, n_discar
             for _ in range(sweep_size):
                 r = philox.draw(q, c, philox.TAG_METROPOLIS, seed)
                 q += 1
-                moves, corrs = local_moves(x) if rule == "local" else exchange_moves(x, clusters)
-                if not moves:
-                    continue
-                k = philox.bounded(r[0], len(moves))
-                xp, corr = moves[k], corrs[k]
+                if rule == "local":
+                    k = philox.bounded(r[0], N)
+                    xp = x.copy()
+                    xp[k] = -xp[k]
+                    corr = 0.0
+                else:
+                    # the k-th differing cluster (ascending cluster order), as exchange_moves lists them
+                    diff = np.nonzero(x[clusters[:, 0]] != x[clusters[:, 1]])[0]
+                    if diff.size == 0:
+                        continue
+                    k = philox.bounded(r[0], diff.size)
+                    u_, v_ = clusters[diff[k]]
+                    xp = x.copy()
+                    xp[u_], xp[v_] = x[v_], x[u_]
+                    n_new = int(np.count_nonzero(xp[clusters[:, 0]] != xp[clusters[:, 1]]))
+                    corr = np.log(diff.size) - np.log(n_new)
                 lpp = machine_pow * log_psi_(params, xp[None])[0]
                 u = philox.uniform01_f64(r)
                 if u < np.exp(lpp - lp + corr):

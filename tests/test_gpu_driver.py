@@ -22,22 +22,27 @@ def _tfim_state(L, n_chains, n_samples, n_discard, seed, alpha=1, dtype=np.float
 
 
 def test_vmc_sr_tfim_l16_converges(gpu):
-    # TFIM L=16 PBC h=J=1, FP64 RBM alpha=1, MetropolisLocal 16 chains,
-    # 1008 samples, 16 discarded sweeps, SR (QGTJacobianDense, diag_shift 0.01,
-    # Cholesky), SGD; the learning rate is raised from 0.01 to 0.05 so that 300
-    # steps reach the converged regime.
+    # The TFIM L=16 workload as configured: PBC h=J=1, FP64 RBM alpha=1,
+    # MetropolisLocal 16 chains, 1008 samples, 16 discarded sweeps, SR
+    # (QGTJacobianDense, diag_shift 0.01, Cholesky), SGD lr 0.01, seed 0.  The
+    # run is deterministic, so the measured errors of the 100-step window mean
+    # are asserted: about 2.6e-4 after the configured 300 steps, and within
+    # 1e-4 of the exact energy by step 1000 (first at step 576), 6.0e-5 at 3000.
     H, vs = _tfim_state(16, 16, 1008, 16, seed=0)
     e0 = O.tfim_chain_e0(16, 1.0)
     sr = nk.optimizer.SR(nk.optimizer.QGTJacobianDense, solver=gsolver.cholesky, diag_shift=0.01)
-    drv = nk.VMC(H, nk.optimizer.Sgd(0.05), variational_state=vs, preconditioner=sr)
+    drv = nk.VMC(H, nk.optimizer.Sgd(0.01), variational_state=vs, preconditioner=sr)
     log = nk.logging.RuntimeLog()
-    drv.run(300, out=log)
-    e = log.data["Energy"]["Mean"]
-    tail = float(np.mean(e[-50:]))
-    rel = abs(tail - e0) / abs(e0)
-    print(f"E(last 50) = {tail:.6f}, E0 = {e0:.6f}, rel err = {rel:.2e}", flush=True)
+    drv.run(3000, out=log)
+    e = np.asarray(log.data["Energy"]["Mean"])
+    rel = lambda k: abs(float(np.mean(e[k - 100:k])) - e0) / abs(e0)
+    first = next(k for k in range(100, 3001) if rel(k) <= 1e-4)
+    print(f"rel err: {rel(300):.2e} at 300, {rel(1000):.2e} at 1000, {rel(3000):.2e} at 3000; "
+          f"first within 1e-4 at step {first}", flush=True)
     assert e[0] > e[-1]
-    assert rel < 2e-3
+    assert rel(300) < 5e-4
+    assert first <= 1000
+    assert rel(3000) <= 1e-4
 
 
 def test_vmc_sr_and_minsr_runs_agree(gpu):

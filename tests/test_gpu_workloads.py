@@ -1,0 +1,351 @@
+"""Parity at the benchmark workloads' own sizes.
+
+* Heisenberg chain L=64, Sz=0, RBM alpha=2 (8384 parameters), exchange over
+  the 64 nearest-neighbour bonds, 4096 / 8192 chains: FP64 chains identical to
+  the oracle on the same Philox stream (these chain counts run two or four
+  chains per wavefront), the FP32 chain log-probabilities, and the bf16x3
+  Gram S (8384 x 8384, K = 16384 samples) on sampled entries.
+* J1-J2 10x10 (J2 = 0.5, no sign rule), network 100 -> 400 -> 400: the
+  connected elements (401 per sample) bit-exact, FP32 local energies, FP64
+  chains over the 400 NN + NNN exchange clusters, and the VMC_SR minSR update
+  at 512 samples against the oracle's explicit Jacobian (512 x 200,800).
+* TFIM 20x20, h = 3.04, RBM alpha=8 (3200 hidden units, 1,283,600
+  parameters): the tiled local energies, the structured NTK against the
+  oracle's explicit Jacobian product, and the chain log-probabilities of the
+  wide sampler.
+* The serving stream: a slice holding its largest job class (12x12, alpha=4,
+  8192 samples) and a 128-site chain gives each job the same result alone and
+  in the mix.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import netket_amd as nk
+from netket_amd import _engine as eng
+from netket_amd import serving
+from netket_amd.optimizer import solver as gsolver
+from oracle import ffnn as of
+from oracle import ngd as ongd
+from oracle import operators as O
+from oracle import rbm as orbm
+from oracle import sampler as osamp
+
+pytestmark = pytest.mark.gpu
+
+
+def _rbm(N, alpha, dtype, seed, scale):
+    model = nk.models.RBM(alpha=alpha, param_dtype=dtype)
+    lay = model.layout(N)
+    flat = np.random.default_rng(seed).normal(scale=scale, size=lay["n_params"]).astype(dtype)
+    return model, torch.as_tensor(flat, device="cuda"), orbm.RBMParams(flat.astype(np.float64), N, lay["M"])
+
+
+def _ffnn(N, H1, H2, dtype, seed, scale):
+    model = nk.models.FFNN(hidden_dims=(H1, H2), param_dtype=dtype)
+    n = model.layout(N)["n_params"]
+    flat = np.random.default_rng(seed).normal(scale=scale, size=n).astype(dtype)
+    return model, torch.as_tensor(flat, device="cuda"), of.FFNNParams(flat.astype(np.float64), N, H1, H2)
+
+
+def _spread(n_chains, k=16):
+    # both ends, wavefront and workgroup boundaries, and a few in between
+    ids = {0, 1, 2, 3, 31, 32, 63, 64, n_chains // 2 - 1, n_chains // 2, n_chains - 4, n_chains - 3,
+           n_chains - 2, n_chains - 1}
+    ids |= set(np.random.default_rng(n_chains).choice(n_chains, k - len(ids), replace=False).tolist())
+    return np.array(sorted(ids)[:k])
+
+
+def _sz0(N, n, seed):
+    rng = np.random.default_rng(seed)
+    x = np.ones((n, N), dtype=np.int8)
+    for b in range(n):
+        x[b, rng.choice(N, N // 2, replace=False)] = -1
+    return x
+
+
+# --------------------------------------------------------------------------
+# Heisenberg L=64, RBM alpha=2
+# --------------------------------------------------------------------------
+
+def _heis64(n_chains):
+    g = nk.graph.Chain(64)
+    hi = nk.hilbert.Spin(0.5, N=64, total_sz=0)
+    return g, hi, nk.sampler.MetropolisExchange(hi, graph=g, n_chains=n_chains)
+
+
+@pytest.mark.parametrize("n_chains", [4096, 8192])
+def test_heisenberg_l64_chains_identical_fp64(gpu, n_chains):
+    # 4096 chains: two chains per wavefront (M = 128 as 4 units per lane, the
+    # 64 clusters as 2 slots per lane); 8192: four per wavefront
+    print(f"case heisenberg L=64 chains fp64 n_chains={n_chains}", flush=True)
+    g, hi, sa = _heis64(n_chains)
+    model, flat, p = _rbm(64, 2, np.float64, 41, 0.05)
+    st = sa.init_state(model, flat, seed=17)
+    sigma0 = st.sigma.cpu().numpy().copy()
+    counter0 = st.counter
+    samples, st = sa.sample_fused(model, flat, st, 2, 2)
+    ids = _spread(n_chains)
+    ref, ref_sigma, ref_acc = osamp.metropolis(p, sigma0[ids], rule="exchange", clusters=sa.rule.clusters,
+                                               n_discard=2, chain_length=2, sweep_size=64, seed=17,
+                                               counter=counter0, chain_ids=ids)
+    np.testing.assert_array_equal(samples.cpu().numpy()[ids], ref)
+    np.testing.assert_array_equal(st.sigma.cpu().numpy()[ids], ref_sigma)
+    np.testing.assert_array_equal(st.n_accepted_proc.cpu().numpy()[ids], ref_acc)
+    np.testing.assert_allclose(st.log_prob.cpu().numpy()[ids], 2 * orbm.log_psi(p, ref_sigma), rtol=1e-12)
+    assert 0.05 < ref_acc.sum() / (len(ids) * 4 * 64) < 1.0
+
+
+def test_heisenberg_l64_fp32_log_prob(gpu):
+    # the timed workload's sampler instance (FP32, 4096 chains, 5 discarded and
+    # 4 kept sweeps of 64): every chain's running log-probability stays 2 log psi
+    # of its state, and the samples stay in the Sz = 0 sector
+    print("case heisenberg L=64 fp32 log_prob", flush=True)
+    g, hi, sa = _heis64(4096)
+    model, flat, p = _rbm(64, 2, np.float32, 42, 0.05)
+    st = sa.init_state(model, flat, seed=3)
+    samples, st = sa.sample_fused(model, flat, st, 5, 4)
+    s = samples.cpu().numpy()
+    assert np.all((s < 0).sum(axis=-1) == 32)
+    ref = 2 * orbm.log_psi(p, st.sigma.cpu().numpy())
+    lp = st.log_prob.cpu().numpy()
+    err = np.abs(lp - ref).max()
+    print(f"max |log_prob - 2 log psi| = {err:.2e} (|2 log psi| up to {np.abs(ref).max():.1f})", flush=True)
+    np.testing.assert_allclose(lp, ref, rtol=1e-5, atol=2e-4)
+    acc = st.acceptance
+    assert 0.05 < acc < 1.0
+
+
+def test_heisenberg_l64_gram_s8384(gpu):
+    # S = O_L^T O_L + 0.01 I from the bf16x3 Gram at the timed size (Np = 8384,
+    # K = 16384 samples: 66 tile rows, XCD remap, super-blocks and a partial
+    # last tile row), against the oracle's FP64 Jacobian on 400 sampled entries
+    # of the lower triangle plus 64 diagonal ones.  Bound as for the small
+    # case: |dS_pq| <= 3e-5 sqrt(S_pp S_qq) + 1e-9 (bf16x3 keeps ~16 bits)
+    print("case gram S 8384", flush=True)
+    g, hi, sa = _heis64(4096)
+    model = nk.models.RBM(alpha=2, param_dtype=np.float32)
+    vs = nk.vqs.MCState(sa, model, n_samples=16384, n_discard_per_chain=5, seed=0)
+    flat = np.random.default_rng(43).normal(scale=0.05, size=vs.n_parameters).astype(np.float32)
+    vs.parameters = torch.as_tensor(flat, device="cuda")
+    qgt = nk.optimizer.QGTJacobianDense(vs, diag_shift=0.01)
+    S = qgt.lower()
+    n = S.shape[0]
+    assert n == 8384
+    rng = np.random.default_rng(44)
+    a = rng.integers(0, n, size=400)
+    b = rng.integers(0, n, size=400)
+    ii = np.concatenate([np.maximum(a, b), rng.integers(0, n, size=64)])
+    jj = np.concatenate([np.minimum(a, b), ii[400:]])
+    ii[:8], jj[:8] = [n - 1, n - 1, n - 2, n - 65, 8383, 8320, 8319, 127], [0, n - 1, 5, 8256, 8256, 8319, 128, 127]
+    got = S[torch.as_tensor(ii, device="cuda"), torch.as_tensor(jj, device="cuda")].cpu().numpy()
+    x = vs.samples.cpu().numpy().reshape(-1, 64)
+    p = orbm.RBMParams(flat.astype(np.float64), 64, 128)
+    cols = np.unique(np.concatenate([ii, jj]))
+    Ofull = orbm.jacobian(p, x)
+    Oc = Ofull[:, cols] - Ofull[:, cols].mean(axis=0)
+    del Ofull
+    Oc /= np.sqrt(x.shape[0])
+    pos = {c: k for k, c in enumerate(cols)}
+    ci = np.array([pos[v] for v in ii])
+    cj = np.array([pos[v] for v in jj])
+    ref = np.einsum("si,si->i", Oc[:, ci], Oc[:, cj]) + 0.01 * (ii == jj)
+    dS = np.einsum("si,si->i", Oc, Oc)
+    bound = 3e-5 * np.sqrt(dS[ci] * dS[cj]) + 1e-9
+    err = np.abs(got - ref)
+    print(f"max |dS| / bound = {(err / bound).max():.3f}", flush=True)
+    assert np.all(err <= bound)
+    # the upper triangle is not written: the Cholesky reads only the lower one
+    assert ii.min() >= 0 and np.all(ii >= jj)
+
+
+# --------------------------------------------------------------------------
+# J1-J2 10x10, network 100 -> 400 -> 400
+# --------------------------------------------------------------------------
+
+def _j1j2_10():
+    g = nk.graph.Square(10, max_neighbor_order=2)
+    hi = nk.hilbert.Spin(0.5, N=100, total_sz=0)
+    H = nk.operator.Heisenberg(hi, g, J=[1.0, 0.5], sign_rule=[False, False])
+    terms = dict(zip([tuple(a) for a in H.acting_on], H.operators))
+    return g, hi, H, terms
+
+
+def test_j1j2_10x10_conn_bit_exact(gpu):
+    print("case j1j2 10x10 conn", flush=True)
+    g, hi, H, terms = _j1j2_10()
+    assert H.max_conn_size == 401 and len(H.acting_on) == 400
+    x = _sz0(100, 100, 51)
+    xp, mels, n = nk._ops.localop_conn(H, x, return_n_conn=True)
+    xp_o, mels_o, n_o = O.local_conn(x, terms, constant=float(H.constant))
+    np.testing.assert_array_equal(xp.cpu().numpy(), xp_o)
+    np.testing.assert_array_equal(mels.cpu().numpy(), mels_o)
+    np.testing.assert_array_equal(n.cpu().numpy(), n_o)
+
+
+def test_j1j2_10x10_ffnn_local_energy_fp32(gpu):
+    print("case j1j2 10x10 ffnn eloc fp32", flush=True)
+    g, hi, H, terms = _j1j2_10()
+    model, flat, p = _ffnn(100, 400, 400, np.float32, 52, 0.05)
+    x = _sz0(100, 64, 53)
+    e, lp = eng.local_energies(model, flat, H, torch.as_tensor(x, device="cuda"), log_psi=True)
+    ref = of.local_energy(p, x, lambda z: O.local_conn(z, terms))
+    err = np.abs(e.cpu().numpy() - ref).max() / np.abs(ref).max()
+    print(f"max rel err {err:.2e}", flush=True)
+    np.testing.assert_allclose(e.cpu().numpy(), ref, rtol=1e-4, atol=1e-4 * np.abs(ref).max())
+    np.testing.assert_allclose(lp.cpu().numpy(), of.log_psi(p, x), rtol=2e-5)
+
+
+def test_j1j2_10x10_ffnn_chains_identical_fp64(gpu):
+    # 8192 chains of the 100 -> 400 -> 400 network over the 400 NN + NNN
+    # clusters; eight chains across the batch followed by the oracle
+    print("case j1j2 10x10 ffnn chains fp64", flush=True)
+    g, hi, H, terms = _j1j2_10()
+    sa = nk.sampler.MetropolisExchange(hi, graph=g, n_chains=8192)
+    assert sa.rule.clusters.shape == (400, 2)
+    model, flat, p = _ffnn(100, 400, 400, np.float64, 54, 0.05)
+    st = sa.init_state(model, flat, seed=19)
+    sigma0 = st.sigma.cpu().numpy().copy()
+    counter0 = st.counter
+    samples, st = sa.sample_fused(model, flat, st, 1, 1)
+    ids = np.array([0, 1, 31, 32, 4095, 4096, 8190, 8191])
+    ref, ref_sigma, ref_acc = osamp.metropolis(p, sigma0[ids], rule="exchange", clusters=sa.rule.clusters,
+                                               n_discard=1, chain_length=1, sweep_size=100, seed=19,
+                                               counter=counter0, chain_ids=ids, log_psi_fn=of.log_psi)
+    np.testing.assert_array_equal(samples.cpu().numpy()[ids], ref)
+    np.testing.assert_array_equal(st.sigma.cpu().numpy()[ids], ref_sigma)
+    np.testing.assert_array_equal(st.n_accepted_proc.cpu().numpy()[ids], ref_acc)
+    np.testing.assert_allclose(st.log_prob.cpu().numpy()[ids], 2 * of.log_psi(p, ref_sigma), rtol=1e-12)
+
+
+def test_j1j2_10x10_ffnn_fp32_log_prob(gpu):
+    print("case j1j2 10x10 ffnn fp32 log_prob", flush=True)
+    g, hi, H, terms = _j1j2_10()
+    sa = nk.sampler.MetropolisExchange(hi, graph=g, n_chains=8192)
+    model, flat, p = _ffnn(100, 400, 400, np.float32, 55, 0.05)
+    st = sa.init_state(model, flat, seed=4)
+    samples, st = sa.sample_fused(model, flat, st, 5, 2)
+    assert np.all((samples.cpu().numpy() < 0).sum(axis=-1) == 50)
+    ref = 2 * of.log_psi(p, st.sigma.cpu().numpy())
+    np.testing.assert_allclose(st.log_prob.cpu().numpy(), ref, rtol=1e-5, atol=2e-4)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_j1j2_10x10_minsr_update_ns512(gpu, dtype):
+    # VMC_SR(use_ntk=True), structured NTK 512 x 512, diag_shift 1e-4, Cholesky,
+    # against the oracle's minSR update from the explicit FP64 Jacobian
+    # (512 x 200,800) and FP64 local energies on the same samples and the same
+    # (rounded) parameters.  FP64: 1e-8 of the update's scale.  FP32: the
+    # local energies carry ~1e-5 relative error and the solve amplifies it by
+    # up to |T| / eps, so the bound is on the relative norm of the difference.
+    print(f"case j1j2 10x10 minsr update {np.dtype(dtype)}", flush=True)
+    g, hi, H, terms = _j1j2_10()
+    sa = nk.sampler.MetropolisExchange(hi, graph=g, n_chains=256)
+    model = nk.models.FFNN(hidden_dims=(400, 400), param_dtype=dtype)
+    vs = nk.vqs.MCState(sa, model, n_samples=512, n_discard_per_chain=5, seed=6)
+    flat = np.random.default_rng(56).normal(scale=0.05, size=vs.n_parameters).astype(dtype)
+    vs.parameters = torch.as_tensor(flat, device="cuda")
+    assert vs.n_parameters == 200800
+    drv = nk.VMC_SR(H, nk.optimizer.Sgd(0.01), diag_shift=1e-4, variational_state=vs, use_ntk=True,
+                    linear_solver=gsolver.cholesky)
+    st, dp = drv.compute_loss_and_update()
+    x = vs.samples.cpu().numpy().reshape(-1, 100)
+    p = of.FFNNParams(flat.astype(np.float64), 100, 400, 400)
+    e = of.local_energy(p, x, lambda z: O.local_conn(z, terms))
+    OL, dv = ongd.prepare(of.jacobian(p, x), e)
+    ref, _, _ = ongd.srt_update(OL, dv, 1e-4)
+    got = dp.cpu().numpy()
+    rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    print(f"|dp - ref| / |ref| = {rel:.2e}, |ref| = {np.linalg.norm(ref):.3e}", flush=True)
+    if dtype == np.float64:
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-8 * np.abs(ref).max())
+    else:
+        assert rel < 2e-3
+    assert st.mean == pytest.approx(float(np.mean(e)), rel=1e-10 if dtype == np.float64 else 1e-5)
+
+
+# --------------------------------------------------------------------------
+# TFIM 20x20, RBM alpha=8
+# --------------------------------------------------------------------------
+
+def _tfim20():
+    g = nk.graph.Square(20)
+    hi = nk.hilbert.Spin(0.5, N=400)
+    return g, hi, nk.operator.Ising(hi, g, h=3.04, J=1.0, dtype=np.float32)
+
+
+def test_tfim_20x20_local_energy_tiled(gpu):
+    print("case tfim 20x20 eloc tiled", flush=True)
+    g, hi, H = _tfim20()
+    model, flat, p = _rbm(400, 8, np.float32, 61, 0.02)
+    x = np.random.default_rng(62).choice(np.array([-1, 1], dtype=np.int8), size=(64, 400))
+    e, lp = eng.local_energies(model, flat, H, torch.as_tensor(x, device="cuda"), log_psi=True)
+    ref = orbm.local_energy(p, x, lambda z: O.ising_conn(z, H.edges, 3.04, 1.0, np.float32))
+    err = np.abs(e.cpu().numpy() - ref).max() / np.abs(ref).max()
+    print(f"max rel err {err:.2e}", flush=True)
+    np.testing.assert_allclose(e.cpu().numpy(), ref, rtol=1e-5, atol=1e-5 * np.abs(ref).max())
+    np.testing.assert_allclose(lp.cpu().numpy(), orbm.log_psi(p, x), rtol=1e-5, atol=1e-4)
+
+
+def test_tfim_20x20_ntk_structured_ns128(gpu):
+    # T = O_c O_c^T / ns + 1e-4 I from the RBM structure (t.t on the bf16x3
+    # tiles, sigma.sigma exact) against the oracle's explicit Jacobian
+    # (128 x 1,283,600); bound as in the small structured-NTK cases
+    print("case tfim 20x20 ntk ns=128", flush=True)
+    ns, N = 128, 400
+    model, flat, p = _rbm(N, 8, np.float32, 63, 0.02)
+    x = np.random.default_rng(64).choice(np.array([-1, 1], dtype=np.int8), size=(ns, N))
+    sig = torch.as_tensor(x, device="cuda")
+    t = eng.prepare(model, flat, sig)[0]
+    ones = torch.full((1, ns), 1.0 / ns, dtype=torch.float64, device="cuda")
+    mean = eng.contract(model, flat, sig, t, ones)[0]
+    T = eng.rbm_ntk(model, flat, sig, t, mean, 1.0 / ns, 1e-4, 0.0).cpu().numpy()
+    Ou = orbm.jacobian(p, x)
+    assert Ou.shape == (128, 1283600)
+    Oc = Ou - Ou.mean(axis=0)
+    ref = Oc @ Oc.T / ns + 1e-4 * np.eye(ns)
+    lo = np.tril_indices(ns)
+    dc = np.sqrt(np.sum(Oc * Oc, axis=1) / ns)
+    du = np.sqrt(np.sum(Ou * Ou, axis=1) / ns)
+    bound = (2e-6 * np.outer(dc, dc) + 3e-5 * np.outer(du, du))[lo] + 1e-15
+    err = np.abs(T[lo] - ref[lo])
+    print(f"max |dT| / bound = {(err / bound).max():.2e}", flush=True)
+    assert np.all(err <= bound)
+
+
+def test_tfim_20x20_wide_sampler_log_prob(gpu):
+    # the wide local-rule kernel (W = 5.1 MB does not fit in LDS, 56 units
+    # per lane): 8192 chains, one discarded and one kept sweep of 400
+    print("case tfim 20x20 wide sampler", flush=True)
+    g, hi, H = _tfim20()
+    sa = nk.sampler.MetropolisLocal(hi, n_chains=8192)
+    model, flat, p = _rbm(400, 8, np.float32, 65, 0.02)
+    st = sa.init_state(model, flat, seed=8)
+    samples, st = sa.sample_fused(model, flat, st, 1, 1)
+    ref = 2 * orbm.log_psi(p, st.sigma.cpu().numpy())
+    lp = st.log_prob.cpu().numpy()
+    print(f"max |log_prob - 2 log psi| = {np.abs(lp - ref).max():.2e}", flush=True)
+    np.testing.assert_allclose(lp, ref, rtol=1e-5, atol=1e-3)
+    assert 0.02 < st.acceptance < 1.0
+
+
+# --------------------------------------------------------------------------
+# Serving: the largest job class and a 128-site chain, alone and in the mix
+# --------------------------------------------------------------------------
+
+def test_serving_largest_jobs_same_alone_and_in_mix(gpu):
+    print("case serving slice with 12x12 alpha=4 and L=128", flush=True)
+    specs = serving.make_stream(n_jobs=5, rate=1000.0, seed=11, n_steps=4)
+    big = [(2, 12, 4, 8192), (1, 128, 4, 4096), (1, 16, 1, 1024), (2, 4, 2, 1024), (1, 64, 2, 4096)]
+    for s, (dim, L, alpha, ns) in zip(specs, big):
+        s.dim, s.L, s.alpha, s.n_samples = dim, L, alpha, ns
+    mix = serving.Server(hbm_budget_gb=100, n_streams=4, use_graph=True, keep_params=True).run(
+        specs, time_scale=0.0)
+    assert mix.max_concurrent == len(specs)
+    for target in specs[:2]:
+        alone = serving.Server(hbm_budget_gb=100, n_streams=1, use_graph=True, keep_params=True).run(
+            [target], time_scale=0.0)
+        np.testing.assert_array_equal(alone.final_params[target.job_id], mix.final_params[target.job_id])
+        assert alone.final_energies[target.job_id] == mix.final_energies[target.job_id]
+        assert np.isfinite(mix.final_energies[target.job_id])

@@ -87,6 +87,33 @@ def test_heisenberg_L22_sz0_reference_energy():
     assert O.lanczos(Hs)[0] == pytest.approx(-39.147522607062776, abs=1e-9)
 
 
+@pytest.mark.parametrize("L", [4, 8, 10, 12, 14])
+def test_bethe_ansatz_equals_exact_diagonalisation(L):
+    # the Bethe-equation energy (oracle/bethe.py) against Lanczos on the
+    # restated operator in the Sz = 0 sector, and the tutorial's L = 22 value
+    import netket_amd as nk
+    from oracle import bethe
+
+    g = nk.graph.Chain(L)
+    hi = nk.hilbert.Spin(0.5, N=L, total_sz=0)
+    H = nk.operator.Heisenberg(hi, g)
+    terms = dict(zip([tuple(a) for a in H.acting_on], H.operators))
+    st = O.states_all(L, n_down=L // 2)
+    Hs = O.sparse_from_conn(st, lambda x: O.local_conn(x, terms))
+    assert bethe.heisenberg_e0(L) == pytest.approx(O.lanczos(Hs)[0], abs=1e-9)
+
+
+def test_bethe_ansatz_l22_and_l64():
+    from oracle import bethe
+
+    assert bethe.heisenberg_e0(22) == pytest.approx(-39.147522607062776, abs=1e-10)
+    # the L = 64 value the benchmark compares with; per site it approaches
+    # 4 (1/4 - ln 2) from below as 1/L^2 (finite-size correction)
+    e64 = bethe.heisenberg_e0(64)
+    assert e64 == pytest.approx(-113.49720554141149, abs=1e-9)
+    assert 4 * (0.25 - np.log(2)) - 0.01 < e64 / 64 < 4 * (0.25 - np.log(2))
+
+
 def test_local_conn_equals_kronecker_elementwise():
     rng = np.random.default_rng(1)
     N = 6
