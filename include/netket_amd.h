@@ -356,6 +356,22 @@ int nka_potrf_f64(double* a, int64_t n, int64_t lda, void* workspace, int32_t* i
 int nka_potrs_f64(const double* a, int64_t n, int64_t lda, const void* workspace, double* b,
                   const int32_t* info, void* stream);
 
+/* Cholesky factorisation and solve in one call with no allocation and no
+ * caller workspace: a = L L^T in place (lower triangle) and b <- A^{-1} b, as
+ * nka_potrf_f64 followed by nka_potrs_f64.  No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Its scratch is a static
+ * device array of fixed size, one per GPU, held until the process ends:
+ * n <= nka_cholesky_solve_noalloc_max_n() (32768, about 17 MB); larger n is
+ * rejected with NKA_ERR_ARG.  All calls on one GPU share that scratch, so
+ * calls that overlap on one GPU (different streams, no ordering) race.
+ * info is cleared by the call, then set as by nka_potrf_f64; b becomes NaN
+ * when the factorisation fails.  Results equal those of the workspace pair up
+ * to the summation order of the split tail tiles of the trailing products. */
+int64_t nka_cholesky_solve_noalloc_max_n(void);
+int nka_cholesky_solve_f64_noalloc(double* a, int64_t n, int64_t lda, double* b, int32_t* info,
+                                   void* stream);
+
 /* x <- src (or 0 when src is NULL) if *flag != 0, else unchanged: the
  * device-side branch of the nan_fallback solver
  * (netket/_src/solvers/nan_fallback.py:34-38, jax.lax.cond on the flag). */

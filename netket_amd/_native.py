@@ -132,6 +132,8 @@ SIGNATURES = {
     "nka_potrs_f64": (i32, [vp, i64, i64, vp, vp, vp, vp]),
     "nka_symmetrize_f64": (i32, [vp, i64, i64, vp]),
     "nka_check_finite": (i32, [vp, i64, vp, vp]),
+    "nka_cholesky_solve_noalloc_max_n": (i64, []),
+    "nka_cholesky_solve_f64_noalloc": (i32, [vp, i64, i64, vp, vp, vp]),
     "nka_select_on_flag_f64": (i32, [vp, vp, vp, i64, vp]),
     "nka_sgd_update": (i32, [vp, i32, vp, i64, f64, vp]),
     "nka_centre_scale": (i32, [vp, i64, vp, f64, vp, vp]),

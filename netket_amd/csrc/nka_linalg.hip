@@ -16,6 +16,7 @@
 //                  vector helpers of the step.
 #include "nka_common.hpp"
 #include "netket_amd.h"
+#include <mutex>
 
 namespace nka {
 
@@ -1777,6 +1778,47 @@ int nka_potrs_f64(const double* a, int64_t n, int64_t lda, const void* workspace
 
 }  // extern "C"
 
+// ---------------------------------------------------------------------------
+// Factor and solve with no allocation and no caller workspace.  The scratch
+// (inverses of the diagonal blocks, the intermediate vector, the flags and
+// the error word of the solves) is a static device array of a fixed size;
+// host code reaches it only through the address hipGetSymbolAddress returns
+// for the current device.  Trailing products run unsplit here (the split of
+// tail tiles would need up to 128 MiB more), so the last bits can differ from
+// nka_potrf_f64 + nka_potrs_f64 by the order of the K sums of those tiles.
+// ---------------------------------------------------------------------------
+constexpr int64_t kNoAllocMaxN = 32768;
+constexpr int64_t kNoAllocNblk = (kNoAllocMaxN + CH_NB - 1) / CH_NB;
+constexpr int64_t kNoAllocHead = kNoAllocNblk * CH_NB * CH_NB * (int64_t)sizeof(double) +
+                                 (kNoAllocMaxN + 64) * (int64_t)sizeof(double) +
+                                 (kNoAllocNblk + 64) * (int64_t)sizeof(uint32_t);
+constexpr int64_t kNoAllocBytes = (kNoAllocHead + 255) / 256 * 256;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64];
+
+// The device address of `scratch` on the current device, looked up once per device.
+static void* scratch_on_current_device() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+    set_error("nka_cholesky_solve_f64_noalloc: no current HIP device");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("nka_cholesky_solve_f64_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+
 // Scale-invariant regularisation of the QGT (netket/optimizer/qgt/qgt_jacobian_common.py:21-60):
 // d_k = sqrt(S_kk + offset) from the diagonal of the unregularised S.
 __global__ void qgt_diag_scale_kernel(const double* __restrict__ a, int64_t n, int64_t lda, double offset,
@@ -1816,6 +1858,29 @@ __global__ void select_on_flag_kernel(const int32_t* __restrict__ flag, const do
 }
 
 extern "C" {
+
+int64_t nka_cholesky_solve_noalloc_max_n(void) { return kNoAllocMaxN; }
+
+int nka_cholesky_solve_f64_noalloc(double* a, int64_t n, int64_t lda, double* b, int32_t* info,
+                                   void* stream) {
+  NKA_CHECK_ARG(n >= 0 && lda >= n && info && (n == 0 || (a && b)),
+                "nka_cholesky_solve_f64_noalloc: bad arguments");
+  NKA_CHECK_ARG(n <= kNoAllocMaxN, "nka_cholesky_solve_f64_noalloc: n = %lld exceeds the maximum %lld of the "
+                "static scratch; use nka_potrf_f64 / nka_potrs_f64 with a workspace",
+                (long long)n, (long long)kNoAllocMaxN);
+  void* ws = scratch_on_current_device();
+  if (!ws) return NKA_ERR_HIP;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, (uint32_t*)info, (int64_t)1);
+  if (n == 0) {
+    NKA_CHECK_LAUNCH("nka_cholesky_solve_f64_noalloc");
+    return NKA_OK;
+  }
+  potrf_run(a, n, lda, ws, nullptr, 0, info, s);
+  potrs_run(a, n, lda, ws, b, info, s);
+  NKA_CHECK_LAUNCH("nka_cholesky_solve_f64_noalloc");
+  return NKA_OK;
+}
 
 int nka_qgt_rescale_f64(double* a, int64_t n, int64_t lda, double offset, double shift, int mode,
                         double* scale, void* stream) {

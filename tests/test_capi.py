@@ -52,3 +52,15 @@ def test_argument_errors_are_reported_without_gpu():
     st = lib.nka_gram_bf16x3(None, None, 10, 12, 12, 0.0, 0.0, None, 10, None)
     assert st == -1
 
+
+def test_noalloc_cholesky_rejects_sizes_beyond_its_scratch():
+    # the size check runs on the host, before the scratch lookup or any launch
+    from netket_amd import _native
+
+    lib = _native.lib()
+    nmax = lib.nka_cholesky_solve_noalloc_max_n()
+    assert nmax == 32768
+    fake = ctypes.c_void_p(256)  # never dereferenced: the call returns before any launch
+    st = lib.nka_cholesky_solve_f64_noalloc(fake, nmax + 1, nmax + 1, fake, fake, None)
+    assert st == -1
+    assert b"exceeds the maximum" in lib.nka_last_error()
