@@ -11,10 +11,20 @@ Synthetic data: parameters drawn N(0, 0.01^2) from seed 0, chains from the
 sampler.  A step is sample -> E_loc -> statistics -> Jacobian -> S -> solve
 -> update, all on the GPU.
 
-Reported beside it: per-stage GPU times from HIP events on the launch stream,
-the roofline of the dominant kernel, the CPU restatement of the reference
-algorithm (FP64 NumPy/LAPACK) timed on one step of the same workload, and an
-accuracy run (TFIM L=16, outside the timed region) against the exact energy.
+Reported beside it, each outside the timed region:
+* per-stage GPU times from HIP events on the launch stream, and the roofline
+  of the largest kernel;
+* the plain FP64 GPU step (the reference algorithm in PyTorch FP64,
+  bench_plain_fp64.py) on the same workload: ``vs_baseline`` is the speed-up
+  over it;
+* accuracy: the TFIM L=16 workload (FP64) against its exact energy, run by
+  both implementations for the same number of steps; the L=64 workload's
+  energy after 500 steps against the Bethe-ansatz ground-state energy;
+* the CPU restatement of the reference algorithm (FP64 NumPy/LAPACK) on one
+  step of the same workload; its local energies, S and solution on the GPU
+  step's own samples give the FP64-oracle error of each GPU result;
+* the J1-J2 10x10 network minSR step, the TFIM 20x20 alpha=8 minSR step and
+  the 256-job serving stream, each timed in this same run.
 """
 
 from __future__ import annotations
@@ -29,6 +39,10 @@ import time
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 FP64_PEAK_TFLOPS = 78.6
+# Ground-state energy of the 64-site periodic Heisenberg chain (Pauli matrices,
+# J = 1) from the Bethe equations; the same computation reproduces exact
+# diagonalisation for L <= 14 and the tutorial's L = 22 value to 1e-12.
+HEIS_L64_E0 = -113.49720554141149
 
 
 def build_heisenberg_l64(seed=0):
@@ -92,8 +106,9 @@ def build_tfim2d_rbm(seed=0):
     return drv, H, vs
 
 
-def run_workload(name, steps, warmup):
-    """Time one of the larger workloads (eager steps) and print its own JSON line."""
+def run_workload(name, steps, warmup, norms=False):
+    """Time one of the larger workloads (eager steps); returns its record.  With
+    ``norms`` the energy, gradient and update norms of every step are added."""
     import torch
 
     from netket_amd import _ops
@@ -112,7 +127,18 @@ def run_workload(name, steps, warmup):
     _ops.TIMER.reset()
     _ops.TIMER.enabled = True
     t0 = time.perf_counter()
-    stats = [step(drv) for _ in range(steps)]
+    if norms:
+        stats, grad_n, upd_n = [], [], []
+        for _ in range(steps):
+            drv.record = {}
+            stats.append(step(drv))
+            rec = drv.record
+            # |grad| = |O_L^T dv| (the SR right-hand side without the shift), |update|
+            grad_n.append(rec["rhs"].norm())
+            upd_n.append(rec["dp"].norm())
+        drv.record = None
+    else:
+        stats = [step(drv) for _ in range(steps)]
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     _ops.TIMER.enabled = False
@@ -135,10 +161,16 @@ def run_workload(name, steps, warmup):
         "energy_per_step": [s.mean for s in stats],
         "acceptance": vs.sampler_state.acceptance,
     }
-    print(json.dumps(out))
+    if norms:
+        out["dv_norm_per_step"] = [float(v) for v in grad_n]
+        out["update_norm_per_step"] = [float(v) for v in upd_n]
+    del drv, vs
+    _ops.WORKSPACE.clear()
+    torch.cuda.empty_cache()
+    return out
 
 
-def run_serving(n_jobs, n_steps, rate=4.0):
+def run_serving(n_jobs, n_steps, rate=4.0, verbose=True):
     """The serving workload: TFIM jobs of uneven size arriving as a Poisson
     stream (rate jobs/s), each running n_steps SR / minSR steps, admitted by
     HBM footprint and stepped as captured HIP graphs on per-job streams."""
@@ -146,7 +178,8 @@ def run_serving(n_jobs, n_steps, rate=4.0):
 
     specs = serving.make_stream(n_jobs=n_jobs, rate=rate, seed=0, n_steps=n_steps)
     def progress(done, active, pending):
-        print(f"serving: {done} done, {active} active, {pending} pending", file=sys.stderr, flush=True)
+        if verbose:
+            print(f"serving: {done} done, {active} active, {pending} pending", file=sys.stderr, flush=True)
 
     rep = serving.Server(hbm_budget_gb=200.0, n_streams=8, use_graph=True, max_active=64).run(
         specs, on_progress=progress)
@@ -169,7 +202,7 @@ def run_serving(n_jobs, n_steps, rate=4.0):
                                f"poisson {rate}/s arrivals, samples/step in {{1024,4096,8192}}",
                    "n_jobs": n_jobs, "n_steps": n_steps, "rate": rate},
     }
-    print(json.dumps(out))
+    return out
 
 
 def step(drv):
@@ -182,10 +215,10 @@ def step(drv):
 
 def accuracy_run(n_steps=3000, n_config=300, window=100):
     """The TFIM L=16 workload (FP64 RBM alpha=1, 16 chains, 1008 samples, 16 discarded
-    sweeps, SR diag_shift 0.01 with Cholesky, SGD lr 0.01) against the exact energy:
-    the relative error of the mean energy over the last `window` steps at the
-    configured n_config steps, and after continuing the same run to n_steps, with
-    the first step at which the window mean is within 1e-4 of the exact energy."""
+    sweeps, SR diag_shift 0.01 with Cholesky, SGD lr 0.01, seed 0) against the exact
+    energy, as captured steps: the relative error of the mean energy over the last
+    `window` steps at the configured n_config steps and at n_steps, and the first
+    step at which the window mean is within 1e-4 of the exact energy."""
     import numpy as np
     import torch
 
@@ -205,14 +238,18 @@ def accuracy_run(n_steps=3000, n_config=300, window=100):
     drv = nk.VMC(H, nk.optimizer.Sgd(lr), variational_state=vs, preconditioner=sr)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    stats = []
-    for _ in range(n_steps):
-        stats.append(step(drv))
+    drv.capture_step(warmup=2)
+    stats = list(drv.warmup_stats)
     torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    for _ in range(n_steps - len(stats)):
+        stats.append(drv.replay_step())
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t1
     e = np.array([s.mean for s in stats])
     rel = lambda k: abs(float(np.mean(e[k - window:k])) - e0) / abs(e0)
     reached = next((k for k in range(window, n_steps + 1) if rel(k) <= 1e-4), None)
+    sps = (n_steps - 2) / dt
     return {
         "workload": "tfim-1d-L16-h1 fp64-rbm-a1 metropolis-local 16ch 1008smp sr-cholesky "
                     f"eps=0.01 sgd-lr={lr}",
@@ -222,16 +259,126 @@ def accuracy_run(n_steps=3000, n_config=300, window=100):
         "rel_err": rel(n_steps),
         "window": window,
         "first_step_within_1e-4": reached,
-        "steps_per_s": n_steps / dt,
+        "steps_per_s": sps,
+        "time_to_1e-4_s": (reached / sps) if reached else None,
+        "captured_steps": True,
+        "setup_and_capture_s": t1 - t0,
     }
 
 
-def cpu_baseline_heisenberg_l64(budget_s: float = 25.0):
-    """One CPU step of the reference algorithm (FP64) on the same workload."""
+def heisenberg_l64_energy(n_steps=500, window=50):
+    """The timed workload run for its configured 500 steps (captured steps, seed 0):
+    the mean energy of the last `window` steps against the Bethe-ansatz E0."""
+    import numpy as np
+    import torch
+
+    drv, H, vs = build_heisenberg_l64(seed=0)
+    drv.capture_step(warmup=2)
+    stats = list(drv.warmup_stats)
+    for _ in range(n_steps - len(stats)):
+        stats.append(drv.replay_step())
+    torch.cuda.synchronize()
+    e = np.array([s.mean for s in stats])
+    tail = float(np.mean(e[-window:]))
+    err = float(np.std(e[-window:]) / math.sqrt(window))
+    return {"workload": "heisenberg-1d-L64-sz0 (the timed workload), seed 0, captured steps",
+            "steps": n_steps, "exact_e0_bethe": HEIS_L64_E0, f"energy_mean_last_{window}": tail,
+            "spread_of_mean": err, "rel_err": abs(tail - HEIS_L64_E0) / abs(HEIS_L64_E0),
+            "energy_every_50_steps": [float(v) for v in e[::50]] + [float(e[-1])],
+            "acceptance": vs.sampler_state.acceptance}
+
+
+def plain_fp64_heisenberg_l64(steps=3, warmup=1):
+    """SR steps/s of the plain FP64 GPU implementation on the timed workload."""
+    import torch
+
+    from bench_plain_fp64 import PlainRBMVMC
+
+    v = PlainRBMVMC(64, 2, 4096, 16384, 5, ("heisenberg", 1.0), 0.01, 0.01, seed=0)
+    for _ in range(warmup):
+        v.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    es = [v.step() for _ in range(steps)]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out = {"value": steps / dt, "unit": "SR steps/s", "ms_per_step": dt / steps * 1e3, "steps": steps,
+           "energy_last": float(es[-1])}
+    del v
+    torch.cuda.empty_cache()
+    return out
+
+
+def plain_fp64_tfim16(max_steps=3000, budget_s=60.0, window=100):
+    """The plain FP64 GPU implementation on the TFIM L=16 workload: steps/s, and
+    the first step whose window mean is within 1e-4 of the exact energy (run
+    until then, up to max_steps or budget_s)."""
+    import numpy as np
+    import torch
+
+    from bench_plain_fp64 import PlainRBMVMC
+
+    L = 16
+    e0 = -float(np.sum(np.sqrt(2.0 - 2.0 * np.cos(np.pi * (2 * np.arange(L) + 1) / L))))
+    v = PlainRBMVMC(L, 1, 16, 1008, 16, ("ising", 1.0, 1.0), 0.01, 0.01, seed=0)
+    es = [v.step()]  # includes the capture of the sampler
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reached = None
+    k = 1
+    while k < max_steps and time.perf_counter() - t0 < budget_s:
+        chunk = [v.step() for _ in range(100)]
+        es.extend(chunk)
+        k += len(chunk)
+        e = torch.stack(es).cpu().numpy()
+        for j in range(max(window, k - 100), k + 1):
+            if abs(float(np.mean(e[j - window:j])) - e0) / abs(e0) <= 1e-4:
+                reached = j
+                break
+        if reached:
+            break
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    e = torch.stack(es).cpu().numpy()
+    sps = (k - 1) / dt
+    del v
+    torch.cuda.empty_cache()
+    return {"steps_run": k, "steps_per_s": sps, "first_step_within_1e-4": reached,
+            "rel_err_last_window": abs(float(np.mean(e[-window:])) - e0) / abs(e0),
+            "time_to_1e-4_s": (reached / sps) if reached else None}
+
+
+def record_step(drv):
+    """One eager step's inputs and results, copied to the host: the samples,
+    parameters, local energies, S (lower triangle), F and the solution."""
+    import numpy as np
+
+    drv.record = {}
+    drv.reset_step()
+    st, dp = drv.compute_loss_and_update()
+    rec = drv.record
+    drv.record = None
+    S = rec["system"].lower()
+    out = {"samples": drv.state.samples.cpu().numpy().reshape(-1, drv.state.hilbert.size),
+           "params": drv.state.parameters_flat.cpu().numpy().astype(np.float64),
+           "eloc": rec["eloc"].cpu().numpy(), "S": np.tril(S.cpu().numpy()),
+           "F": rec["rhs"].cpu().numpy(), "x": rec["solution"].cpu().numpy()}
+    del S
+    drv.update_parameters(dp)
+    return out
+
+
+def cpu_baseline_heisenberg_l64(rec, sample_budget_s: float = 30.0):
+    """One CPU step of the reference algorithm (FP64) on the timed workload: the
+    sampler on its own 4096 chains (5 + 4 sweeps of 64 full-forward exchange
+    proposals), then local energies, Jacobian, S and the Cholesky solve on the
+    GPU step's samples and parameters.  The stage results on those samples are
+    the FP64 oracle of the GPU step's local energies, S and solution."""
     import numpy as np
 
     import netket_amd as nk
     from oracle import cpu_baseline as cb
+    from oracle.rbm import RBMParams
 
     L, M = 64, 128
     g = nk.graph.Chain(L)
@@ -240,25 +387,32 @@ def cpu_baseline_heisenberg_l64(budget_s: float = 25.0):
     terms = dict(zip([tuple(a) for a in H.acting_on], H.operators))
     clusters = nk.sampler.compute_clusters(g, 1).astype(np.int64)
     rng = np.random.default_rng(0)
-    flat = rng.normal(scale=0.01, size=M + L * M + L)
     C = 4096
     sigma = np.ones((C, L), dtype=np.int8)
     for c in range(C):
         sigma[c, rng.permutation(L)[: L // 2]] = -1
     timings = {}
-    n = 0
+    flat = rec["params"]
     t0 = time.perf_counter()
-    while True:
-        ts = time.perf_counter()
-        flat, sigma, e = cb.vmc_sr_step(flat, L, M, sigma, terms, clusters, n_discard=5,
-                                        chain_length=4, sweep_size=L, diag_shift=0.01, lr=0.01,
-                                        rng=rng, timings=timings)
-        n += 1
-        last = time.perf_counter() - ts
-        if time.perf_counter() - t0 + last > budget_s or n >= 3:
-            break
-    dt = time.perf_counter() - t0
-    return n / dt, n, timings
+    cb.exchange_sweeps(RBMParams(flat, L, M), sigma, clusters, 5 + 4, L, rng, 5)
+    timings["sample"] = time.perf_counter() - t0
+    e, S, F, dp = cb.sr_stages(flat, L, M, rec["samples"].astype(np.int8), terms, 0.01, timings)
+    step_s = sum(timings.values())
+    Sg, xg, eg = rec["S"], rec["x"], rec["eloc"]
+    lo = np.tril_indices_from(S)
+    d = np.sqrt(np.diag(S) - 0.01)
+    dS = np.abs(Sg[lo] - S[lo])
+    errors = {
+        "eloc_max_abs_err_rel": float(np.abs(eg - e).max() / np.abs(e).max()),
+        "energy_mean_rel_err": float(abs(eg.mean() - e.mean()) / abs(e.mean())),
+        "S_max_abs_err_rel": float(dS.max() / np.abs(S[lo]).max()),
+        "S_max_err_over_sqrt_SiiSjj": float((dS / (np.outer(d, d)[lo] + 1e-300)).max()),
+        "F_rel_err": float(np.linalg.norm(rec["F"] - F) / np.linalg.norm(F)),
+        "x_rel_err": float(np.linalg.norm(xg - dp) / np.linalg.norm(dp)),
+        "reference": "FP64 NumPy/LAPACK restatement of the reference algorithm on the GPU step's "
+                     "own samples and (FP32) parameters",
+    }
+    return 1.0 / step_s, timings, errors
 
 
 def roofline_for(stage: str, mean_ms: float):
@@ -326,6 +480,23 @@ def measured_traffic(kernel: str):
         return None
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def guarded(name, fn, *a, **kw):
+    """Run a reporting leg; a failure is reported in the JSON line (and its
+    traceback on stderr) instead of losing the timed result."""
+    import traceback
+
+    try:
+        return fn(*a, **kw)
+    except Exception as e:  # noqa: BLE001 - reported, not hidden
+        traceback.print_exc()
+        log(f"{name} failed: {e!r}")
+        return {"error": f"{type(e).__name__}: {e}"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -333,16 +504,22 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=25.0)
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 GPU baseline")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the J1-J2 FFNN, TFIM 20x20 and serving workloads")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving stream only")
     ap.add_argument("--no-graph", action="store_true", help="skip the captured-graph replays")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
     ap.add_argument("--workload", default="heisenberg-l64",
                     choices=["heisenberg-l64", "j1j2-ffnn", "tfim2d-rbm-a8", "serving"],
-                    help="the default Heisenberg L=64 SR step, one of the larger minSR workloads, "
-                         "or the serving stream")
+                    help="the default Heisenberg L=64 SR step (with every report), or only one of "
+                         "the larger workloads, printed as its own line")
+    ap.add_argument("--norms", action="store_true",
+                    help="with --workload j1j2-ffnn / tfim2d-rbm-a8: per-step gradient and update norms")
     ap.add_argument("--serving-jobs", type=int, default=256)
     ap.add_argument("--serving-steps", type=int, default=200)
+    ap.add_argument("--workload-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -358,11 +535,12 @@ def main():
         copy_calibration()
         return
     if args.workload == "serving":
-        run_serving(args.serving_jobs, args.serving_steps)
+        print(json.dumps(run_serving(args.serving_jobs, args.serving_steps)))
         return
     if args.workload != "heisenberg-l64":
-        run_workload(args.workload, args.steps, args.warmup)
+        print(json.dumps(run_workload(args.workload, args.steps, args.warmup, norms=args.norms)))
         return
+    log("timed workload: heisenberg L=64")
     drv, H, vs = build_heisenberg_l64()
     for _ in range(args.warmup):
         step(drv)
@@ -398,16 +576,51 @@ def main():
     roof["traffic"] = measured_traffic("gram_bf16x3_kernel")
     others = {k: roofline_for(k, stages[k]["mean_ms"]) for k in ("potrf", "metropolis", "local_energy")
               if k in stages}
-    acc = None if args.no_accuracy else accuracy_run()
-    cpu = None
-    if not args.no_cpu_baseline:
+    # one more eager step with its results kept for the oracle comparison
+    rec = None if args.no_cpu_baseline else record_step(drv)
+    del drv, vs
+    _ops.WORKSPACE.clear()
+    torch.cuda.empty_cache()
+
+    acc = acc64 = None
+    if not args.no_accuracy:
+        log("accuracy: tfim L=16 (3000 captured steps)")
+        acc = guarded("accuracy tfim16", accuracy_run)
+        log("accuracy: heisenberg L=64 (500 captured steps) vs Bethe ansatz")
+        acc64 = guarded("accuracy heisenberg64", heisenberg_l64_energy)
+        _ops.WORKSPACE.clear()
+        torch.cuda.empty_cache()
+    plain = plain16 = None
+    vs_baseline = None
+    if not args.no_plain:
+        log("plain FP64 GPU baseline: heisenberg L=64")
+        plain = guarded("plain heisenberg64", plain_fp64_heisenberg_l64)
+        if "value" in plain:
+            vs_baseline = sps / plain["value"]
+        if not args.no_accuracy:
+            log("plain FP64 GPU baseline: tfim L=16 to 1e-4")
+            plain16 = guarded("plain tfim16", plain_fp64_tfim16)
+    cpu = errors = None
+    if rec is not None:
+        log("CPU restatement: one step on the GPU step's samples")
         cores = len(os.sched_getaffinity(0))
-        rate, n_cpu, tim = cpu_baseline_heisenberg_l64(args.cpu_budget)
+        res = guarded("cpu baseline", cpu_baseline_heisenberg_l64, rec)
+        rate, tim, errors = res if isinstance(res, tuple) else (None, {}, res)
         threads = int(os.environ.get("OMP_NUM_THREADS", cores))
-        cpu = {"value": rate, "unit": "SR steps/s", "cores": min(cores, threads), "kind": "port",
-               "sample": f"{n_cpu} full step(s) of the same workload (L=64 Heisenberg, 16384 samples, "
-                         f"S 8384^2) in FP64 NumPy/LAPACK with the reference algorithm; "
-                         f"stage seconds {json.dumps({k: round(v, 2) for k, v in tim.items()})}"}
+        cpu = None if rate is None else {"value": rate, "unit": "SR steps/s", "cores": min(cores, threads), "kind": "port",
+               "sample": "1 full step of the same workload (L=64 Heisenberg, 4096 chains, 16384 samples, "
+                         "S 8384^2) in FP64 NumPy/LAPACK with the reference algorithm: its sampler on its "
+                         "own chains, the rest on the GPU step's samples; stage seconds "
+                         f"{json.dumps({k: round(v, 2) for k, v in tim.items()})}"}
+        del rec
+    workloads = {}
+    if not args.no_workloads:
+        for name in ("j1j2-ffnn", "tfim2d-rbm-a8"):
+            log(f"workload {name}")
+            workloads[name] = guarded(name, run_workload, name, args.workload_steps, 2)
+        if not args.no_serving:
+            log(f"serving: {args.serving_jobs} jobs x {args.serving_steps} steps")
+            workloads["serving"] = guarded("serving", run_serving, args.serving_jobs, args.serving_steps)
     out = {
         "metric": "SR steps/s & samples/s at <=1e-4 rel. energy err vs ED; p50/p99 job latency",
         "value": sps,
@@ -421,7 +634,11 @@ def main():
         "samples_per_s": sps * 16384,
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": None,
+        "vs_baseline": vs_baseline,
+        "vs_baseline_definition": "value / SR steps/s of the plain FP64 GPU step (the reference algorithm "
+                                  "in PyTorch FP64 on this GPU: full forward per proposal, materialised "
+                                  "connected elements, dense FP64 Jacobian, rocBLAS S, rocSOLVER Cholesky) "
+                                  "on the same workload, measured in this run",
         "dtype": "f32 (bf16x3 MFMA Gram, FP64 Cholesky/means)",
         "data": "synthetic (random-init RBM, sampler-generated configurations)",
         "config": {"workload": "heisenberg-1d-L64-sz0 fp32-rbm-a2 metropolis-exchange 4096ch "
@@ -435,8 +652,12 @@ def main():
         "roofline": roof,
         "roofline_other_stages": others,
         "cpu_baseline": cpu,
+        "fp64_oracle_error": errors,
+        "plain_fp64_gpu": {"heisenberg_l64": plain, "tfim_l16": plain16},
         "accuracy": acc,
+        "accuracy_heisenberg_l64": acc64,
         "energy_first_last": [energies[0], energies[-1]],
+        "workloads": workloads,
     }
     print(json.dumps(out))
 

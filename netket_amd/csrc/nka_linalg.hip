@@ -48,11 +48,9 @@ inline unsigned grid_mult8(int64_t tiles) { return (unsigned)(((tiles + 7) / 8) 
 // ---------------------------------------------------------------------------
 // bf16x3 Gram: 128x128 output tile per workgroup, 4 waves of 64x64, K step 64.
 // ---------------------------------------------------------------------------
-constexpr int GB_T = 128;   // tile edge
 constexpr int GB_K = 64;    // K step (bf16 elements): one 128-byte line per row and plane
 constexpr int GB_LD = 72;   // LDS row stride in bf16 (64 + 8 pad: 144 bytes)
 constexpr int GB_CPR = GB_K / 8;              // 16-byte chunks per row and plane
-constexpr int GB_NIT = GB_T * GB_CPR / 256;   // chunks per thread and plane
 
 // Epilogue of the RBM neural tangent kernel, T = O_c O_c^T scaled, from
 // Theta = t t^T (the bf16x3 product) and S = sigma sigma^T:
@@ -67,40 +65,48 @@ struct GramRbmEpi {
   double hb, ha, scale;
 };
 
-template <bool RBM>
-__global__ void __launch_bounds__(256) gram_bf16x3_kernel(const uint16_t* __restrict__ xh,
+// WT: wave tile edge (64 or 96); the workgroup tile is 2 WT x 2 WT (four
+// waves), so each wave holds (WT/32)^2 accumulators of 32x32.  A 192-wide
+// tile reads 12 operand fragments from LDS per 27 products (128-wide: 8 per
+// 12) and re-reads each operand row panel from L2 a third fewer times; it
+// needs 110 KB of LDS, so one workgroup runs per CU.
+template <bool RBM, int WT>
+__global__ void __launch_bounds__(256, 1) gram_bf16x3_kernel(const uint16_t* __restrict__ xh,
                                                           const uint16_t* __restrict__ xl, int64_t n,
                                                           int64_t K, int64_t ldx, double shift,
                                                           double offset, double* __restrict__ out,
                                                           int64_t ldo, int64_t n_tiles_total,
                                                           GramRbmEpi ep) {
-  __shared__ __attribute__((aligned(16))) uint16_t sA[2][GB_T * GB_LD];  // [plane][row*LD + k]
-  __shared__ __attribute__((aligned(16))) uint16_t sB[2][GB_T * GB_LD];
+  constexpr int TT = 2 * WT;                 // workgroup tile edge
+  constexpr int MI = WT / 32;                // 32x32 tiles per wave edge
+  constexpr int NIT = TT * GB_CPR / 256;     // 16-byte chunks per thread and plane
+  __shared__ __attribute__((aligned(16))) uint16_t sA[2][TT * GB_LD];  // [plane][row*LD + k]
+  __shared__ __attribute__((aligned(16))) uint16_t sB[2][TT * GB_LD];
   const int64_t bid = xcd_remap(blockIdx.x, gridDim.x);
   if (bid >= n_tiles_total) return;
   int ti, tj;
-  tri_supertile(bid, (n + GB_T - 1) / GB_T, ti, tj);
-  const int64_t i0 = (int64_t)ti * GB_T, j0 = (int64_t)tj * GB_T;
+  tri_supertile(bid, (n + TT - 1) / TT, ti, tj);
+  const int64_t i0 = (int64_t)ti * TT, j0 = (int64_t)tj * TT;
   const int tid = threadIdx.x, wid = tid / 64, lane = lane_id();
-  const int wm = (wid >> 1) * 64, wn = (wid & 1) * 64;  // wave tile origin in block tile
+  const int wm = (wid >> 1) * WT, wn = (wid & 1) * WT;  // wave tile origin in block tile
   const bool diag_tile = (ti == tj);
 
-  floatx16 acc[2][2];
+  floatx16 acc[MI][MI];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < MI; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < MI; ++b)
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[a][b][e] = 0.0f;
 
-  // Each operand slab is 128 rows x GB_K k = 128 GB_CPR chunks of 8 bf16;
-  // thread tid moves chunks tid + 256 it (it < GB_NIT) of each of the (up to)
+  // Each operand slab is TT rows x GB_K k = TT GB_CPR chunks of 8 bf16;
+  // thread tid moves chunks tid + 256 it (it < NIT) of each of the (up to)
   // four planes.  The next slab is fetched into registers while the current
   // one is used.
-  uint4 pf[GB_NIT][4];
+  uint4 pf[NIT][4];
   auto fetch = [&](int64_t k0) {
 #pragma unroll
-    for (int it = 0; it < GB_NIT; ++it) {
+    for (int it = 0; it < NIT; ++it) {
       const int chunk = tid + it * 256;
       const int r = chunk / GB_CPR, kc = (chunk % GB_CPR) * 8;
       const int64_t kk = k0 + kc;
@@ -123,7 +129,7 @@ __global__ void __launch_bounds__(256) gram_bf16x3_kernel(const uint16_t* __rest
   for (int64_t k0 = 0; k0 < K; k0 += GB_K) {
     __syncthreads();
 #pragma unroll
-    for (int it = 0; it < GB_NIT; ++it) {
+    for (int it = 0; it < NIT; ++it) {
       const int chunk = tid + it * 256;
       const int r = chunk / GB_CPR, kc = (chunk % GB_CPR) * 8;
       *(uint4*)(&sA[0][r * GB_LD + kc]) = pf[it][0];
@@ -140,15 +146,15 @@ __global__ void __launch_bounds__(256) gram_bf16x3_kernel(const uint16_t* __rest
 #pragma unroll
     for (int ks = 0; ks < GB_K; ks += 16) {
       const int kq = ks + (lane >> 5) * 8;
-      bf16x8 ah[2], al[2], bh[2], bl[2];
+      bf16x8 ah[MI], al[MI], bh[MI], bl[MI];
 #pragma unroll
-      for (int a = 0; a < 2; ++a) {
+      for (int a = 0; a < MI; ++a) {
         const int row = wm + a * 32 + (lane & 31);
         ah[a] = *(const bf16x8*)(&sA[0][row * GB_LD + kq]);
         al[a] = *(const bf16x8*)(&sA[1][row * GB_LD + kq]);
       }
 #pragma unroll
-      for (int b = 0; b < 2; ++b) {
+      for (int b = 0; b < MI; ++b) {
         const int row = wn + b * 32 + (lane & 31);
         bh[b] = *(const bf16x8*)(&Bh[row * GB_LD + kq]);
         bl[b] = *(const bf16x8*)(&Bl[row * GB_LD + kq]);
@@ -156,28 +162,28 @@ __global__ void __launch_bounds__(256) gram_bf16x3_kernel(const uint16_t* __rest
       // small products first, four independent accumulators between
       // consecutive updates of the same accumulator
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int a = 0; a < MI; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < MI; ++b)
           acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int a = 0; a < MI; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < MI; ++b)
           acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int a = 0; a < MI; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < MI; ++b)
           acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
     }
   }
   // store: element e of lane l in a 32x32 tile is (row (e/4)*8 + (l/32)*4 + e%4, col l%32)
   if constexpr (!RBM) {
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < MI; ++a)
 #pragma unroll
-      for (int b = 0; b < 2; ++b)
+      for (int b = 0; b < MI; ++b)
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
           const int64_t gi = i0 + wm + a * 32 + (e / 4) * 8 + (lane >> 5) * 4 + (e & 3);
@@ -194,7 +200,7 @@ __global__ void __launch_bounds__(256) gram_bf16x3_kernel(const uint16_t* __rest
     uint32_t* rb = (uint32_t*)&sA[0][0];
     uint32_t* cb = (uint32_t*)&sB[0][0];
     const int nwp = ep.nwp;
-    for (int idx = tid; idx < GB_T * nwp; idx += 256) {
+    for (int idx = tid; idx < TT * nwp; idx += 256) {
       const int r = idx / nwp, w = idx % nwp;
       uint32_t x = 0, y = 0;
       if (w < ep.nw) {
@@ -205,34 +211,34 @@ __global__ void __launch_bounds__(256) gram_bf16x3_kernel(const uint16_t* __rest
       cb[idx] = y;
     }
     __syncthreads();
-    int cnt[2][2][16];
+    int cnt[MI][MI][16];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < MI; ++a)
 #pragma unroll
-      for (int b = 0; b < 2; ++b)
+      for (int b = 0; b < MI; ++b)
 #pragma unroll
         for (int e = 0; e < 16; ++e) cnt[a][b][e] = 0;
     for (int w = 0; w < nwp; w += 4) {
-      uint4 cq[2];
+      uint4 cq[MI];
 #pragma unroll
-      for (int b = 0; b < 2; ++b) cq[b] = *(const uint4*)&cb[(wn + b * 32 + (lane & 31)) * nwp + w];
+      for (int b = 0; b < MI; ++b) cq[b] = *(const uint4*)&cb[(wn + b * 32 + (lane & 31)) * nwp + w];
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int a = 0; a < MI; ++a)
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
           const int rl = wm + a * 32 + (e / 4) * 8 + (lane >> 5) * 4 + (e & 3);
           const uint4 rq = *(const uint4*)&rb[rl * nwp + w];
 #pragma unroll
-          for (int b = 0; b < 2; ++b)
+          for (int b = 0; b < MI; ++b)
             cnt[a][b][e] += __builtin_popcount(rq.x ^ cq[b].x) + __builtin_popcount(rq.y ^ cq[b].y) +
                             __builtin_popcount(rq.z ^ cq[b].z) + __builtin_popcount(rq.w ^ cq[b].w);
         }
     }
     const double cc = *ep.c;
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < MI; ++a)
 #pragma unroll
-      for (int b = 0; b < 2; ++b)
+      for (int b = 0; b < MI; ++b)
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
           const int64_t gi = i0 + wm + a * 32 + (e / 4) * 8 + (lane >> 5) * 4 + (e & 3);
@@ -249,8 +255,11 @@ __global__ void __launch_bounds__(256) gram_bf16x3_kernel(const uint16_t* __rest
 }
 
 // Largest LDS row stride (words) of the packed spins in the RBM epilogue: the
-// 128 rows must fit in one operand buffer.
-constexpr int kGramRbmMaxNwp = (int)(2 * GB_T * GB_LD * sizeof(uint16_t) / (GB_T * sizeof(uint32_t))) / 4 * 4;
+// tile's rows must fit in one operand buffer (2 planes x rows x GB_LD bf16).
+constexpr int kGramRbmMaxNwp = (int)(2 * GB_LD * sizeof(uint16_t) / sizeof(uint32_t)) / 4 * 4;
+// wave tile edges of the Gram and of the RBM tangent kernel
+constexpr int kGramWT = 96;
+constexpr int kGramRbmWT = 64;
 
 // ---------------------------------------------------------------------------
 // FP64 NT GEMM: C = beta C + alpha A B^T (+ shift on the diagonal),
@@ -1487,11 +1496,11 @@ bool launch_gram_rbm(const uint16_t* xh, const uint16_t* xl, int64_t n, int64_t 
                      hipStream_t s) {
   const int nwp = (nw + 3) / 4 * 4;
   if (nwp > kGramRbmMaxNwp) return false;
-  const int64_t T = (n + GB_T - 1) / GB_T;
+  const int64_t T = (n + 2 * kGramRbmWT - 1) / (2 * kGramRbmWT);
   const int64_t tiles = T * (T + 1) / 2;
   GramRbmEpi ep{bits, nw, nwp, N, v, c, hb, ha, scale};
-  hipLaunchKernelGGL((gram_bf16x3_kernel<true>), dim3(grid_mult8(tiles)), dim3(256), 0, s, xh, xl, n, K, ldx,
-                     shift, offset, out, ldo, tiles, ep);
+  hipLaunchKernelGGL((gram_bf16x3_kernel<true, kGramRbmWT>), dim3(grid_mult8(tiles)), dim3(256), 0, s, xh, xl,
+                     n, K, ldx, shift, offset, out, ldo, tiles, ep);
   return true;
 }
 
@@ -1509,11 +1518,11 @@ int nka_gram_bf16x3(const uint16_t* x_hi, const uint16_t* x_lo, int64_t n, int64
   NKA_CHECK_ARG(((uintptr_t)x_hi % 16) == 0 && ((uintptr_t)x_lo % 16) == 0,
                 "nka_gram_bf16x3: operands must be 16-byte aligned");
   if (n == 0) return NKA_OK;
-  const int64_t T = (n + GB_T - 1) / GB_T;
+  const int64_t T = (n + 2 * kGramWT - 1) / (2 * kGramWT);
   const int64_t tiles = T * (T + 1) / 2;
   NKA_CHECK_ARG(tiles < (1LL << 31), "nka_gram_bf16x3: matrix too large");
-  hipLaunchKernelGGL((gram_bf16x3_kernel<false>), dim3(grid_mult8(tiles)), dim3(256), 0, (hipStream_t)stream,
-                     x_hi, x_lo, n, k, ldx, shift, offset, out, ldo, tiles, GramRbmEpi{});
+  hipLaunchKernelGGL((gram_bf16x3_kernel<false, kGramWT>), dim3(grid_mult8(tiles)), dim3(256), 0,
+                     (hipStream_t)stream, x_hi, x_lo, n, k, ldx, shift, offset, out, ldo, tiles, GramRbmEpi{});
   NKA_CHECK_LAUNCH("nka_gram_bf16x3");
   return NKA_OK;
 }

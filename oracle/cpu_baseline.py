@@ -49,6 +49,36 @@ def exchange_sweeps(p: RBMParams, sigma: np.ndarray, clusters: np.ndarray, n_swe
     return np.stack(out, axis=1), x
 
 
+def sr_stages(flat: np.ndarray, N: int, M: int, x: np.ndarray, terms: dict, diag_shift: float,
+              timings: dict | None = None):
+    """E_loc, S, F and the SR update on given samples x (Ns, N), FP64.
+    Returns (e, S, F, dp)."""
+    t1 = time.perf_counter()
+    p = RBMParams(flat, N, M)
+    xp, mels, _ = O.local_conn(x, terms)
+    B, K, _ = xp.shape
+    lpx = log_psi(p, x)
+    lpp = log_psi(p, xp.reshape(-1, N)).reshape(B, K)
+    e = np.sum(mels * np.exp(lpp - lpx[:, None]), axis=1)
+    t2 = time.perf_counter()
+    Ns = x.shape[0]
+    Ojac = jacobian(p, x)
+    OL = (Ojac - Ojac.mean(axis=0, keepdims=True)) / np.sqrt(Ns)
+    del Ojac
+    dv = 2.0 * (e - e.mean()) / np.sqrt(Ns)
+    F = OL.T @ dv
+    S = OL.T @ OL
+    del OL
+    S[np.diag_indices_from(S)] += diag_shift
+    t3 = time.perf_counter()
+    dp = sla.cho_solve(sla.cho_factor(S, lower=False), F)
+    t4 = time.perf_counter()
+    if timings is not None:
+        for k, v in (("local_energy", t2 - t1), ("jacobian_gram", t3 - t2), ("cholesky", t4 - t3)):
+            timings[k] = timings.get(k, 0.0) + v
+    return e, S, F, dp
+
+
 def vmc_sr_step(flat: np.ndarray, N: int, M: int, sigma: np.ndarray, terms: dict,
                 clusters: np.ndarray, *, n_discard: int, chain_length: int, sweep_size: int,
                 diag_shift: float, lr: float, rng: np.random.Generator, timings: dict | None = None):
@@ -58,27 +88,10 @@ def vmc_sr_step(flat: np.ndarray, N: int, M: int, sigma: np.ndarray, terms: dict
     samples, sigma = exchange_sweeps(p, sigma, clusters, n_discard + chain_length, sweep_size, rng,
                                      n_discard)
     t1 = time.perf_counter()
+    if timings is not None:
+        timings["sample"] = timings.get("sample", 0.0) + (t1 - t0)
     C, L, _ = samples.shape
     x = samples.reshape(-1, N)
-    xp, mels, _ = O.local_conn(x, terms)
-    B, K, _ = xp.shape
-    lpx = log_psi(p, x)
-    lpp = log_psi(p, xp.reshape(-1, N)).reshape(B, K)
-    e = np.sum(mels * np.exp(lpp - lpx[:, None]), axis=1)
+    e, S, F, dp = sr_stages(flat, N, M, x, terms, diag_shift, timings)
     st = ostats.statistics(e.reshape(C, L))
-    t2 = time.perf_counter()
-    Ns = x.shape[0]
-    Ojac = jacobian(p, x)
-    OL = (Ojac - Ojac.mean(axis=0, keepdims=True)) / np.sqrt(Ns)
-    dv = 2.0 * (e - e.mean()) / np.sqrt(Ns)
-    F = OL.T @ dv
-    S = OL.T @ OL
-    S[np.diag_indices_from(S)] += diag_shift
-    t3 = time.perf_counter()
-    dp = sla.cho_solve(sla.cho_factor(S, lower=False), F)
-    t4 = time.perf_counter()
-    if timings is not None:
-        for k, v in (("sample", t1 - t0), ("local_energy", t2 - t1), ("jacobian_gram", t3 - t2),
-                     ("cholesky", t4 - t3)):
-            timings[k] = timings.get(k, 0.0) + v
     return flat - lr * dp, sigma, st["mean"]
