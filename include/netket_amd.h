@@ -127,6 +127,24 @@ int nka_rbm_local_energy(const nka_rbm* m, int op_kind, const void* op, const in
                          int64_t n, double* eloc, void* log_psi, void* workspace,
                          int64_t workspace_bytes, void* stream);
 
+/* The local energies of a batch of jobs of one shape, Ising operators on one
+ * lattice (each job its own h and J), in one launch (the serving loop's shared
+ * launches).  jobs is an array in device memory; shape and op_shape give the
+ * shared sizes and edges (their parameter pointer and couplings are not read);
+ * n samples per job.  Each job's values equal nka_rbm_local_energy's.  Models
+ * that take the tiled path are refused (nka_rbm_local_energy_batchable
+ * returns 0 for them). */
+typedef struct nka_eloc_job {
+  nka_rbm model;
+  nka_ising op;
+  const int8_t* sigma;
+  double* eloc;
+  void* log_psi;
+} nka_eloc_job;
+int nka_rbm_local_energy_batchable(const nka_rbm* shape, const nka_ising* op);
+int nka_rbm_local_energy_batched(const nka_eloc_job* jobs, int n_jobs, const nka_rbm* shape,
+                                 const nka_ising* op_shape, int64_t n, void* stream);
+
 /* ------------------------------------------------------------------------
  * Two-layer dense network: log psi(x) = sum_k log_cosh(z2_k) with
  * z2 = log_cosh(x W1 + b1) W2 + b2 (Dense -> log_cosh -> Dense -> log_cosh ->
@@ -240,6 +258,29 @@ int nka_rbm_metropolis(const nka_rbm* m, int rule, const int32_t* clusters, int 
                        int n_discard, int chain_length, double machine_pow, uint64_t seed,
                        uint64_t counter, uint64_t* counter_dev, int8_t* samples,
                        void* log_prob_samples, int64_t* n_accepted, void* stream);
+
+/* The sweeps of a batch of independent jobs of one shape in one launch (the
+ * serving loop's shared launches; NetKet runs one sampler per process, so
+ * this has no single counterpart beyond nka_rbm_metropolis above).  Job k's
+ * parameters, chains, outputs and Philox stream are in jobs[k] (an array in
+ * device memory); shape gives n_visible, n_hidden, dtype and the parameter
+ * offsets shared by all jobs (its params pointer is not read); the other
+ * arguments are shared.  Each job's chains are exactly those that
+ * nka_rbm_metropolis gives it alone. */
+typedef struct nka_sweep_job {
+  nka_rbm model;
+  int8_t* sigma;
+  void* log_prob;
+  int8_t* samples;
+  void* log_prob_samples;
+  int64_t* n_accepted;
+  uint64_t seed;
+  uint64_t counter;
+  uint64_t* counter_dev;
+} nka_sweep_job;
+int nka_rbm_metropolis_batched(const nka_sweep_job* jobs, int n_jobs, const nka_rbm* shape, int rule,
+                               const int32_t* clusters, int n_clusters, int n_chains, int sweep_size,
+                               int n_discard, int chain_length, double machine_pow, void* stream);
 
 /* ------------------------------------------------------------------------
  * Statistics (netket/stats/mc_stats_old.py:87-196): out[0..7] =

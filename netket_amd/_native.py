@@ -72,6 +72,20 @@ class NkaFfnn(C.Structure):
     ]
 
 
+class NkaSweepJob(C.Structure):
+    _fields_ = [
+        ("model", NkaRbm),
+        ("sigma", C.c_void_p),
+        ("log_prob", C.c_void_p),
+        ("samples", C.c_void_p),
+        ("log_prob_samples", C.c_void_p),
+        ("n_accepted", C.c_void_p),
+        ("seed", C.c_uint64),
+        ("counter", C.c_uint64),
+        ("counter_dev", C.c_void_p),
+    ]
+
+
 class NkaIsing(C.Structure):
     _fields_ = [
         ("edges", C.c_void_p),
@@ -79,6 +93,16 @@ class NkaIsing(C.Structure):
         ("mel_dtype", C.c_int),
         ("h", C.c_double),
         ("J", C.c_double),
+    ]
+
+
+class NkaElocJob(C.Structure):
+    _fields_ = [
+        ("model", NkaRbm),
+        ("op", NkaIsing),
+        ("sigma", C.c_void_p),
+        ("eloc", C.c_void_p),
+        ("log_psi", C.c_void_p),
     ]
 
 
@@ -111,6 +135,9 @@ SIGNATURES = {
     "nka_rbm_local_energy_workspace": (i64, [C.POINTER(NkaRbm), i32, vp, i64]),
     "nka_rbm_local_energy": (i32, [C.POINTER(NkaRbm), i32, vp, vp, i64, vp, vp, vp, i64, vp]),
     "nka_random_states": (i32, [vp, i32, i32, i32, u64, u64, vp]),
+    "nka_rbm_metropolis_batched": (i32, [vp, i32, C.POINTER(NkaRbm), i32, vp, i32, i32, i32, i32, i32, f64, vp]),
+    "nka_rbm_local_energy_batchable": (i32, [C.POINTER(NkaRbm), C.POINTER(NkaIsing)]),
+    "nka_rbm_local_energy_batched": (i32, [vp, i32, C.POINTER(NkaRbm), C.POINTER(NkaIsing), i64, vp]),
     "nka_rbm_metropolis": (
         i32,
         [C.POINTER(NkaRbm), i32, vp, i32, vp, vp, i32, i32, i32, i32, f64, u64, u64, vp, vp, vp, vp,

@@ -54,9 +54,9 @@ __device__ inline double load_mel(const void* p, int64_t i, int dtype) {
 }
 
 template <class T, int JL, bool STAGE>
-__global__ void __launch_bounds__(256) rbm_eloc_kernel(nka_rbm m, OpArgs op, const int8_t* __restrict__ sigma,
-                                                       int64_t n, double* __restrict__ eloc,
-                                                       T* __restrict__ log_psi) {
+__device__ __forceinline__ void rbm_eloc_body(const nka_rbm& m, const OpArgs& op, const int8_t* __restrict__ sigma,
+                                              int64_t n, double* __restrict__ eloc, T* __restrict__ log_psi,
+                                              int bx, int gx) {
   extern __shared__ unsigned char smem[];
   const RbmView<T> v = rbm_view<T>(m);
   const int N = v.N, M = v.M;
@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(256) rbm_eloc_kernel(nka_rbm m, OpArgs op, con
     stage_w<T>((T*)smem, v.W, N, M);
     W = (const T*)smem;
   }
-  for (int64_t b = (int64_t)blockIdx.x * waves + wid; b < n; b += (int64_t)gridDim.x * waves) {
+  for (int64_t b = (int64_t)bx * waves + wid; b < n; b += (int64_t)gx * waves) {
     for (int i = lane; i < N; i += NKA_WAVE) xs[i] = sigma[b * N + i];
     wave_lds_sync();
     HiddenState<T, JL> h;
@@ -136,6 +136,24 @@ __global__ void __launch_bounds__(256) rbm_eloc_kernel(nka_rbm m, OpArgs op, con
     if (lane == 0) eloc[b] = E;
     wave_lds_sync();
   }
+}
+
+template <class T, int JL, bool STAGE>
+__global__ void __launch_bounds__(256) rbm_eloc_kernel(nka_rbm m, OpArgs op, const int8_t* __restrict__ sigma,
+                                                       int64_t n, double* __restrict__ eloc,
+                                                       T* __restrict__ log_psi) {
+  rbm_eloc_body<T, JL, STAGE>(m, op, sigma, n, eloc, log_psi, blockIdx.x, gridDim.x);
+}
+
+// The local energies of a batch of jobs of one shape (Ising operators on the
+// same lattice, each with its own couplings): blockIdx.y selects the job.
+template <class T, int JL, bool STAGE>
+__global__ void __launch_bounds__(256) rbm_eloc_kernel_batched(const nka_eloc_job* __restrict__ jobs, int64_t n) {
+  const nka_eloc_job& j = jobs[blockIdx.y];
+  OpArgs op{};
+  op.kind = NKA_OP_ISING;
+  op.ising = j.op;
+  rbm_eloc_body<T, JL, STAGE>(j.model, op, j.sigma, n, j.eloc, (T*)j.log_psi, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -561,12 +579,24 @@ static void launch_log_psi_f64(const nka_rbm& m, const int8_t* sg, int64_t n, vo
 
 template <int JL, class T>
 static void launch_eloc(const nka_rbm& m, const OpArgs& op, const int8_t* sigma, int64_t n,
-                        double* eloc, T* lp, hipStream_t s) {
+                        double* eloc, T* lp, hipStream_t s, const nka_eloc_job* jobs = nullptr,
+                        int n_jobs = 0) {
   const int waves = 4;
   const bool stage = stage_fits<T>(m, waves);
   const size_t lds = (stage ? pad16((size_t)m.n_visible * m.n_hidden * sizeof(T)) : 0) +
                      waves * pad16(m.n_visible);
   const unsigned g = grid_for(n, waves, 2048);
+  if (jobs) {
+    // fewer blocks per job when the batch fills the device anyway
+    const unsigned gb = std::max<unsigned>(1, std::min<unsigned>(g, (unsigned)((4096 + n_jobs - 1) / n_jobs)));
+    if (stage)
+      hipLaunchKernelGGL((rbm_eloc_kernel_batched<T, JL, true>), dim3(gb, n_jobs), dim3(64 * waves), lds, s,
+                         jobs, n);
+    else
+      hipLaunchKernelGGL((rbm_eloc_kernel_batched<T, JL, false>), dim3(gb, n_jobs), dim3(64 * waves), lds, s,
+                         jobs, n);
+    return;
+  }
   if (stage)
     hipLaunchKernelGGL((rbm_eloc_kernel<T, JL, true>), dim3(g), dim3(64 * waves), lds, s, m, op,
                        sigma, n, eloc, lp);
@@ -576,13 +606,13 @@ static void launch_eloc(const nka_rbm& m, const OpArgs& op, const int8_t* sigma,
 }
 template <int JL>
 static void launch_eloc_f32(const nka_rbm& m, const OpArgs& op, const int8_t* sg, int64_t n,
-                            double* e, void* lp, hipStream_t s) {
-  launch_eloc<JL, float>(m, op, sg, n, e, (float*)lp, s);
+                            double* e, void* lp, hipStream_t s, const nka_eloc_job* jobs, int n_jobs) {
+  launch_eloc<JL, float>(m, op, sg, n, e, (float*)lp, s, jobs, n_jobs);
 }
 template <int JL>
 static void launch_eloc_f64(const nka_rbm& m, const OpArgs& op, const int8_t* sg, int64_t n,
-                            double* e, void* lp, hipStream_t s) {
-  launch_eloc<JL, double>(m, op, sg, n, e, (double*)lp, s);
+                            double* e, void* lp, hipStream_t s, const nka_eloc_job* jobs, int n_jobs) {
+  launch_eloc<JL, double>(m, op, sg, n, e, (double*)lp, s, jobs, n_jobs);
 }
 
 // theta = sigma W + b for models whose W does not fit in LDS, as a tiled
@@ -984,11 +1014,44 @@ int nka_rbm_local_energy(const nka_rbm* m, int op_kind, const void* op, const in
   }
   const int jl = pick_jl(m->n_hidden);
   if (m->dtype == NKA_F32) {
-    NKA_DISPATCH_JL(jl, launch_eloc_f32, *m, a, sigma, n, eloc, log_psi, s);
+    NKA_DISPATCH_JL(jl, launch_eloc_f32, *m, a, sigma, n, eloc, log_psi, s, (const nka_eloc_job*)nullptr, 0);
   } else {
-    NKA_DISPATCH_JL(jl, launch_eloc_f64, *m, a, sigma, n, eloc, log_psi, s);
+    NKA_DISPATCH_JL(jl, launch_eloc_f64, *m, a, sigma, n, eloc, log_psi, s, (const nka_eloc_job*)nullptr, 0);
   }
   NKA_CHECK_LAUNCH("nka_rbm_local_energy");
+  return NKA_OK;
+}
+
+int nka_rbm_local_energy_batchable(const nka_rbm* shape, const nka_ising* op) {
+  if (!shape || !op) return 0;
+  return ising_tiled(*shape, NKA_OP_ISING, op) ? 0 : 1;
+}
+
+int nka_rbm_local_energy_batched(const nka_eloc_job* jobs, int n_jobs, const nka_rbm* shape,
+                                 const nka_ising* op_shape, int64_t n, void* stream) {
+  NKA_CHECK_ARG(jobs && shape && op_shape && n_jobs > 0 && n_jobs <= 65535 && n > 0,
+                "nka_rbm_local_energy_batched: bad batch");
+  NKA_CHECK_ARG(shape->n_visible > 0 && shape->n_hidden > 0 && shape->n_hidden <= kMaxJL * NKA_WAVE &&
+                    (shape->dtype == NKA_F32 || shape->dtype == NKA_F64),
+                "nka_rbm_local_energy_batched: unsupported model");
+  if (ising_tiled(*shape, NKA_OP_ISING, op_shape)) {
+    set_error("nka_rbm_local_energy_batched: this model takes the tiled path (W beyond the LDS budget); "
+              "call nka_rbm_local_energy per job");
+    return NKA_ERR_UNSUPPORTED;
+  }
+  OpArgs a{};
+  a.kind = NKA_OP_ISING;
+  a.ising = *op_shape;
+  const int jl = pick_jl(shape->n_hidden);
+  hipStream_t s = (hipStream_t)stream;
+  if (shape->dtype == NKA_F32) {
+    NKA_DISPATCH_JL(jl, launch_eloc_f32, *shape, a, (const int8_t*)nullptr, n, (double*)nullptr, (void*)nullptr,
+                    s, jobs, n_jobs);
+  } else {
+    NKA_DISPATCH_JL(jl, launch_eloc_f64, *shape, a, (const int8_t*)nullptr, n, (double*)nullptr, (void*)nullptr,
+                    s, jobs, n_jobs);
+  }
+  NKA_CHECK_LAUNCH("nka_rbm_local_energy_batched");
   return NKA_OK;
 }
 

@@ -69,10 +69,7 @@ struct SweepArgs {
 };
 
 template <class T, int JL, bool STAGE>
-__global__ void __launch_bounds__(256) rbm_metropolis_kernel(nka_rbm m, SweepArgs sa, int8_t* __restrict__ sigma,
-                                                             T* __restrict__ log_prob, int8_t* __restrict__ samples,
-                                                             T* __restrict__ lp_samples,
-                                                             int64_t* __restrict__ n_accepted) {
+__device__ __forceinline__ void rbm_metropolis_kernel_body(const nka_rbm& m, const SweepArgs& sa, int8_t* __restrict__ sigma, T* __restrict__ log_prob, int8_t* __restrict__ samples, T* __restrict__ lp_samples, int64_t* __restrict__ n_accepted, int bx) {
   extern __shared__ unsigned char smem[];
   const RbmView<T> v = rbm_view<T>(m);
   const int N = v.N, M = v.M;
@@ -85,7 +82,7 @@ __global__ void __launch_bounds__(256) rbm_metropolis_kernel(nka_rbm m, SweepArg
     stage_w<T>((T*)smem, v.W, N, M);
     W = (const T*)smem;
   }
-  const int c = blockIdx.x * waves + wid;
+  const int c = bx * waves + wid;
   if (c >= sa.C) return;
 
   for (int i = lane; i < N; i += NKA_WAVE) xs[i] = sigma[(int64_t)c * N + i];
@@ -230,6 +227,24 @@ __global__ void __launch_bounds__(256) rbm_metropolis_kernel(nka_rbm m, SweepArg
   }
 }
 
+template <class T, int JL, bool STAGE>
+__global__ void __launch_bounds__(256) rbm_metropolis_kernel(nka_rbm m, SweepArgs sa, int8_t* __restrict__ sigma, T* __restrict__ log_prob, int8_t* __restrict__ samples, T* __restrict__ lp_samples, int64_t* __restrict__ n_accepted) {
+  rbm_metropolis_kernel_body<T, JL, STAGE>(m, sa, sigma, log_prob, samples, lp_samples, n_accepted, blockIdx.x);
+}
+
+// The same sweeps for a batch of jobs of one shape: blockIdx.y selects the job
+// (its parameters, chains, outputs and random stream); the shape arguments are shared.
+template <class T, int JL, bool STAGE>
+__global__ void __launch_bounds__(256) rbm_metropolis_kernel_batched(const nka_sweep_job* __restrict__ jobs, SweepArgs sa) {
+  const nka_sweep_job& j = jobs[blockIdx.y];
+  SweepArgs s = sa;
+  s.k0 = (uint32_t)j.seed;
+  s.k1 = (uint32_t)(j.seed >> 32);
+  s.counter = j.counter;
+  s.counter_dev = j.counter_dev;
+  rbm_metropolis_kernel_body<T, JL, STAGE>(j.model, s, j.sigma, (T*)j.log_prob, j.samples, (T*)j.log_prob_samples, j.n_accepted, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Local-rule sweeps of FP32 models whose W does not fit in LDS.  One
 // wavefront per chain, lane l owning the units j = l + 64 jj, as in
@@ -249,18 +264,14 @@ constexpr float kKappaS = 2.8853900817779268f;    // 2 / ln 2
 constexpr float kLn2S = 0.69314718055994531f;
 
 template <int JL>
-__global__ void __launch_bounds__(256) rbm_metropolis_wide_kernel(nka_rbm m, SweepArgs sa, int8_t* __restrict__ sigma,
-                                                                  float* __restrict__ log_prob,
-                                                                  int8_t* __restrict__ samples,
-                                                                  float* __restrict__ lp_samples,
-                                                                  int64_t* __restrict__ n_accepted) {
+__device__ __forceinline__ void rbm_metropolis_wide_kernel_body(const nka_rbm& m, const SweepArgs& sa, int8_t* __restrict__ sigma, float* __restrict__ log_prob, int8_t* __restrict__ samples, float* __restrict__ lp_samples, int64_t* __restrict__ n_accepted, int bx) {
   extern __shared__ unsigned char smem[];
   const RbmView<float> v = rbm_view<float>(m);
   const int N = v.N, M = v.M;
   const int wid = threadIdx.x / NKA_WAVE, waves = blockDim.x / NKA_WAVE;
   const int lane = lane_id();
   int8_t* xs = (int8_t*)smem + wid * pad16(N);
-  const int c = blockIdx.x * waves + wid;
+  const int c = bx * waves + wid;
   if (c >= sa.C) return;
   for (int i = lane; i < N; i += NKA_WAVE) xs[i] = sigma[(int64_t)c * N + i];
   wave_lds_sync();
@@ -369,6 +380,24 @@ __global__ void __launch_bounds__(256) rbm_metropolis_wide_kernel(nka_rbm m, Swe
   }
 }
 
+template <int JL>
+__global__ void __launch_bounds__(256) rbm_metropolis_wide_kernel(nka_rbm m, SweepArgs sa, int8_t* __restrict__ sigma, float* __restrict__ log_prob, int8_t* __restrict__ samples, float* __restrict__ lp_samples, int64_t* __restrict__ n_accepted) {
+  rbm_metropolis_wide_kernel_body<JL>(m, sa, sigma, log_prob, samples, lp_samples, n_accepted, blockIdx.x);
+}
+
+// The same sweeps for a batch of jobs of one shape: blockIdx.y selects the job
+// (its parameters, chains, outputs and random stream); the shape arguments are shared.
+template <int JL>
+__global__ void __launch_bounds__(256) rbm_metropolis_wide_kernel_batched(const nka_sweep_job* __restrict__ jobs, SweepArgs sa) {
+  const nka_sweep_job& j = jobs[blockIdx.y];
+  SweepArgs s = sa;
+  s.k0 = (uint32_t)j.seed;
+  s.k1 = (uint32_t)(j.seed >> 32);
+  s.counter = j.counter;
+  s.counter_dev = j.counter_dev;
+  rbm_metropolis_wide_kernel_body<JL>(j.model, s, j.sigma, (float*)j.log_prob, j.samples, (float*)j.log_prob_samples, j.n_accepted, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Several chains per wavefront.  For small RBMs (M <= 8 * 64/G hidden units)
 // a proposal's arithmetic is a few hundred instructions that the per-proposal
@@ -422,12 +451,7 @@ __device__ inline T group_sum(T v) {
 }
 
 template <class T, int G, int JG, int RSG, bool STAGE>
-__global__ void __launch_bounds__(256) rbm_metropolis_group_kernel(nka_rbm m, SweepArgs sa,
-                                                                   int8_t* __restrict__ sigma,
-                                                                   T* __restrict__ log_prob,
-                                                                   int8_t* __restrict__ samples,
-                                                                   T* __restrict__ lp_samples,
-                                                                   int64_t* __restrict__ n_accepted) {
+__device__ __forceinline__ void rbm_metropolis_group_kernel_body(const nka_rbm& m, const SweepArgs& sa, int8_t* __restrict__ sigma, T* __restrict__ log_prob, int8_t* __restrict__ samples, T* __restrict__ lp_samples, int64_t* __restrict__ n_accepted, int bx) {
   constexpr int LPC = NKA_WAVE / G;
   constexpr uint64_t GMASK = (LPC == 64) ? ~0ull : ((1ull << LPC) - 1ull);
   extern __shared__ unsigned char smem[];
@@ -451,8 +475,8 @@ __global__ void __launch_bounds__(256) rbm_metropolis_group_kernel(nka_rbm m, Sw
   } else {
     __syncthreads();
   }
-  const int c_raw = (blockIdx.x * waves + wid) * G + g;
-  if ((blockIdx.x * waves + wid) * G >= sa.C) return;  // whole wavefront past the last chain
+  const int c_raw = (bx * waves + wid) * G + g;
+  if ((bx * waves + wid) * G >= sa.C) return;  // whole wavefront past the last chain
   const bool live = c_raw < sa.C;                        // groups past the last chain repeat it
   const int c = live ? c_raw : sa.C - 1;
 
@@ -629,6 +653,24 @@ __global__ void __launch_bounds__(256) rbm_metropolis_group_kernel(nka_rbm m, Sw
   }
 }
 
+template <class T, int G, int JG, int RSG, bool STAGE>
+__global__ void __launch_bounds__(256) rbm_metropolis_group_kernel(nka_rbm m, SweepArgs sa, int8_t* __restrict__ sigma, T* __restrict__ log_prob, int8_t* __restrict__ samples, T* __restrict__ lp_samples, int64_t* __restrict__ n_accepted) {
+  rbm_metropolis_group_kernel_body<T, G, JG, RSG, STAGE>(m, sa, sigma, log_prob, samples, lp_samples, n_accepted, blockIdx.x);
+}
+
+// The same sweeps for a batch of jobs of one shape: blockIdx.y selects the job
+// (its parameters, chains, outputs and random stream); the shape arguments are shared.
+template <class T, int G, int JG, int RSG, bool STAGE>
+__global__ void __launch_bounds__(256) rbm_metropolis_group_kernel_batched(const nka_sweep_job* __restrict__ jobs, SweepArgs sa) {
+  const nka_sweep_job& j = jobs[blockIdx.y];
+  SweepArgs s = sa;
+  s.k0 = (uint32_t)j.seed;
+  s.k1 = (uint32_t)(j.seed >> 32);
+  s.counter = j.counter;
+  s.counter_dev = j.counter_dev;
+  rbm_metropolis_group_kernel_body<T, G, JG, RSG, STAGE>(j.model, s, j.sigma, (T*)j.log_prob, j.samples, (T*)j.log_prob_samples, j.n_accepted, blockIdx.x);
+}
+
 // counter += inc (one thread), after the sweep that consumed the counter
 __global__ void advance_counter_kernel(uint64_t* __restrict__ c, uint64_t inc) {
   if (threadIdx.x == 0 && blockIdx.x == 0) c[0] = c[0] + inc;
@@ -681,40 +723,52 @@ static int pick_group(const nka_rbm& m, const SweepArgs& sa, int waves) {
   return 0;
 }
 
+// Every launch helper takes an optional batch: with jobs != nullptr the
+// kernel runs once per job of the batch (grid y), each job's parameters,
+// chains and random stream read from jobs[] (device memory), the shape from m.
 template <class T, int G, int JG, int RSG>
 static void launch_group_rs(const nka_rbm& m, const SweepArgs& sa, int8_t* sigma, T* lp, int8_t* smp,
-                            T* lps, int64_t* nacc, int waves, hipStream_t s) {
+                            T* lps, int64_t* nacc, int waves, const nka_sweep_job* jobs, int n_jobs,
+                            hipStream_t s) {
   const size_t lds = group_lds<T>(m, sa, waves, G);
   const int per_block = waves * G;
   const unsigned g = (unsigned)((sa.C + per_block - 1) / per_block);
-  hipLaunchKernelGGL((rbm_metropolis_group_kernel<T, G, JG, RSG, true>), dim3(g), dim3(64 * waves), lds, s,
-                     m, sa, sigma, lp, smp, lps, nacc);
+  if (jobs)
+    hipLaunchKernelGGL((rbm_metropolis_group_kernel_batched<T, G, JG, RSG, true>), dim3(g, n_jobs),
+                       dim3(64 * waves), lds, s, jobs, sa);
+  else
+    hipLaunchKernelGGL((rbm_metropolis_group_kernel<T, G, JG, RSG, true>), dim3(g), dim3(64 * waves), lds, s,
+                       m, sa, sigma, lp, smp, lps, nacc);
 }
 
 // cluster slots per lane rounded up to a power of two (1 for the local rule)
 template <class T, int G, int JG>
 static void launch_group(const nka_rbm& m, const SweepArgs& sa, int8_t* sigma, T* lp, int8_t* smp,
-                         T* lps, int64_t* nacc, int waves, hipStream_t s) {
+                         T* lps, int64_t* nacc, int waves, const nka_sweep_job* jobs, int n_jobs,
+                         hipStream_t s) {
   const int lpc = NKA_WAVE / G;
   const int need = sa.rule == NKA_RULE_EXCHANGE ? (sa.n_clusters + lpc - 1) / lpc : 1;
-  if (need <= 1) launch_group_rs<T, G, JG, 1>(m, sa, sigma, lp, smp, lps, nacc, waves, s);
-  else if (need <= 2) launch_group_rs<T, G, JG, 2>(m, sa, sigma, lp, smp, lps, nacc, waves, s);
-  else if (need <= 4) launch_group_rs<T, G, JG, 4>(m, sa, sigma, lp, smp, lps, nacc, waves, s);
-  else if (need <= 8) launch_group_rs<T, G, JG, 8>(m, sa, sigma, lp, smp, lps, nacc, waves, s);
-  else launch_group_rs<T, G, JG, 16>(m, sa, sigma, lp, smp, lps, nacc, waves, s);
+  if (need <= 1) launch_group_rs<T, G, JG, 1>(m, sa, sigma, lp, smp, lps, nacc, waves, jobs, n_jobs, s);
+  else if (need <= 2) launch_group_rs<T, G, JG, 2>(m, sa, sigma, lp, smp, lps, nacc, waves, jobs, n_jobs, s);
+  else if (need <= 4) launch_group_rs<T, G, JG, 4>(m, sa, sigma, lp, smp, lps, nacc, waves, jobs, n_jobs, s);
+  else if (need <= 8) launch_group_rs<T, G, JG, 8>(m, sa, sigma, lp, smp, lps, nacc, waves, jobs, n_jobs, s);
+  else launch_group_rs<T, G, JG, 16>(m, sa, sigma, lp, smp, lps, nacc, waves, jobs, n_jobs, s);
 }
 
 template <int JL, class T>
 static void launch_sweep(const nka_rbm& m, const SweepArgs& sa, int8_t* sigma, T* lp, int8_t* smp,
-                         T* lps, int64_t* nacc, hipStream_t s) {
+                         T* lps, int64_t* nacc, const nka_sweep_job* jobs, int n_jobs, hipStream_t s) {
   const int waves = 4;
   const int G = pick_group<T>(m, sa, waves);
   if (G) {
     const int lpc = NKA_WAVE / G;
     const int need = (m.n_hidden + lpc - 1) / lpc;
     const int jg = need <= 1 ? 1 : need <= 2 ? 2 : need <= 4 ? 4 : 8;
-#define NKA_GROUP_CASE(GG, JJ) \
-    if (G == GG && jg == JJ) { launch_group<T, GG, JJ>(m, sa, sigma, lp, smp, lps, nacc, waves, s); return; }
+#define NKA_GROUP_CASE(GG, JJ)                                                                  \
+    if (G == GG && jg == JJ) {                                                                  \
+      launch_group<T, GG, JJ>(m, sa, sigma, lp, smp, lps, nacc, waves, jobs, n_jobs, s);        \
+      return;                                                                                   \
+    }
     NKA_GROUP_CASE(4, 1) NKA_GROUP_CASE(4, 2) NKA_GROUP_CASE(4, 4) NKA_GROUP_CASE(4, 8)
     NKA_GROUP_CASE(2, 1) NKA_GROUP_CASE(2, 2) NKA_GROUP_CASE(2, 4) NKA_GROUP_CASE(2, 8)
 #undef NKA_GROUP_CASE
@@ -728,16 +782,29 @@ static void launch_sweep(const nka_rbm& m, const SweepArgs& sa, int8_t* sigma, T
       // units per lane rounded up to a multiple of 8 (registers: theta' and
       // two W rows per unit)
       const int jlw = (m.n_hidden + 8 * NKA_WAVE - 1) / (8 * NKA_WAVE) * 8;
-#define NKA_WIDE_CASE(J)                                                                              \
-      if (jlw == J) {                                                                                 \
-        hipLaunchKernelGGL((rbm_metropolis_wide_kernel<J>), dim3(g), dim3(64 * waves), lds, s, m, sa, \
-                           sigma, (float*)lp, smp, (float*)lps, nacc);                                \
-        return;                                                                                       \
+#define NKA_WIDE_CASE(J)                                                                                 \
+      if (jlw == J) {                                                                                    \
+        if (jobs)                                                                                        \
+          hipLaunchKernelGGL((rbm_metropolis_wide_kernel_batched<J>), dim3(g, n_jobs), dim3(64 * waves), \
+                             lds, s, jobs, sa);                                                          \
+        else                                                                                             \
+          hipLaunchKernelGGL((rbm_metropolis_wide_kernel<J>), dim3(g), dim3(64 * waves), lds, s, m, sa,  \
+                             sigma, (float*)lp, smp, (float*)lps, nacc);                                 \
+        return;                                                                                          \
       }
       NKA_WIDE_CASE(8) NKA_WIDE_CASE(16) NKA_WIDE_CASE(24) NKA_WIDE_CASE(32)
       NKA_WIDE_CASE(40) NKA_WIDE_CASE(48) NKA_WIDE_CASE(56) NKA_WIDE_CASE(64)
 #undef NKA_WIDE_CASE
     }
+  }
+  if (jobs) {
+    if (stage)
+      hipLaunchKernelGGL((rbm_metropolis_kernel_batched<T, JL, true>), dim3(g, n_jobs), dim3(64 * waves), lds,
+                         s, jobs, sa);
+    else
+      hipLaunchKernelGGL((rbm_metropolis_kernel_batched<T, JL, false>), dim3(g, n_jobs), dim3(64 * waves), lds,
+                         s, jobs, sa);
+    return;
   }
   if (stage)
     hipLaunchKernelGGL((rbm_metropolis_kernel<T, JL, true>), dim3(g), dim3(64 * waves), lds, s, m,
@@ -748,13 +815,21 @@ static void launch_sweep(const nka_rbm& m, const SweepArgs& sa, int8_t* sigma, T
 }
 template <int JL>
 static void launch_sweep_f32(const nka_rbm& m, const SweepArgs& sa, int8_t* sg, void* lp,
-                             int8_t* smp, void* lps, int64_t* na, hipStream_t s) {
-  launch_sweep<JL, float>(m, sa, sg, (float*)lp, smp, (float*)lps, na, s);
+                             int8_t* smp, void* lps, int64_t* na, const nka_sweep_job* jobs, int n_jobs,
+                             hipStream_t s) {
+  launch_sweep<JL, float>(m, sa, sg, (float*)lp, smp, (float*)lps, na, jobs, n_jobs, s);
 }
 template <int JL>
 static void launch_sweep_f64(const nka_rbm& m, const SweepArgs& sa, int8_t* sg, void* lp,
-                             int8_t* smp, void* lps, int64_t* na, hipStream_t s) {
-  launch_sweep<JL, double>(m, sa, sg, (double*)lp, smp, (double*)lps, na, s);
+                             int8_t* smp, void* lps, int64_t* na, const nka_sweep_job* jobs, int n_jobs,
+                             hipStream_t s) {
+  launch_sweep<JL, double>(m, sa, sg, (double*)lp, smp, (double*)lps, na, jobs, n_jobs, s);
+}
+
+// counter += inc for every job of a batch that keeps its counter on the device
+__global__ void advance_counters_kernel(const nka_sweep_job* __restrict__ jobs, int n_jobs, uint64_t inc) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n_jobs && jobs[k].counter_dev) jobs[k].counter_dev[0] = jobs[k].counter_dev[0] + inc;
 }
 
 }  // namespace nka
@@ -814,16 +889,58 @@ int nka_rbm_metropolis(const nka_rbm* m, int rule, const int32_t* clusters, int 
   hipStream_t s = (hipStream_t)stream;
   if (m->dtype == NKA_F32) {
     NKA_DISPATCH_JL(jl, launch_sweep_f32, *m, sa, sigma, log_prob, samples, log_prob_samples,
-                    n_accepted, s);
+                    n_accepted, (const nka_sweep_job*)nullptr, 0, s);
   } else {
     NKA_DISPATCH_JL(jl, launch_sweep_f64, *m, sa, sigma, log_prob, samples, log_prob_samples,
-                    n_accepted, s);
+                    n_accepted, (const nka_sweep_job*)nullptr, 0, s);
   }
   if (counter_dev) {
     const uint64_t inc = (uint64_t)(n_discard + chain_length) * (uint64_t)sweep_size;
     hipLaunchKernelGGL(advance_counter_kernel, dim3(1), dim3(64), 0, s, counter_dev, inc);
   }
   NKA_CHECK_LAUNCH("nka_rbm_metropolis");
+  return NKA_OK;
+}
+
+int nka_rbm_metropolis_batched(const nka_sweep_job* jobs, int n_jobs, const nka_rbm* shape, int rule,
+                               const int32_t* clusters, int n_clusters, int n_chains, int sweep_size,
+                               int n_discard, int chain_length, double machine_pow, void* stream) {
+  NKA_CHECK_ARG(shape && jobs && n_jobs > 0 && n_jobs <= 65535, "nka_rbm_metropolis_batched: bad batch");
+  NKA_CHECK_ARG(shape->n_visible > 0 && shape->n_hidden > 0 && shape->n_hidden <= kMaxJL * NKA_WAVE,
+                "nka_rbm_metropolis_batched: unsupported sizes");
+  NKA_CHECK_ARG(shape->dtype == NKA_F32 || shape->dtype == NKA_F64, "nka_rbm_metropolis_batched: bad dtype");
+  NKA_CHECK_ARG(rule == NKA_RULE_LOCAL || rule == NKA_RULE_EXCHANGE,
+                "nka_rbm_metropolis_batched: unknown rule %d", rule);
+  NKA_CHECK_ARG(rule == NKA_RULE_LOCAL ||
+                    (clusters && n_clusters > 0 && n_clusters <= kMaxClusterSlots * NKA_WAVE),
+                "nka_rbm_metropolis_batched: exchange rule needs 1..%d clusters",
+                kMaxClusterSlots * NKA_WAVE);
+  NKA_CHECK_ARG(n_chains > 0 && sweep_size >= 0 && n_discard >= 0 && chain_length >= 0,
+                "nka_rbm_metropolis_batched: bad sizes");
+  SweepArgs sa;
+  sa.rule = rule;
+  sa.clusters = clusters;
+  sa.n_clusters = rule == NKA_RULE_EXCHANGE ? n_clusters : 0;
+  sa.C = n_chains;
+  sa.sweep_size = sweep_size;
+  sa.n_discard = n_discard;
+  sa.chain_length = chain_length;
+  sa.machine_pow = machine_pow;
+  sa.k0 = sa.k1 = 0;
+  sa.counter = 0;
+  sa.counter_dev = nullptr;
+  const int jl = pick_jl(shape->n_hidden);
+  hipStream_t s = (hipStream_t)stream;
+  if (shape->dtype == NKA_F32) {
+    NKA_DISPATCH_JL(jl, launch_sweep_f32, *shape, sa, (int8_t*)nullptr, (void*)nullptr, (int8_t*)nullptr,
+                    (void*)nullptr, (int64_t*)nullptr, jobs, n_jobs, s);
+  } else {
+    NKA_DISPATCH_JL(jl, launch_sweep_f64, *shape, sa, (int8_t*)nullptr, (void*)nullptr, (int8_t*)nullptr,
+                    (void*)nullptr, (int64_t*)nullptr, jobs, n_jobs, s);
+  }
+  const uint64_t inc = (uint64_t)(n_discard + chain_length) * (uint64_t)sweep_size;
+  hipLaunchKernelGGL(advance_counters_kernel, dim3((n_jobs + 255) / 256), dim3(256), 0, s, jobs, n_jobs, inc);
+  NKA_CHECK_LAUNCH("nka_rbm_metropolis_batched");
   return NKA_OK;
 }
 

@@ -254,9 +254,13 @@ class VMC_SR(AbstractVariationalDriver):
     def _hyper(self, v):
         return v(self._step_count) if callable(v) else v
 
-    def compute_loss_and_update(self):
+    def compute_loss_and_update(self, eloc=None):
+        """(Stats, update).  ``eloc`` (n_chains, chain_length) may be given when
+        the local energies of the current samples were computed elsewhere (the
+        serving loop's shared launches); otherwise they are computed here."""
         vs = self.state
-        eloc = vs.local_estimators(self._ham)
+        if eloc is None:
+            eloc = vs.local_estimators(self._ham)
         st = statistics(eloc)
         e = eloc.reshape(-1)
         ns = e.numel()
