@@ -1072,6 +1072,13 @@ static void contract_one(const nka_ffnn& m, const int8_t* sigma, const T* h1, co
 }
 
 // y = alpha (raw - mean.v) + beta y (nka_ntk.hip)
+void launch_ntk_split(const float* t, int64_t ns, int M, int ldt, uint16_t* hi, uint16_t* lo, hipStream_t s);
+void launch_ntk_pack_spins(const int8_t* sigma, int64_t ns, int N, int nw, uint32_t* bits, hipStream_t s);
+bool launch_gram_ffnn(const uint16_t* h1h, const uint16_t* h1l, int64_t K0, int64_t ld0, const uint16_t* g2h,
+                      const uint16_t* g2l, int64_t K1, int64_t ld1, const uint16_t* d1h, const uint16_t* d1l,
+                      int64_t K2, int64_t ld2, int64_t n, const uint32_t* bits, int nw, int N, const double* v,
+                      const double* c, double hb1, double hb2, double scale, double shift, double offset,
+                      double* out, int64_t ldo, hipStream_t s);
 void launch_jvp_finish(const double* raw, int64_t ns, const double* mean, const double* v, int64_t np,
                        double* dot_slot, double alpha, double beta, double* y, hipStream_t s);
 
@@ -1222,10 +1229,22 @@ int nka_ffnn_contract(const nka_ffnn* m, const int8_t* sigma, const void* h1, co
   return NKA_OK;
 }
 
+static int64_t ffnn_ntk_base_bytes(const nka_ffnn& m, int64_t ns) {
+  const int64_t H1 = m.n_hidden1, H2 = m.n_hidden2, N = m.n_visible;
+  return (ns * (H1 + H2 + H1) + ns * N + ns * H1 + ns + 8) * 8;
+}
+static int64_t ffnn_ld8(int h) { return (h + 7) / 8 * 8; }
+
 int64_t nka_ffnn_ntk_workspace(const nka_ffnn* m, int64_t ns) {
   if (!m || ns <= 0) return 0;
-  const int64_t H1 = m->n_hidden1, H2 = m->n_hidden2, N = m->n_visible;
-  return (ns * (H1 + H2 + H1) + ns * N + ns * H1 + ns + 8) * 8 + 256;
+  int64_t b = ffnn_ntk_base_bytes(*m, ns) + 256;
+  if (m->dtype == NKA_F32) {
+    // bf16 hi + lo planes of h1, g2 and d1, and the packed spins
+    const int64_t ld = 2 * ffnn_ld8(m->n_hidden1) + ffnn_ld8(m->n_hidden2);
+    b += (2 * ns * ld * (int64_t)sizeof(uint16_t) + 255) / 256 * 256 + 256;
+    b += ns * ((m->n_visible + 31) / 32) * (int64_t)sizeof(uint32_t);
+  }
+  return b;
 }
 
 int nka_ffnn_ntk(const nka_ffnn* m, const int8_t* sigma, const void* h1, const void* d1, const void* g2, int64_t ns,
@@ -1261,6 +1280,32 @@ int nka_ffnn_ntk(const nka_ffnn* m, const int8_t* sigma, const void* h1, const v
   const int64_t np = ffnn_np(*m);
   hipLaunchKernelGGL(ffnn_ntk_vec_kernel, dim3((unsigned)std::min<int64_t>((ns + 3) / 4, 4096)), dim3(256), 0, s,
                      h1d, g2d, sigma, d1d, U1, U2, N, H1, H2, ns, mean, m->b1_off, m->b2_off, np, vv, cc);
+  const double hb1 = m->b1_off >= 0 ? 1.0 : 0.0, hb2 = m->b2_off >= 0 ? 1.0 : 0.0;
+  if (m->dtype == NKA_F32) {
+    // H, G and D on bf16 matrix cores (three products per operand pair, FP32
+    // accumulation), X and the centring exact in the epilogue
+    unsigned char* p = (unsigned char*)workspace + (ffnn_ntk_base_bytes(*m, ns) + 255) / 256 * 256;
+    const int64_t l1 = ffnn_ld8(H1), l2 = ffnn_ld8(H2);
+    uint16_t* h1h = (uint16_t*)p;
+    uint16_t* h1l = h1h + ns * l1;
+    uint16_t* g2h = h1l + ns * l1;
+    uint16_t* g2l = g2h + ns * l2;
+    uint16_t* d1h = g2l + ns * l2;
+    uint16_t* d1l = d1h + ns * l1;
+    uint32_t* bits = (uint32_t*)(p + (2 * ns * (2 * l1 + l2) * (int64_t)sizeof(uint16_t) + 255) / 256 * 256);
+    const int nw = (N + 31) / 32;
+    launch_ntk_split((const float*)h1, ns, H1, (int)l1, h1h, h1l, s);
+    launch_ntk_split((const float*)g2, ns, H2, (int)l2, g2h, g2l, s);
+    launch_ntk_split((const float*)d1, ns, H1, (int)l1, d1h, d1l, s);
+    launch_ntk_pack_spins(sigma, ns, N, nw, bits, s);
+    const int64_t T2 = (ns + 127) / 128;
+    NKA_CHECK_ARG(T2 * (T2 + 1) / 2 < (1LL << 31), "nka_ffnn_ntk: too large");
+    if (launch_gram_ffnn(h1h, h1l, l1, l1, g2h, g2l, l2, l2, d1h, d1l, l1, l1, ns, bits, nw, N, vv, cc, hb1, hb2,
+                         scale, shift, offset, out, ldo, s)) {
+      NKA_CHECK_LAUNCH("nka_ffnn_ntk");
+      return NKA_OK;
+    }
+  }
   const int64_t T = (ns + NF_T - 1) / NF_T;
   const int64_t tiles = T * (T + 1) / 2;
   NKA_CHECK_ARG(tiles < (1LL << 31), "nka_ffnn_ntk: too large");

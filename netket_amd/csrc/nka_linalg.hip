@@ -63,6 +63,16 @@ struct GramRbmEpi {
   const double* v;
   const double* c;
   double hb, ha, scale;
+  // two-layer network (NP = 3): the passes 1 and 2 take these planes, and
+  //   T_ij = scale ((hb2 + H_ij) G_ij + (hb + X_ij) D_ij - v_i - v_j + c) + offset
+  // with H = h1 h1^T (pass 0), G = g2 g2^T (pass 1), D = d1 d1^T (pass 2),
+  // X = x x^T from the packed spins
+  const uint16_t* xh1;
+  const uint16_t* xl1;
+  const uint16_t* xh2;
+  const uint16_t* xl2;
+  int64_t K1, K2, ld1, ld2;
+  double hb2;
 };
 
 // WT: wave tile edge (64 or 96); the workgroup tile is 2 WT x 2 WT (four
@@ -70,7 +80,9 @@ struct GramRbmEpi {
 // tile reads 12 operand fragments from LDS per 27 products (128-wide: 8 per
 // 12) and re-reads each operand row panel from L2 a third fewer times; it
 // needs 110 KB of LDS, so one workgroup runs per CU.
-template <bool RBM, int WT>
+// NP: products per tile, 1 (the Gram, or the RBM tangent kernel when RBM) or
+// 3 (the two-layer network's tangent kernel, RBM set), each K loop in turn.
+template <bool RBM, int WT, int NP = 1>
 __global__ void __launch_bounds__(256, 1) gram_bf16x3_kernel(const uint16_t* __restrict__ xh,
                                                           const uint16_t* __restrict__ xl, int64_t n,
                                                           int64_t K, int64_t ldx, double shift,
@@ -91,91 +103,101 @@ __global__ void __launch_bounds__(256, 1) gram_bf16x3_kernel(const uint16_t* __r
   const int wm = (wid >> 1) * WT, wn = (wid & 1) * WT;  // wave tile origin in block tile
   const bool diag_tile = (ti == tj);
 
-  floatx16 acc[MI][MI];
+  floatx16 accs[NP][MI][MI];
 #pragma unroll
-  for (int a = 0; a < MI; ++a)
+  for (int p = 0; p < NP; ++p)
 #pragma unroll
-    for (int b = 0; b < MI; ++b)
+    for (int a = 0; a < MI; ++a)
 #pragma unroll
-      for (int e = 0; e < 16; ++e) acc[a][b][e] = 0.0f;
+      for (int b = 0; b < MI; ++b)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) accs[p][a][b][e] = 0.0f;
 
-  // Each operand slab is TT rows x GB_K k = TT GB_CPR chunks of 8 bf16;
-  // thread tid moves chunks tid + 256 it (it < NIT) of each of the (up to)
-  // four planes.  The next slab is fetched into registers while the current
-  // one is used.
-  uint4 pf[NIT][4];
-  auto fetch = [&](int64_t k0) {
 #pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-      const int chunk = tid + it * 256;
-      const int r = chunk / GB_CPR, kc = (chunk % GB_CPR) * 8;
-      const int64_t kk = k0 + kc;
-      uint4 z = {0, 0, 0, 0};
-      pf[it][0] = z;
-      pf[it][1] = z;
-      pf[it][2] = z;
-      pf[it][3] = z;
-      if (i0 + r < n && kk < K) {
-        pf[it][0] = *(const uint4*)(xh + (i0 + r) * ldx + kk);
-        pf[it][1] = *(const uint4*)(xl + (i0 + r) * ldx + kk);
+  for (int pass = 0; pass < NP; ++pass) {
+    const uint16_t* xh_p = pass == 0 ? xh : pass == 1 ? ep.xh1 : ep.xh2;
+    const uint16_t* xl_p = pass == 0 ? xl : pass == 1 ? ep.xl1 : ep.xl2;
+    const int64_t K_p = pass == 0 ? K : pass == 1 ? ep.K1 : ep.K2;
+    const int64_t ld_p = pass == 0 ? ldx : pass == 1 ? ep.ld1 : ep.ld2;
+    if (pass > 0) __syncthreads();  // the previous pass's last slab has been read
+    // Each operand slab is TT rows x GB_K k = TT GB_CPR chunks of 8 bf16;
+    // thread tid moves chunks tid + 256 it (it < NIT) of each of the (up to)
+    // four planes.  The next slab is fetched into registers while the current
+    // one is used.
+    uint4 pf[NIT][4];
+    auto fetch = [&](int64_t k0) {
+  #pragma unroll
+      for (int it = 0; it < NIT; ++it) {
+        const int chunk = tid + it * 256;
+        const int r = chunk / GB_CPR, kc = (chunk % GB_CPR) * 8;
+        const int64_t kk = k0 + kc;
+        uint4 z = {0, 0, 0, 0};
+        pf[it][0] = z;
+        pf[it][1] = z;
+        pf[it][2] = z;
+        pf[it][3] = z;
+        if (i0 + r < n && kk < K_p) {
+          pf[it][0] = *(const uint4*)(xh_p + (i0 + r) * ld_p + kk);
+          pf[it][1] = *(const uint4*)(xl_p + (i0 + r) * ld_p + kk);
+        }
+        if (!diag_tile && j0 + r < n && kk < K_p) {
+          pf[it][2] = *(const uint4*)(xh_p + (j0 + r) * ld_p + kk);
+          pf[it][3] = *(const uint4*)(xl_p + (j0 + r) * ld_p + kk);
+        }
       }
-      if (!diag_tile && j0 + r < n && kk < K) {
-        pf[it][2] = *(const uint4*)(xh + (j0 + r) * ldx + kk);
-        pf[it][3] = *(const uint4*)(xl + (j0 + r) * ldx + kk);
+    };
+    fetch(0);
+    for (int64_t k0 = 0; k0 < K_p; k0 += GB_K) {
+      __syncthreads();
+  #pragma unroll
+      for (int it = 0; it < NIT; ++it) {
+        const int chunk = tid + it * 256;
+        const int r = chunk / GB_CPR, kc = (chunk % GB_CPR) * 8;
+        *(uint4*)(&sA[0][r * GB_LD + kc]) = pf[it][0];
+        *(uint4*)(&sA[1][r * GB_LD + kc]) = pf[it][1];
+        if (!diag_tile) {
+          *(uint4*)(&sB[0][r * GB_LD + kc]) = pf[it][2];
+          *(uint4*)(&sB[1][r * GB_LD + kc]) = pf[it][3];
+        }
       }
-    }
-  };
-  fetch(0);
-  for (int64_t k0 = 0; k0 < K; k0 += GB_K) {
-    __syncthreads();
-#pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-      const int chunk = tid + it * 256;
-      const int r = chunk / GB_CPR, kc = (chunk % GB_CPR) * 8;
-      *(uint4*)(&sA[0][r * GB_LD + kc]) = pf[it][0];
-      *(uint4*)(&sA[1][r * GB_LD + kc]) = pf[it][1];
-      if (!diag_tile) {
-        *(uint4*)(&sB[0][r * GB_LD + kc]) = pf[it][2];
-        *(uint4*)(&sB[1][r * GB_LD + kc]) = pf[it][3];
+      __syncthreads();
+      if (k0 + GB_K < K_p) fetch(k0 + GB_K);
+      const uint16_t* Bh = diag_tile ? sA[0] : sB[0];
+      const uint16_t* Bl = diag_tile ? sA[1] : sB[1];
+  #pragma unroll
+      for (int ks = 0; ks < GB_K; ks += 16) {
+        const int kq = ks + (lane >> 5) * 8;
+        bf16x8 ah[MI], al[MI], bh[MI], bl[MI];
+  #pragma unroll
+        for (int a = 0; a < MI; ++a) {
+          const int row = wm + a * 32 + (lane & 31);
+          ah[a] = *(const bf16x8*)(&sA[0][row * GB_LD + kq]);
+          al[a] = *(const bf16x8*)(&sA[1][row * GB_LD + kq]);
+        }
+  #pragma unroll
+        for (int b = 0; b < MI; ++b) {
+          const int row = wn + b * 32 + (lane & 31);
+          bh[b] = *(const bf16x8*)(&Bh[row * GB_LD + kq]);
+          bl[b] = *(const bf16x8*)(&Bl[row * GB_LD + kq]);
+        }
+        // small products first, four independent accumulators between
+        // consecutive updates of the same accumulator
+  #pragma unroll
+        for (int a = 0; a < MI; ++a)
+  #pragma unroll
+          for (int b = 0; b < MI; ++b)
+            accs[pass][a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[b], accs[pass][a][b], 0, 0, 0);
+  #pragma unroll
+        for (int a = 0; a < MI; ++a)
+  #pragma unroll
+          for (int b = 0; b < MI; ++b)
+            accs[pass][a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[b], accs[pass][a][b], 0, 0, 0);
+  #pragma unroll
+        for (int a = 0; a < MI; ++a)
+  #pragma unroll
+          for (int b = 0; b < MI; ++b)
+            accs[pass][a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[b], accs[pass][a][b], 0, 0, 0);
       }
-    }
-    __syncthreads();
-    if (k0 + GB_K < K) fetch(k0 + GB_K);
-    const uint16_t* Bh = diag_tile ? sA[0] : sB[0];
-    const uint16_t* Bl = diag_tile ? sA[1] : sB[1];
-#pragma unroll
-    for (int ks = 0; ks < GB_K; ks += 16) {
-      const int kq = ks + (lane >> 5) * 8;
-      bf16x8 ah[MI], al[MI], bh[MI], bl[MI];
-#pragma unroll
-      for (int a = 0; a < MI; ++a) {
-        const int row = wm + a * 32 + (lane & 31);
-        ah[a] = *(const bf16x8*)(&sA[0][row * GB_LD + kq]);
-        al[a] = *(const bf16x8*)(&sA[1][row * GB_LD + kq]);
-      }
-#pragma unroll
-      for (int b = 0; b < MI; ++b) {
-        const int row = wn + b * 32 + (lane & 31);
-        bh[b] = *(const bf16x8*)(&Bh[row * GB_LD + kq]);
-        bl[b] = *(const bf16x8*)(&Bl[row * GB_LD + kq]);
-      }
-      // small products first, four independent accumulators between
-      // consecutive updates of the same accumulator
-#pragma unroll
-      for (int a = 0; a < MI; ++a)
-#pragma unroll
-        for (int b = 0; b < MI; ++b)
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
-#pragma unroll
-      for (int a = 0; a < MI; ++a)
-#pragma unroll
-        for (int b = 0; b < MI; ++b)
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
-#pragma unroll
-      for (int a = 0; a < MI; ++a)
-#pragma unroll
-        for (int b = 0; b < MI; ++b)
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
     }
   }
   // store: element e of lane l in a 32x32 tile is (row (e/4)*8 + (l/32)*4 + e%4, col l%32)
@@ -189,7 +211,7 @@ __global__ void __launch_bounds__(256, 1) gram_bf16x3_kernel(const uint16_t* __r
           const int64_t gi = i0 + wm + a * 32 + (e / 4) * 8 + (lane >> 5) * 4 + (e & 3);
           const int64_t gj = j0 + wn + b * 32 + (lane & 31);
           if (gi < n && gj <= gi) {
-            double v = (double)acc[a][b][e] + offset;
+            double v = (double)accs[0][a][b][e] + offset;
             if (gi == gj) v += shift;
             out[gi * ldo + gj] = v;
           }
@@ -245,8 +267,16 @@ __global__ void __launch_bounds__(256, 1) gram_bf16x3_kernel(const uint16_t* __r
           const int64_t gj = j0 + wn + b * 32 + (lane & 31);
           if (gi < n && gj <= gi) {
             const double S = (double)(ep.N - 2 * cnt[a][b][e]);
-            const double Th = (double)acc[a][b][e];
-            double v = ep.scale * ((ep.hb + S) * Th + ep.ha * S - ep.v[gi] - ep.v[gj] + cc) + offset;
+            double v;
+            if constexpr (NP == 1) {
+              const double Th = (double)accs[0][a][b][e];
+              v = ep.scale * ((ep.hb + S) * Th + ep.ha * S - ep.v[gi] - ep.v[gj] + cc) + offset;
+            } else {
+              const double Hh = (double)accs[0][a][b][e];
+              const double Gg = (double)accs[NP - 2][a][b][e];
+              const double Dd = (double)accs[NP - 1][a][b][e];
+              v = ep.scale * ((ep.hb2 + Hh) * Gg + (ep.hb + S) * Dd - ep.v[gi] - ep.v[gj] + cc) + offset;
+            }
             if (gi == gj) v += shift;
             out[gi * ldo + gj] = v;
           }
@@ -1501,6 +1531,25 @@ bool launch_gram_rbm(const uint16_t* xh, const uint16_t* xl, int64_t n, int64_t 
   GramRbmEpi ep{bits, nw, nwp, N, v, c, hb, ha, scale};
   hipLaunchKernelGGL((gram_bf16x3_kernel<true, kGramRbmWT>), dim3(grid_mult8(tiles)), dim3(256), 0, s, xh, xl,
                      n, K, ldx, shift, offset, out, ldo, tiles, ep);
+  return true;
+}
+
+// The two-layer network's tangent kernel on the bf16x3 Gram tiles (called from
+// the network's NTK entry point): H, G and D from the split h1, g2 and d1
+// (hi + lo planes, n rows each), X from the packed spins.  Returns false when
+// the spins do not fit the epilogue.
+bool launch_gram_ffnn(const uint16_t* h1h, const uint16_t* h1l, int64_t K0, int64_t ld0, const uint16_t* g2h,
+                      const uint16_t* g2l, int64_t K1, int64_t ld1, const uint16_t* d1h, const uint16_t* d1l,
+                      int64_t K2, int64_t ld2, int64_t n, const uint32_t* bits, int nw, int N, const double* v,
+                      const double* c, double hb1, double hb2, double scale, double shift, double offset,
+                      double* out, int64_t ldo, hipStream_t s) {
+  const int nwp = (nw + 3) / 4 * 4;
+  if (nwp > kGramRbmMaxNwp) return false;
+  const int64_t T = (n + 2 * kGramRbmWT - 1) / (2 * kGramRbmWT);
+  const int64_t tiles = T * (T + 1) / 2;
+  GramRbmEpi ep{bits, nw, nwp, N, v, c, hb1, 0.0, scale, g2h, g2l, d1h, d1l, K1, K2, ld1, ld2, hb2};
+  hipLaunchKernelGGL((gram_bf16x3_kernel<true, kGramRbmWT, 3>), dim3(grid_mult8(tiles)), dim3(256), 0, s, h1h,
+                     h1l, n, K0, ld0, shift, offset, out, ldo, tiles, ep);
   return true;
 }
 

@@ -92,6 +92,16 @@ __global__ void ntk_pack_spins_kernel(const int8_t* __restrict__ sigma, int64_t 
   }
 }
 
+// Host launchers of the two kernels above, for the other tangent kernels.
+void launch_ntk_split(const float* t, int64_t ns, int M, int ldt, uint16_t* hi, uint16_t* lo, hipStream_t s) {
+  const unsigned g = (unsigned)std::min<int64_t>((ns * ldt + 255) / 256, 65536);
+  hipLaunchKernelGGL(ntk_split_kernel, dim3(g), dim3(256), 0, s, t, ns, M, ldt, hi, lo);
+}
+void launch_ntk_pack_spins(const int8_t* sigma, int64_t ns, int N, int nw, uint32_t* bits, hipStream_t s) {
+  const unsigned g = (unsigned)std::min<int64_t>((ns * nw + 255) / 256, 65536);
+  hipLaunchKernelGGL(ntk_pack_spins_kernel, dim3(g), dim3(256), 0, s, sigma, ns, N, nw, bits);
+}
+
 // dot = a.b over n entries in one workgroup, summed in a fixed order.
 __global__ void __launch_bounds__(1024) dot_f64_kernel(const double* __restrict__ a, const double* __restrict__ b,
                                                        int64_t n, double* __restrict__ dot) {
