@@ -401,7 +401,7 @@ def cpu_baseline_heisenberg_l64(rec, sample_budget_s: float = 30.0):
     hi = nk.hilbert.Spin(0.5, N=L, total_sz=0)
     H = nk.operator.Heisenberg(hi, g)
     terms = dict(zip([tuple(a) for a in H.acting_on], H.operators))
-    clusters = nk.sampler.compute_clusters(g, 1).astype(np.int64)
+    clusters = nk.sampler.compute_clusters(g, 1)[0].astype(np.int64)
     rng = np.random.default_rng(0)
     C = 4096
     sigma = np.ones((C, L), dtype=np.int8)

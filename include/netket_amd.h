@@ -184,7 +184,7 @@ int nka_ffnn_local_energy(const nka_ffnn* m, int op_kind, const void* op, const 
 /* Metropolis sweeps for the network, with the arguments and semantics of
  * nka_rbm_metropolis (netket/sampler/metropolis.py:382-505). */
 int nka_ffnn_metropolis(const nka_ffnn* m, int rule, const int32_t* clusters, int n_clusters,
-                        int8_t* sigma, void* log_prob, int n_chains, int sweep_size, int n_discard,
+                        const uint32_t* cluster_weights, int8_t* sigma, void* log_prob, int n_chains, int sweep_size, int n_discard,
                         int chain_length, double machine_pow, uint64_t seed, uint64_t counter,
                         uint64_t* counter_dev, int8_t* samples, void* log_prob_samples,
                         int64_t* n_accepted, void* stream);
@@ -252,9 +252,14 @@ int nka_random_states(int8_t* sigma, int n_chains, int n_sites, int n_down, uint
  * base counter is read from that device word instead and advanced there by
  * (n_discard + chain_length) * sweep_size after the sweep, so a captured
  * graph replays with fresh draws.  clusters: (n_clusters, 2) int32 (exchange
- * only).  n_accepted (n_chains,) is incremented. */
+ * only).  cluster_weights (exchange only, may be NULL for equal weights):
+ * (n_clusters,) integers in [1, 2^20], device memory; a differing cluster is
+ * proposed with probability weight / (total weight of the differing
+ * clusters) and the log-correction is log total - log total' (ExchangeRule's
+ * `probabilities`, netket/sampler/rules/exchange.py:86-90, 147-184).
+ * n_accepted (n_chains,) is incremented. */
 int nka_rbm_metropolis(const nka_rbm* m, int rule, const int32_t* clusters, int n_clusters,
-                       int8_t* sigma, void* log_prob, int n_chains, int sweep_size,
+                       const uint32_t* cluster_weights, int8_t* sigma, void* log_prob, int n_chains, int sweep_size,
                        int n_discard, int chain_length, double machine_pow, uint64_t seed,
                        uint64_t counter, uint64_t* counter_dev, int8_t* samples,
                        void* log_prob_samples, int64_t* n_accepted, void* stream);

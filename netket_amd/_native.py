@@ -122,7 +122,7 @@ SIGNATURES = {
     "nka_ffnn_log_psi": (i32, [C.POINTER(NkaFfnn), vp, i64, vp, vp]),
     "nka_ffnn_local_energy_workspace": (i64, [C.POINTER(NkaFfnn), i32, vp, i64]),
     "nka_ffnn_local_energy": (i32, [C.POINTER(NkaFfnn), i32, vp, vp, i64, vp, vp, vp, i64, vp]),
-    "nka_ffnn_metropolis": (i32, [C.POINTER(NkaFfnn), i32, vp, i32, vp, vp, i32, i32, i32, i32, f64,
+    "nka_ffnn_metropolis": (i32, [C.POINTER(NkaFfnn), i32, vp, i32, vp, vp, vp, i32, i32, i32, i32, f64,
                                   C.c_uint64, C.c_uint64, vp, vp, vp, vp, vp]),
     "nka_ffnn_prepare_workspace": (i64, [C.POINTER(NkaFfnn), i64]),
     "nka_ffnn_prepare": (i32, [C.POINTER(NkaFfnn), vp, i64, vp, vp, vp, vp, vp]),
@@ -140,8 +140,8 @@ SIGNATURES = {
     "nka_rbm_local_energy_batched": (i32, [vp, i32, C.POINTER(NkaRbm), C.POINTER(NkaIsing), i64, vp]),
     "nka_rbm_metropolis": (
         i32,
-        [C.POINTER(NkaRbm), i32, vp, i32, vp, vp, i32, i32, i32, i32, f64, u64, u64, vp, vp, vp, vp,
-         vp],
+        [C.POINTER(NkaRbm), i32, vp, i32, vp, vp, vp, i32, i32, i32, i32, f64, u64, u64, vp, vp, vp,
+         vp, vp],
     ),
     "nka_stats_moments": (i32, [vp, i32, i32, vp, vp]),
     "nka_rbm_prepare": (i32, [C.POINTER(NkaRbm), vp, i64, vp, vp, vp]),

@@ -132,6 +132,42 @@ __device__ inline int kth_set_bit(uint64_t m, int k) {
   return pos;
 }
 
+// Sum of a 32-bit integer over the wavefront, in every lane (exact).
+__device__ inline uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off);
+  return v;
+}
+
+// Weighted choice among the clusters a wavefront holds in slots (cluster
+// lane + 64 r carries integer weight w[r], 0 when it cannot be picked): the
+// cluster whose cumulative weight, in cluster order, first exceeds k
+// (0 <= k < the total).  Integer sums, so the choice is exact; -1 if none.
+template <int R>
+__device__ inline int weighted_pick(const uint32_t (&w)[R], int RS, uint32_t k) {
+  const int lane = lane_id();
+  int sel = -1;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (sel < 0 && r < RS) {
+      const uint32_t tot = wave_sum_u32(w[r]);
+      if (k < tot) {
+        uint32_t inc = w[r];   // inclusive prefix sum over the lanes
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          const uint32_t t = (uint32_t)__shfl_up((int)inc, off);
+          if (lane >= off) inc += t;
+        }
+        const uint64_t over = __ballot(inc > k);
+        sel = r * 64 + (int)__builtin_ctzll(over);
+      } else {
+        k -= tot;
+      }
+    }
+  }
+  return sel;
+}
+
 // Order this lane's LDS writes before later LDS reads of other lanes in the
 // same wavefront.
 __device__ inline void wave_lds_sync() {

@@ -673,6 +673,7 @@ struct FfSweepArgs {
   int rule;
   const int32_t* clusters;
   int n_clusters;
+  const uint32_t* weights;      // exchange: integer weight per cluster (NULL: uniform)
   int C, sweep_size, n_discard, chain_length;
   double machine_pow;
   uint32_t k0, k1;
@@ -761,6 +762,43 @@ __global__ void __launch_bounds__(FF_THREADS) ffnn_metropolis_kernel(nka_ffnn m,
         T corr = (T)0;
         if (!exch) {
           s0 = (int)bounded(uniform_value<uint32_t>(rnd.x), (uint32_t)N);
+        } else if (sa.weights) {
+          // weighted choice among the differing clusters (as the RBM sampler)
+          uint32_t wd[FF_MAXCL];
+          uint32_t tot = 0;
+#pragma unroll
+          for (int rr = 0; rr < FF_MAXCL; ++rr) {
+            const bool d = rr < RS && cu[rr] >= 0 && x[cu[rr]] != x[cv[rr]];
+            wd[rr] = d ? sa.weights[lane + 64 * rr] : 0u;
+            if (rr < RS) tot += wave_sum_u32(wd[rr]);
+          }
+          tot = uniform_value<uint32_t>(tot);
+          if (tot == 0) {
+            nf = 0;
+          } else {
+            const uint32_t k = bounded(uniform_value<uint32_t>(rnd.x), tot);
+            int sel = uniform_value<int>(weighted_pick<FF_MAXCL>(wd, RS, k));
+            if (sel < 0 || sel >= sa.n_clusters) sel = 0;  // unreachable: k < tot
+            s0 = sa.clusters[2 * sel];
+            s1 = sa.clusters[2 * sel + 1];
+            nf = 2;
+            uint32_t tot_new = 0;
+#pragma unroll
+            for (int rr = 0; rr < FF_MAXCL; ++rr) {
+              if (rr < RS) {
+                bool d = false;
+                if (cu[rr] >= 0) {
+                  const int ua = cu[rr], ub = cv[rr];
+                  const int8_t xa = ua == s0 ? x[s1] : (ua == s1 ? x[s0] : x[ua]);
+                  const int8_t xb = ub == s0 ? x[s1] : (ub == s1 ? x[s0] : x[ub]);
+                  d = xa != xb;
+                }
+                tot_new += wave_sum_u32(d ? sa.weights[lane + 64 * rr] : 0u);
+              }
+            }
+            tot_new = uniform_value<uint32_t>(tot_new);
+            corr = uniform_value<T>(log_t<T>((T)tot) - log_t<T>((T)tot_new));
+          }
         } else {
           uint64_t bal[FF_MAXCL];
           int n_diff = 0;
@@ -1128,7 +1166,8 @@ int nka_ffnn_local_energy(const nka_ffnn* m, int op_kind, const void* op, const 
   return NKA_OK;
 }
 
-int nka_ffnn_metropolis(const nka_ffnn* m, int rule, const int32_t* clusters, int n_clusters, int8_t* sigma,
+int nka_ffnn_metropolis(const nka_ffnn* m, int rule, const int32_t* clusters, int n_clusters,
+                        const uint32_t* cluster_weights, int8_t* sigma,
                         void* log_prob, int n_chains, int sweep_size, int n_discard, int chain_length,
                         double machine_pow, uint64_t seed, uint64_t counter, uint64_t* counter_dev,
                         int8_t* samples, void* log_prob_samples, int64_t* n_accepted, void* stream) {
@@ -1146,6 +1185,7 @@ int nka_ffnn_metropolis(const nka_ffnn* m, int rule, const int32_t* clusters, in
   sa.rule = rule;
   sa.clusters = clusters;
   sa.n_clusters = rule == NKA_RULE_EXCHANGE ? n_clusters : 0;
+  sa.weights = rule == NKA_RULE_EXCHANGE ? cluster_weights : nullptr;
   sa.C = n_chains;
   sa.sweep_size = sweep_size;
   sa.n_discard = n_discard;

@@ -61,6 +61,7 @@ struct SweepArgs {
   int rule;
   const int32_t* clusters;
   int n_clusters;
+  const uint32_t* weights;      // exchange: integer weight per cluster (NULL: uniform)
   int C, sweep_size, n_discard, chain_length;
   double machine_pow;
   uint32_t k0, k1;
@@ -136,6 +137,46 @@ __device__ __forceinline__ void rbm_metropolis_kernel_body(const nka_rbm& m, con
       if (!exch) {
         sites[0] = (int)bounded(uniform_value<uint32_t>(rnd.x), (uint32_t)N);
         nf = 1;
+      } else if (sa.weights) {
+        // weighted choice: cluster cl of the differing ones with probability
+        // weights[cl] / (their total); correction log total - log total'
+        uint32_t wd[kMaxClusterSlots];
+        uint32_t tot = 0;
+#pragma unroll
+        for (int r = 0; r < kMaxClusterSlots; ++r) {
+          const bool d = r < RS && cu[r] >= 0 && xs[cu[r]] != xs[cv[r]];
+          wd[r] = d ? sa.weights[lane + NKA_WAVE * r] : 0u;
+          if (r < RS) tot += wave_sum_u32(wd[r]);
+        }
+        tot = uniform_value<uint32_t>(tot);
+        if (tot == 0) {
+          valid = false;
+          nf = 0;
+        } else {
+          const uint32_t k = bounded(uniform_value<uint32_t>(rnd.x), tot);
+          int sel = uniform_value<int>(weighted_pick<kMaxClusterSlots>(wd, RS, k));
+          if (sel < 0 || sel >= sa.n_clusters) sel = 0;  // unreachable: k < tot
+          const int su = sa.clusters[2 * sel], sv = sa.clusters[2 * sel + 1];
+          sites[0] = su;
+          sites[1] = sv;
+          nf = 2;
+          uint32_t tot_new = 0;
+#pragma unroll
+          for (int r = 0; r < kMaxClusterSlots; ++r) {
+            if (r < RS) {
+              bool d = false;
+              if (cu[r] >= 0) {
+                const int a = cu[r], bb = cv[r];
+                const int8_t xa = a == su ? xs[sv] : (a == sv ? xs[su] : xs[a]);
+                const int8_t xb = bb == su ? xs[sv] : (bb == sv ? xs[su] : xs[bb]);
+                d = xa != xb;
+              }
+              tot_new += wave_sum_u32(d ? sa.weights[lane + NKA_WAVE * r] : 0u);
+            }
+          }
+          tot_new = uniform_value<uint32_t>(tot_new);
+          corr = uniform_value<T>(log_t<T>((T)tot) - log_t<T>((T)tot_new));
+        }
       } else {
         // differing clusters of the current state
         uint64_t bal[kMaxClusterSlots];
@@ -712,6 +753,7 @@ static int pick_group(const nka_rbm& m, const SweepArgs& sa, int waves) {
   // SIMD (the chains are latency-bound): measured on MI355X with 4096 chains,
   // M = 128: two chains per wavefront 0.79 ms, four 1.05 ms
   const bool four_ok = (int64_t)sa.C >= (int64_t)4 * 8 * device_cus();
+  if (sa.weights) return 0;  // weighted exchange: one chain per wavefront
   for (int G : {4, 2}) {
     if (G == 4 && !four_ok) continue;
     const int lpc = NKA_WAVE / G;
@@ -852,7 +894,7 @@ int nka_random_states(int8_t* sigma, int n_chains, int n_sites, int n_down, uint
 }
 
 int nka_rbm_metropolis(const nka_rbm* m, int rule, const int32_t* clusters, int n_clusters,
-                       int8_t* sigma, void* log_prob, int n_chains, int sweep_size,
+                       const uint32_t* cluster_weights, int8_t* sigma, void* log_prob, int n_chains, int sweep_size,
                        int n_discard, int chain_length, double machine_pow, uint64_t seed,
                        uint64_t counter, uint64_t* counter_dev, int8_t* samples,
                        void* log_prob_samples, int64_t* n_accepted, void* stream) {
@@ -876,6 +918,7 @@ int nka_rbm_metropolis(const nka_rbm* m, int rule, const int32_t* clusters, int 
   sa.rule = rule;
   sa.clusters = clusters;
   sa.n_clusters = rule == NKA_RULE_EXCHANGE ? n_clusters : 0;
+  sa.weights = rule == NKA_RULE_EXCHANGE ? cluster_weights : nullptr;
   sa.C = n_chains;
   sa.sweep_size = sweep_size;
   sa.n_discard = n_discard;
@@ -921,6 +964,7 @@ int nka_rbm_metropolis_batched(const nka_sweep_job* jobs, int n_jobs, const nka_
   sa.rule = rule;
   sa.clusters = clusters;
   sa.n_clusters = rule == NKA_RULE_EXCHANGE ? n_clusters : 0;
+  sa.weights = nullptr;
   sa.C = n_chains;
   sa.sweep_size = sweep_size;
   sa.n_discard = n_discard;

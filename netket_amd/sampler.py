@@ -47,38 +47,76 @@ class ExchangeRule(MetropolisRule):
     Clusters come from ``clusters`` or, given a ``graph``, from all pairs at
     graph distance ``<= d_max`` in row-major order with ``i < j``
     (``compute_clusters``, netket/sampler/rules/exchange.py:190-205).
+
+    ``probabilities`` (netket/sampler/rules/exchange.py:86-90, 116-141) gives
+    each cluster (or, with ``graph``, each graph distance 1..d_max) a relative
+    weight; a differing cluster is proposed with probability proportional to
+    it and the move carries the log-correction log W(x) - log W(x'), W the
+    total weight of the differing clusters.  The GPU sampler draws with
+    integer weights, ``round(2^20 p / max p)`` (at least 1): the choice and the
+    totals are then exact integer arithmetic, and the chain's stationary
+    distribution stays |psi|^2 exactly, since the same weights enter the
+    correction; only the proposal probabilities are rounded (relative 2^-20).
     """
 
     kind = nat.RULE_EXCHANGE
+    WEIGHT_BITS = 20
 
     def __init__(self, *, clusters=None, graph=None, d_max: int = 1, probabilities=None):
         if probabilities is not None:
-            raise NotImplementedError(
-                "weighted cluster probabilities are not supported on the GPU sampler")
+            probabilities = np.atleast_1d(np.asarray(probabilities, dtype=np.float64))
+            if not np.all(probabilities > 0):
+                raise ValueError("Probabilities must be positive")
         if clusters is None and graph is not None:
-            clusters = compute_clusters(graph, d_max)
+            clusters, probabilities = compute_clusters(graph, d_max, probabilities)
         elif not (clusters is not None and graph is None):
             raise ValueError("You must either provide the list of exchange-clusters or a netket graph")
         self.clusters = np.asarray(clusters, dtype=np.int32).reshape(-1, 2)
+        if probabilities is not None and len(probabilities) != len(self.clusters):
+            raise TypeError("Number of clusters and probabilities don't match: "
+                            f"{len(self.clusters)} != {len(probabilities)}")
+        self.probabilities = probabilities
+        self.weights = None if probabilities is None else integer_weights(probabilities, self.WEIGHT_BITS)
         self._dev = {}
 
     def device_clusters(self):
+        return self._device("clusters", self.clusters)
+
+    def device_weights(self):
+        """The integer cluster weights on the current device (None: uniform)."""
+        return None if self.weights is None else self._device("weights", self.weights.view(np.int32))
+
+    def _device(self, name, arr):
         dev = _ops.device()
-        t = self._dev.get(dev)
+        t = self._dev.get((name, dev))
         if t is None:
-            t = torch.as_tensor(self.clusters, device=dev).contiguous()
-            self._dev[dev] = t
+            t = torch.as_tensor(arr, device=dev).contiguous()
+            self._dev[(name, dev)] = t
         return t
 
     def __repr__(self):
         return f"ExchangeRule(# of clusters: {len(self.clusters)})"
 
 
-def compute_clusters(graph, d_max: int) -> np.ndarray:
+def integer_weights(p, bits: int = 20) -> np.ndarray:
+    """round(2^bits p / max p), at least 1, as uint32."""
+    p = np.asarray(p, dtype=np.float64)
+    w = np.rint(p / p.max() * float(1 << bits))
+    return np.maximum(w, 1.0).astype(np.uint32)
+
+
+def compute_clusters(graph, d_max: int, prob=None):
+    """(clusters, per-cluster probabilities or None); ``prob`` has one entry per
+    graph distance 1..d_max (netket/sampler/rules/exchange.py:190-205)."""
     distances = np.asarray(graph.distances())
     clusters = np.argwhere(distances <= d_max)
     clusters = clusters[clusters[:, 0] < clusters[:, 1]]
-    return clusters.astype(np.int32)
+    if prob is not None:
+        prob = np.asarray(prob, dtype=np.float64)
+        if prob.shape != (d_max,):
+            raise ValueError(f"Expected d_max = {d_max} probabilities, got {prob}")
+        prob = prob[distances[clusters[:, 0], clusters[:, 1]].astype(np.int64) - 1]
+    return clusters.astype(np.int32), prob
 
 
 @dataclass
@@ -200,13 +238,17 @@ class MetropolisSampler:
         lps = (torch.empty((self.n_chains, chain_length), dtype=mt, device=dev)
                if return_log_probabilities else None)
         m = machine.native(parameters, N)
+        w_ptr = C.c_void_p(0)
         if isinstance(self.rule, ExchangeRule):
             cl = self.rule.device_clusters()
             cl_ptr, n_cl = _ops.ptr(cl), int(cl.shape[0])
+            wt = self.rule.device_weights()
+            if wt is not None:
+                w_ptr = _ops.ptr(wt)
         else:
             cl_ptr, n_cl = C.c_void_p(0), 0
         with _ops.TIMER.region("metropolis"):
-            nat.call(machine.metropolis_symbol, C.byref(m), self.rule.kind, cl_ptr, n_cl,
+            nat.call(machine.metropolis_symbol, C.byref(m), self.rule.kind, cl_ptr, n_cl, w_ptr,
                      _ops.ptr(state.sigma), _ops.ptr(state.log_prob), self.n_chains,
                      self.sweep_size, int(n_discard), int(chain_length), float(self.machine_pow),
                      state.seed & (2**64 - 1), state.counter, _ops.ptr(state.counter_dev),

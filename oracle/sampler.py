@@ -5,7 +5,9 @@
 (netket/sampler/rules/local.py:40-49) or ExchangeRule
 (netket/sampler/rules/exchange.py:143-218), driven by the same Philox4x32-10
 draws as the GPU kernel (proposal q of chain c at counter (q, c, "METR")):
-word 0 picks the site / the k-th differing cluster, words 1-2 the uniform.
+word 0 picks the site / the k-th differing cluster (with integer cluster
+weights: the differing cluster whose cumulative weight first exceeds a draw
+uniform in [0, total weight)), words 1-2 the uniform.
 log psi is recomputed with a full forward per proposal (the reference
 algorithm), in FP64.
 
@@ -41,23 +43,34 @@ def random_states(C: int, N: int, n_down: int | None, seed: int, counter: int = 
     return out
 
 
-def exchange_moves(x: np.ndarray, clusters: np.ndarray):
-    """Proposals of the exchange rule from x: (list of x', log-correction list).
+def exchange_moves(x: np.ndarray, clusters: np.ndarray, weights=None):
+    """Proposals of the exchange rule from x: (list of x', log-corrections,
+    proposal probabilities).
 
-    Each x' swaps the two sites of one cluster whose spins differ; it is drawn
-    with probability 1 / len(list); the correction is
-    log n_diff(x) - log n_diff(x') (netket/sampler/rules/exchange.py:143-184).
+    Each x' swaps the two sites of one cluster whose spins differ, drawn with
+    probability w_c / W(x), W(x) the total weight of the differing clusters
+    (w = 1 without weights); the correction is log W(x) - log W(x')
+    (netket/sampler/rules/exchange.py:143-184, with ``probabilities`` as w).
     """
+    w = np.ones(len(clusters)) if weights is None else np.asarray(weights, dtype=np.float64)
     diff = np.nonzero(x[clusters[:, 0]] != x[clusters[:, 1]])[0]
-    moves, corrs = [], []
+    tot = float(w[diff].sum())
+    moves, corrs, probs = [], [], []
     for c in diff:
         u_, v_ = clusters[c]
         xp = x.copy()
         xp[u_], xp[v_] = x[v_], x[u_]
-        n_new = int(np.count_nonzero(xp[clusters[:, 0]] != xp[clusters[:, 1]]))
+        tot_new = float(w[np.nonzero(xp[clusters[:, 0]] != xp[clusters[:, 1]])[0]].sum())
         moves.append(xp)
-        corrs.append(np.log(diff.size) - np.log(n_new))
-    return moves, corrs
+        corrs.append(np.log(tot) - np.log(tot_new))
+        probs.append(w[c] / tot)
+    return moves, corrs, probs
+
+
+def pick_weighted(w: np.ndarray, k: int) -> int:
+    """Index of the entry whose cumulative weight (in order) first exceeds k,
+    0 <= k < sum(w); each index i is picked by exactly w[i] values of k."""
+    return int(np.searchsorted(np.cumsum(np.asarray(w, dtype=np.int64)), k, side="right"))
 
 
 def local_moves(x: np.ndarray):
@@ -67,12 +80,12 @@ def local_moves(x: np.ndarray):
         xp = x.copy()
         xp[i] = -xp[i]
         moves.append(xp)
-    return moves, [0.0] * x.size
+    return moves, [0.0] * x.size, [1.0 / x.size] * x.size
 
 
 def metropolis(params, sigma0: np.ndarray, *, rule: str, clusters=None, n_discard: int,
                chain_length: int, sweep_size: int, seed: int, counter: int,
-               machine_pow: float = 2.0, log_psi_fn=None, chain_ids=None):
+               machine_pow: float = 2.0, log_psi_fn=None, chain_ids=None, weights=None):
     """Returns (samples (C, L, N), final sigma, n_accepted per chain).  The
     ansatz is the RBM of ``params`` unless ``log_psi_fn(params, x)`` is given.
     ``chain_ids`` (default 0..C-1) are the chain indices of the rows of
@@ -100,7 +113,7 @@ def metropolis(params, sigma0: np.ndarray, *, rule: str, clusters=None, n_discar
                     xp = x.copy()
                     xp[k] = -xp[k]
                     corr = 0.0
-                else:
+                elif weights is None:
                     # the k-th differing cluster (ascending cluster order), as exchange_moves lists them
                     diff = np.nonzero(x[clusters[:, 0]] != x[clusters[:, 1]])[0]
                     if diff.size == 0:
@@ -111,6 +124,20 @@ def metropolis(params, sigma0: np.ndarray, *, rule: str, clusters=None, n_discar
                     xp[u_], xp[v_] = x[v_], x[u_]
                     n_new = int(np.count_nonzero(xp[clusters[:, 0]] != xp[clusters[:, 1]]))
                     corr = np.log(diff.size) - np.log(n_new)
+                else:
+                    # integer weights: k uniform in [0, W(x)), the differing cluster
+                    # whose cumulative weight (ascending cluster order) first exceeds k
+                    wi = np.asarray(weights, dtype=np.int64)
+                    diff = np.nonzero(x[clusters[:, 0]] != x[clusters[:, 1]])[0]
+                    tot = int(wi[diff].sum())
+                    if tot == 0:
+                        continue
+                    c_ = diff[pick_weighted(wi[diff], philox.bounded(r[0], tot))]
+                    u_, v_ = clusters[c_]
+                    xp = x.copy()
+                    xp[u_], xp[v_] = x[v_], x[u_]
+                    tot_new = int(wi[np.nonzero(xp[clusters[:, 0]] != xp[clusters[:, 1]])[0]].sum())
+                    corr = np.log(float(tot)) - np.log(float(tot_new))
                 lpp = machine_pow * log_psi_(params, xp[None])[0]
                 u = philox.uniform01_f64(r)
                 if u < np.exp(lpp - lp + corr):

@@ -184,28 +184,35 @@ def test_vmc_minsr_spring_momentum_two_steps(gpu):
 
 
 @pytest.mark.parametrize("rule,H1,H2,scale", [("local", 9, 7, 0.4), ("exchange", 9, 7, 0.4),
-                                               ("exchange", 420, 30, 0.05)])
+                                               ("exchange", 420, 30, 0.05), ("exchange-weighted", 9, 7, 0.4)])
 def test_chains_identical_to_oracle_fp64(gpu, rule, H1, H2, scale):
     # H1 = 420: 53 first-layer units per thread, the 56-unit sampler instance
     print(f"case ffnn chains {rule} H1={H1}", flush=True)
     N = 10
     model, flat, p = _model(N, H1, H2, np.float64, 3, scale)
     g = nk.graph.Chain(N)
+    weights = None
     if rule == "local":
         hi = nk.hilbert.Spin(0.5, N=N)
         sa = nk.sampler.MetropolisLocal(hi, n_chains=40, sweep_size=7)
         clusters = None
-    else:
+    elif rule == "exchange":
         hi = nk.hilbert.Spin(0.5, N=N, total_sz=0)
         sa = nk.sampler.MetropolisExchange(hi, graph=g, n_chains=40, sweep_size=7)
         clusters = sa.rule.clusters
+    else:
+        hi = nk.hilbert.Spin(0.5, N=N, total_sz=0)
+        sa = nk.sampler.MetropolisExchange(hi, graph=g, d_max=2, probabilities=[0.4, 1.0], n_chains=40,
+                                           sweep_size=7)
+        clusters, weights = sa.rule.clusters, sa.rule.weights
+        rule = "exchange"
     st = sa.init_state(model, flat, seed=99)
     sigma0 = st.sigma.cpu().numpy().copy()
     counter0 = st.counter
     samples, st = sa.sample_fused(model, flat, st, 3, 9)
     ref, ref_sigma, ref_acc = osamp.metropolis(p, sigma0, rule=rule, clusters=clusters, n_discard=3,
                                                chain_length=9, sweep_size=7, seed=99, counter=counter0,
-                                               log_psi_fn=of.log_psi)
+                                               log_psi_fn=of.log_psi, weights=weights)
     np.testing.assert_array_equal(samples.cpu().numpy(), ref)
     np.testing.assert_array_equal(st.sigma.cpu().numpy(), ref_sigma)
     np.testing.assert_array_equal(st.n_accepted_proc.cpu().numpy(), ref_acc)

@@ -35,21 +35,31 @@ def test_random_states_match_oracle(gpu, N, n_down):
 
 
 @pytest.mark.parametrize("rule,n_chains", [("local", 6), ("exchange", 6), ("local", 8192),
-                                           ("exchange", 8192)])
+                                           ("exchange", 8192), ("exchange-weighted", 6),
+                                           ("exchange-weighted", 8192)])
 def test_chains_identical_to_oracle_fp64(gpu, rule, n_chains):
-    # 8192 chains of a small model run four chains per wavefront, fewer run two
+    # 8192 chains of a small model run four chains per wavefront, fewer run
+    # two; weighted exchange (NN and NNN clusters, ExchangeRule probabilities
+    # per distance) runs one chain per wavefront
     print(f"case chains {rule} n_chains={n_chains}", flush=True)
     N = 10
     model, flat, p = _model(N, 2, np.float64, 3, 0.4)
     g = nk.graph.Chain(N)
+    weights = None
     if rule == "local":
         hi = nk.hilbert.Spin(0.5, N=N)
         sa = nk.sampler.MetropolisLocal(hi, n_chains=n_chains, sweep_size=7)
         clusters = None
-    else:
+    elif rule == "exchange":
         hi = nk.hilbert.Spin(0.5, N=N, total_sz=0)
         sa = nk.sampler.MetropolisExchange(hi, graph=g, n_chains=n_chains, sweep_size=7)
         clusters = sa.rule.clusters
+    else:
+        hi = nk.hilbert.Spin(0.5, N=N, total_sz=0)
+        sa = nk.sampler.MetropolisExchange(hi, graph=g, d_max=2, probabilities=[1.0, 0.37],
+                                           n_chains=n_chains, sweep_size=7)
+        clusters, weights = sa.rule.clusters, sa.rule.weights
+        rule = "exchange"
     st = sa.init_state(model, flat, seed=99)
     sigma0 = st.sigma.cpu().numpy().copy()
     counter0 = st.counter
@@ -59,7 +69,7 @@ def test_chains_identical_to_oracle_fp64(gpu, rule, n_chains):
     ids = np.arange(n_chains) if n_chains <= 64 else np.array([0, 1, 2, 3, 4, 4095, 4096, 8190, 8191])
     ref, ref_sigma, ref_acc = osamp.metropolis(p, sigma0[ids], rule=rule, clusters=clusters, n_discard=3,
                                                chain_length=9, sweep_size=7, seed=99, counter=counter0,
-                                               chain_ids=ids)
+                                               chain_ids=ids, weights=weights)
     np.testing.assert_array_equal(samples.cpu().numpy()[ids], ref)
     np.testing.assert_array_equal(st.sigma.cpu().numpy()[ids], ref_sigma)
     np.testing.assert_array_equal(st.n_accepted_proc.cpu().numpy()[ids], ref_acc)
@@ -74,7 +84,8 @@ def _kl(counts, probs):
 
 
 @pytest.mark.parametrize("rule,dtype", [("local", np.float64), ("local", np.float32),
-                                        ("exchange", np.float64), ("exchange", np.float32)])
+                                        ("exchange", np.float64), ("exchange", np.float32),
+                                        ("exchange-weighted", np.float32)])
 def test_distribution_matches_exact(gpu, rule, dtype):
     # sampled frequencies vs exact |psi|^2 (FullSumState semantics), KL divergence
     print(f"case distribution {rule} {np.dtype(dtype)}", flush=True)
@@ -86,9 +97,15 @@ def test_distribution_matches_exact(gpu, rule, dtype):
         hi = nk.hilbert.Spin(0.5, N=N)
         sa = nk.sampler.MetropolisLocal(hi, n_chains=1024)
         states = O.states_all(N)
-    else:
+    elif rule == "exchange":
         hi = nk.hilbert.Spin(0.5, N=N, total_sz=0)
         sa = nk.sampler.MetropolisExchange(hi, graph=nk.graph.Chain(N), n_chains=1024)
+        states = O.states_all(N, n_down=4)
+    else:
+        # distance-3 swaps tried 5x as often as nearest-neighbour ones
+        hi = nk.hilbert.Spin(0.5, N=N, total_sz=0)
+        sa = nk.sampler.MetropolisExchange(hi, graph=nk.graph.Chain(N), d_max=3, probabilities=[1.0, 2.0, 5.0],
+                                           n_chains=1024)
         states = O.states_all(N, n_down=4)
     st = sa.init_state(model, flat, seed=5)
     samples, st = sa.sample_fused(model, flat, st, 100, 200)
@@ -104,7 +121,7 @@ def test_distribution_matches_exact(gpu, rule, dtype):
     print(f"KL = {kl:.3e} over {len(states)} states, {s.shape[0]} samples", flush=True)
     # expected KL of an exact sampler ~ (K-1)/(2 n) = 6e-4 (local), 2e-4 (exchange)
     assert kl < 4e-3
-    if rule == "exchange":
+    if rule != "local":
         assert np.all((s < 0).sum(axis=1) == N // 2)
 
 

@@ -241,6 +241,37 @@ def test_cholesky_solve(gpu, n):
     np.testing.assert_allclose(x.cpu().numpy(), ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max())
 
 
+@pytest.mark.parametrize("n,lda", [(1536, 1536), (2047, 2048), (2560, 2561), (4192, 4192), (8384, 8384),
+                                   (16382, 16382)])
+def test_cholesky_many_panels(gpu, n, lda):
+    # several 512-column panels, a narrow last panel, even and odd row
+    # strides and padded rows: L L^T against A on sampled rows, the solve's
+    # residual, and L against rocSOLVER's factor
+    print(f"case cholesky panels n={n} lda={lda}", flush=True)
+    g = torch.Generator(device="cuda").manual_seed(n)
+    X = torch.randn(n, n // 2 + 7, dtype=torch.float64, device="cuda", generator=g)
+    A0 = X @ X.T / n + 0.05 * torch.eye(n, dtype=torch.float64, device="cuda")
+    del X
+    buf = torch.full((n, lda), float("nan"), dtype=torch.float64, device="cuda")
+    buf[:, :n] = A0
+    a = buf[:, :n]
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ws = _ops.potrf_(a, info)
+    b = torch.ones(n, dtype=torch.float64, device="cuda")
+    _ops.potrs_(a, ws, b, info)
+    assert int(info.item()) == 0
+    assert (A0 @ b - 1.0).abs().max().item() < 1e-9
+    L = torch.tril(a)
+    Lref = torch.linalg.cholesky(A0)
+    assert (L - Lref).abs().max().item() < 2e-9 * Lref.abs().max().item()
+    rows = torch.arange(0, n, 97, device="cuda")
+    LLt = L[rows] @ L[rows].T
+    assert (LLt - A0[rows][:, rows]).abs().max().item() < 1e-11
+    if lda > n:   # the padding columns are never written
+        assert torch.isnan(buf[:, n:]).all()
+    del a, A0, ws, buf, L, Lref
+
+
 def test_cholesky_large_many_panel_workgroups(gpu):
     # n > 512 * 64: the first panel launches more workgroups than the chip
     # holds at once, so some start after workgroup 0 has factored the diagonal

@@ -264,22 +264,55 @@ def test_cholesky_and_cg_and_pinv():
 
 
 # --- sampler: the proposal rules satisfy detailed balance w.r.t. |psi|^2 -----------------
-@pytest.mark.parametrize("rule", ["local", "exchange"])
+def test_weighted_pick_gives_each_cluster_its_weight():
+    # k uniform over [0, W): cluster i must be chosen by exactly w_i of the W draws
+    from oracle import sampler as osamp
+
+    w = np.array([3, 1, 0, 7, 2, 5], dtype=np.int64)
+    picks = np.array([osamp.pick_weighted(w, k) for k in range(int(w.sum()))])
+    np.testing.assert_array_equal(np.bincount(picks, minlength=w.size), w)
+
+
+def test_exchange_rule_integer_weights_and_distance_probabilities():
+    import netket_amd as nk
+
+    g = nk.graph.Chain(6)
+    r = nk.sampler.ExchangeRule(graph=g, d_max=2, probabilities=[1.0, 0.25])
+    d = np.asarray(g.distances())[r.clusters[:, 0], r.clusters[:, 1]]
+    np.testing.assert_array_equal(r.probabilities, np.where(d == 1, 1.0, 0.25))
+    np.testing.assert_array_equal(r.weights, np.where(d == 1, 1 << 20, 1 << 18).astype(np.uint32))
+    assert nk.sampler.integer_weights([1e-9, 1.0]).tolist() == [1, 1 << 20]
+    with pytest.raises(ValueError):
+        nk.sampler.ExchangeRule(graph=g, probabilities=[0.0])
+    with pytest.raises(TypeError):
+        nk.sampler.ExchangeRule(clusters=[(0, 1), (1, 2)], probabilities=[1.0])
+    with pytest.raises(ValueError):
+        nk.sampler.ExchangeRule(graph=g, d_max=2, probabilities=[1.0])
+
+
+
+@pytest.mark.parametrize("rule", ["local", "exchange", "exchange-weighted"])
 def test_metropolis_transition_matrix_is_stationary_on_psi2(rule):
     from oracle import sampler as osamp
 
     N = 8
     p = _rbm(N, 8, seed=7, scale=0.5)
-    states = O.states_all(N, n_down=4 if rule == "exchange" else None)
+    states = O.states_all(N, n_down=4 if rule != "local" else None)
     idx = {int(k): i for i, k in enumerate(O.state_index(states))}
     lp = 2 * orbm.log_psi(p, states)
-    clusters = np.sort(np.array([(i, (i + 1) % N) for i in range(N)]), axis=1)
+    # nearest and next-nearest neighbours; weighted: distance 2 three times as likely
+    clusters = np.sort(np.array([(i, (i + d) % N) for d in (1, 2) for i in range(N)]), axis=1)
+    weights = np.where(np.arange(2 * N) < N, 1.0, 3.0) if rule == "exchange-weighted" else None
     T = np.zeros((len(states), len(states)))
     for i, x in enumerate(states):
-        moves, corrs = osamp.local_moves(x) if rule == "local" else osamp.exchange_moves(x, clusters)
-        for xp, c in zip(moves, corrs):
+        if rule == "local":
+            moves, corrs, probs = osamp.local_moves(x)
+        else:
+            moves, corrs, probs = osamp.exchange_moves(x, clusters, weights)
+        assert sum(probs) == pytest.approx(1.0)
+        for xp, c, q in zip(moves, corrs, probs):
             j = idx[int(O.state_index(xp[None])[0])]
-            T[i, j] += min(1.0, np.exp(lp[j] - lp[i] + c)) / len(moves)
+            T[i, j] += min(1.0, np.exp(lp[j] - lp[i] + c)) * q
         T[i, i] += 1.0 - T[i].sum()
     pi = orbm.exact_probability(p, states)
     np.testing.assert_allclose(pi @ T, pi, rtol=1e-10, atol=1e-14)
