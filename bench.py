@@ -68,7 +68,8 @@ def build_j1j2_ffnn(seed=0, n_chains=8192):
     Sz=0, no sign rule), FP32 FFNN 100->400->400 (log_cosh, sum pooled,
     N(0, 0.01^2) init), MetropolisExchange over the 200 NN + 200 NNN bonds,
     16384 samples from 8192 chains (2 per chain, 5 discarded sweeps of 100),
-    minSR with the structured NTK (16384^2, FP64), diag_shift 1e-4, Cholesky."""
+    minSR with the structured NTK (16384^2 on bf16x3 tiles, FP64 output),
+    diag_shift 1e-4, Cholesky."""
     import numpy as np
 
     import netket_amd as nk
@@ -89,7 +90,8 @@ def build_tfim2d_rbm(seed=0):
     """2-D TFIM on the 20x20 periodic square lattice, J=1, h=3.04, FP32 RBM
     alpha=8 (3200 hidden units, 1,283,600 parameters), MetropolisLocal 8192
     chains, 32768 samples (4 per chain, 5 discarded sweeps of 400), minSR with
-    the structured NTK (32768^2, FP64), diag_shift 1e-4, Cholesky."""
+    the structured NTK (32768^2 on bf16x3 tiles, FP64 output), diag_shift 1e-4,
+    Cholesky."""
     import numpy as np
 
     import netket_amd as nk
@@ -153,7 +155,7 @@ def run_workload(name, steps, warmup, norms=False):
         "ms_per_step": dt / steps * 1e3,
         "samples_per_s": steps * ns / dt,
         "higher_is_better": True,
-        "dtype": "f32 model, FP64 NTK / Cholesky",
+        "dtype": "f32 model, bf16x3 NTK (FP32 accumulate, FP64 epilogue), FP64 Cholesky",
         "data": "synthetic (random-init model, sampler-generated configurations)",
         "config": {"workload": desc, "n_samples": ns, "n_params": vs.n_parameters},
         "stages_ms_per_step": {k: round(v["total_ms"] / steps, 3) for k, v in sorted(
