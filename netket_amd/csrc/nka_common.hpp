@@ -168,6 +168,23 @@ __device__ inline int weighted_pick(const uint32_t (&w)[R], int RS, uint32_t k) 
   return sel;
 }
 
+// The two sites of cluster sel (wave-uniform) from the lanes' register slots
+// (cluster lane + 64 r in slot r): a lane read instead of a dependent global
+// load on the proposal's critical path.
+template <int R>
+__device__ inline void cluster_sites(const int (&cu)[R], const int (&cv)[R], int sel, int& s0, int& s1) {
+  const int rs = sel >> 6, ln = sel & 63;
+  s0 = 0;
+  s1 = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (r == rs) {
+      s0 = __builtin_amdgcn_readlane(cu[r], ln);
+      s1 = __builtin_amdgcn_readlane(cv[r], ln);
+    }
+  }
+}
+
 // Order this lane's LDS writes before later LDS reads of other lanes in the
 // same wavefront.
 __device__ inline void wave_lds_sync() {

@@ -66,6 +66,8 @@ __host__ __device__ inline FfnnView<T> ffnn_view(const nka_ffnn& m) {
 
 __host__ __device__ inline int lda_of(int H1) { return H1 + ((H1 & 1) ? 2 : 1); }  // odd, >= H1 + 1
 __host__ __device__ inline int64_t ffpad(int64_t b) { return (b + 15) & ~(int64_t)15; }
+// LDS scratch of rows_gemm's K-split blocks for ncols output columns (FP32)
+__host__ __device__ inline int64_t split_scratch_bytes(int ncols) { return (int64_t)(((ncols + 31) / 32) & 3) * 4 * 1024 * 4; }
 constexpr int64_t FF_LDS_MAX = 160 * 1024;
 
 // ---------------------------------------------------------------------------
@@ -74,32 +76,38 @@ constexpr int64_t FF_LDS_MAX = 160 * 1024;
 // is applied to every element and its values are summed per row into
 // rowsum[r] (0 <= r < 32) in a fixed order.  part is LDS scratch of
 // 32 * FF_MAXBLK values.  All threads of the workgroup must call it.
+// The 32-column blocks go to the four wavefronts in turn; with rbuf (LDS
+// scratch of (nblk % 4) * 4 * 1024 values) the nblk % 4 blocks left over
+// after whole rounds are split along K over the four wavefronts instead, and
+// their partial tiles are summed in a fixed order, so every wavefront does the
+// same number of products.
 // ---------------------------------------------------------------------------
 template <class Epi>
 __device__ void rows_gemm(const float* As, int lda, int K, int ncols, const float* __restrict__ Bp, int sk,
-                          int sc, Epi epi, float* part, float* rowsum) {
+                          int sc, Epi epi, float* part, float* rowsum, float* rbuf = nullptr) {
   const int wid = threadIdx.x >> 6, lane = lane_id();
   const int nblk = (ncols + 31) >> 5;
   const int kh = lane >> 5, lc = lane & 31;
-  for (int cb = wid; cb < nblk; cb += 4) {
-    floatx16 acc;
-#pragma unroll
-    for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+  constexpr int KB = 16;
+  // acc += A[:, ka:kb] B(ka:kb, block cb); lane (lc, kh) supplies row lc and
+  // column cb*32 + lc at k = k0 + 2q + kh of each pair of k-rows.  The B
+  // loads run three groups of 16 k-rows ahead of their products (three
+  // register sets in turn; near the end of the range a group reloads the
+  // last full group, unused, so that every load is unconditional).
+  auto mfma_range = [&](floatx16& acc, int cb, int ka, int kb) {
     const int col = cb * 32 + lc;
     const bool cok = col < ncols;
-    // B loads of a lane past ncols use the last column (in bounds; its
-    // output column is discarded), so they need no branch; the loads of the
-    // next 16 k-rows are issued before the current 8 products
     const int colc = cok ? col : ncols - 1;
-    // B(k, c) = Bp[k sk + c sc], 32-bit element offsets
     const uint32_t boff = (uint32_t)colc * (uint32_t)sc + (uint32_t)kh * (uint32_t)sk;
     const uint32_t s2 = 2u * (uint32_t)sk;
-    constexpr int KB = 16;
-    const int kfull = K / KB * KB;
-    int k = 0;
-    // two register sets in turn: the loads of the next 16 k-rows are issued
-    // before the current 8 products (the last group reloads its own rows,
-    // unused, so that every load is unconditional)
+    const int kend = ka + (kb - ka) / KB * KB;
+    int k = ka;
+    auto loadB = [&](int kk0, float* bv) {
+      const int kc = kk0 < kend ? kk0 : kend - KB;
+      const uint32_t kn = (uint32_t)kc * (uint32_t)sk + boff;
+#pragma unroll
+      for (int q = 0; q < KB / 2; ++q) bv[q] = Bp[kn + (uint32_t)q * s2];
+    };
     auto group = [&](int kk0, const float* bv) {
       float a[KB / 2];
 #pragma unroll
@@ -107,38 +115,39 @@ __device__ void rows_gemm(const float* As, int lda, int K, int ncols, const floa
 #pragma unroll
       for (int q = 0; q < KB / 2; ++q) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q], bv[q], acc, 0, 0, 0);
     };
-    if (kfull > 0) {
-      float b0[KB / 2], b1[KB / 2];
-#pragma unroll
-      for (int q = 0; q < KB / 2; ++q) b0[q] = Bp[boff + (uint32_t)q * s2];
+    if (kend > ka) {
+      float b0[KB / 2], b1[KB / 2], b2[KB / 2];
+      loadB(k, b0);
+      loadB(k + KB, b1);
       while (true) {
-        {
-          const uint32_t kn = (uint32_t)(k + KB < kfull ? k + KB : k) * (uint32_t)sk + boff;
-#pragma unroll
-          for (int q = 0; q < KB / 2; ++q) b1[q] = Bp[kn + (uint32_t)q * s2];
-        }
+        loadB(k + 2 * KB, b2);
         __builtin_amdgcn_sched_barrier(0);
         group(k, b0);
         k += KB;
-        if (k >= kfull) break;
-        {
-          const uint32_t kn = (uint32_t)(k + KB < kfull ? k + KB : k) * (uint32_t)sk + boff;
-#pragma unroll
-          for (int q = 0; q < KB / 2; ++q) b0[q] = Bp[kn + (uint32_t)q * s2];
-        }
+        if (k >= kend) break;
+        loadB(k + 2 * KB, b0);
         __builtin_amdgcn_sched_barrier(0);
         group(k, b1);
         k += KB;
-        if (k >= kfull) break;
+        if (k >= kend) break;
+        loadB(k + 2 * KB, b1);
+        __builtin_amdgcn_sched_barrier(0);
+        group(k, b2);
+        k += KB;
+        if (k >= kend) break;
       }
     }
-    for (; k < K; k += 2) {
+    for (; k < kb; k += 2) {
       const int kk = k + kh;
-      const float a = kk < K ? As[lc * lda + kk] : 0.0f;
-      const float b = (kk < K && cok) ? Bp[(uint32_t)kk * (uint32_t)sk + (uint32_t)col * (uint32_t)sc] : 0.0f;
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+      const float av = kk < kb ? As[lc * lda + kk] : 0.0f;
+      const float bv = (kk < kb && cok) ? Bp[(uint32_t)kk * (uint32_t)sk + (uint32_t)col * (uint32_t)sc] : 0.0f;
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
     }
-    // element e of lane l is (row (e/4)*8 + (l/32)*4 + e%4, col l%32)
+  };
+  // element e of lane l is (row (e/4)*8 + (l/32)*4 + e%4, col l%32)
+  auto finish = [&](const floatx16& acc, int cb) {
+    const int col = cb * 32 + lc;
+    const bool cok = col < ncols;
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
       const int row = (e >> 2) * 8 + kh * 4 + (e & 3);
@@ -146,6 +155,41 @@ __device__ void rows_gemm(const float* As, int lda, int K, int ncols, const floa
 #pragma unroll
       for (int off = 16; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
       if (lc == 0) part[row * FF_MAXBLK + cb] = v;
+    }
+  };
+  const int rem = rbuf ? (nblk & 3) : 0;
+  const int nfull = nblk - rem;
+  for (int cb = wid; cb < nfull; cb += 4) {
+    floatx16 acc;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+    mfma_range(acc, cb, 0, K);
+    finish(acc, cb);
+  }
+  if (rem > 0) {
+    // K quarters rounded up to whole groups; wavefront w takes quarter w
+    const int kq = ((K + 3) / 4 + KB - 1) / KB * KB;
+    const int k0 = min(K, wid * kq), k1 = min(K, k0 + kq);
+    for (int j = 0; j < rem; ++j) {
+      floatx16 acc;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+      mfma_range(acc, nfull + j, k0, k1);
+      float* dst = rbuf + (j * 4 + wid) * 1024;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) dst[e * 64 + lane] = acc[e];
+    }
+    __syncthreads();
+    if (wid < rem) {
+      floatx16 acc;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        float t = 0.0f;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) t += rbuf[(wid * 4 + w) * 1024 + e * 64 + lane];
+        acc[e] = t;
+      }
+      finish(acc, nfull + wid);
     }
   }
   __syncthreads();
@@ -159,7 +203,7 @@ __device__ void rows_gemm(const float* As, int lda, int K, int ncols, const floa
 
 template <class Epi>
 __device__ void rows_gemm(const double* As, int lda, int K, int ncols, const double* __restrict__ Bp, int sk,
-                          int sc, Epi epi, double* part, double* rowsum) {
+                          int sc, Epi epi, double* part, double* rowsum, double* rbuf = nullptr) {
   const int r = threadIdx.x & 31, g = threadIdx.x >> 5;
   double s = 0.0;
   for (int c = g; c < ncols; c += FF_GROUPS) {
@@ -211,6 +255,7 @@ __global__ void __launch_bounds__(FF_THREADS) ffnn_forward_kernel(nka_ffnn m, co
   int* dsite = (int*)(rowsum + FF_ROWS);               // 32 x (FF_MAXD + 1): count, sites
   int8_t* xl = (int8_t*)(dsite + FF_ROWS * (FF_MAXD + 1));  // 32 x N (+ 32 x N base rows)
   int8_t* xb = xl + FF_ROWS * N;
+  T* rbuf = sizeof(T) == 4 ? (T*)(smem + ffpad((int64_t)(xb - (int8_t*)smem) + (int64_t)FF_ROWS * N)) : nullptr;
   const int64_t r0 = (int64_t)blockIdx.x * FF_ROWS;
   // MODE 1: tile rows r0 .. r0+31 of the row list; the grid covers the
   // longest possible list, so tiles past its end leave at once
@@ -284,7 +329,7 @@ __global__ void __launch_bounds__(FF_THREADS) ffnn_forward_kernel(nka_ffnn m, co
     if (g2_out && r0 + rr < rows) g2_out[(r0 + rr) * H2 + c] = tanh_t<T>(zz);
     return log_cosh<T>(zz);
   };
-  rows_gemm(hA, lda, H1, H2, W2, H2, 1, epi, part, rowsum);
+  rows_gemm(hA, lda, H1, H2, W2, H2, 1, epi, part, rowsum, rbuf);
   if (threadIdx.x < FF_ROWS) {
     const int64_t go = row_of(threadIdx.x);
     if (go >= 0) logpsi[go] = rowsum[threadIdx.x];
@@ -295,7 +340,8 @@ template <class T>
 inline size_t forward_lds(const nka_ffnn& m) {
   return ffpad((size_t)FF_ROWS * lda_of(m.n_hidden1) * sizeof(T)) +
          ffpad((size_t)FF_ROWS * FF_MAXBLK * sizeof(T)) + ffpad(FF_ROWS * sizeof(T)) +
-         ffpad((size_t)FF_ROWS * (FF_MAXD + 1) * sizeof(int)) + ffpad((size_t)2 * FF_ROWS * m.n_visible);
+         ffpad((size_t)FF_ROWS * (FF_MAXD + 1) * sizeof(int)) + ffpad((size_t)2 * FF_ROWS * m.n_visible) +
+         (sizeof(T) == 4 ? split_scratch_bytes(m.n_hidden2) : 0);
 }
 
 // List of the connected rows with a nonzero matrix element (only those enter
@@ -704,6 +750,23 @@ __global__ void __launch_bounds__(FF_THREADS) ffnn_metropolis_kernel(nka_ffnn m,
   int* pacc = pnf + FF_ROWS;                      // accepted flag
   int* nacc = pacc + FF_ROWS;
   int8_t* xs = (int8_t*)(nacc + FF_ROWS);         // 32 x N spins
+  // exchange rule with equal weights: each chain's differing clusters as a
+  // bit set (word w, bit b: cluster 64 w + b) and their count, the clusters,
+  // and for every site the clusters that contain it (other site, cluster)
+  const bool exch = sa.rule == NKA_RULE_EXCHANGE;
+  const bool fast_exch = exch && !sa.weights;
+  const int ncl = fast_exch ? sa.n_clusters : 0;
+  const int NWc = (ncl + 63) / 64;
+  unsigned char* xb = smem + ffpad((int64_t)((unsigned char*)xs - smem) + (int64_t)FF_ROWS * N);
+  uint64_t* dmask = (uint64_t*)xb;                                   // FF_ROWS x NWc
+  int* cls = (int*)(xb + ffpad((int64_t)FF_ROWS * NWc * 8));         // 2 ncl
+  int* adj_off = cls + 2 * ncl;                                      // N + 1
+  int* adj_pos = adj_off + (N + 1);                                  // N
+  int* adj_t = adj_pos + N;                                          // 2 ncl
+  int* adj_c = adj_t + 2 * ncl;                                      // 2 ncl
+  int* ndiff = adj_c + 2 * ncl;                                      // FF_ROWS
+  int* pnew = ndiff + FF_ROWS;                                       // FF_ROWS
+  T* rbuf = sizeof(T) == 4 ? (T*)(smem + ffpad((int64_t)((unsigned char*)(pnew + FF_ROWS) - smem))) : nullptr;
   const int tid = threadIdx.x, wid = tid >> 6, lane = lane_id();
   const int c0 = blockIdx.x * FF_ROWS;
   const int cr = tid / FF_GROUPS, g = tid % FF_GROUPS;   // chain row and unit group of this thread
@@ -714,6 +777,44 @@ __global__ void __launch_bounds__(FF_THREADS) ffnn_metropolis_kernel(nka_ffnn m,
     xs[idx] = (c0 + r < sa.C) ? sigma[(int64_t)(c0 + r) * N + i] : (int8_t)1;
   }
   if (tid < FF_ROWS) nacc[tid] = 0;
+  if (fast_exch) {
+    for (int i = tid; i <= N; i += FF_THREADS) adj_off[i] = 0;
+    for (int i = tid; i < 2 * ncl; i += FF_THREADS) cls[i] = sa.clusters[i];
+    __syncthreads();
+    for (int c = tid; c < ncl; c += FF_THREADS) {
+      atomicAdd(&adj_off[cls[2 * c] + 1], 1);
+      atomicAdd(&adj_off[cls[2 * c + 1] + 1], 1);
+    }
+    __syncthreads();
+    if (tid == 0)
+      for (int i = 0; i < N; ++i) adj_off[i + 1] += adj_off[i];
+    __syncthreads();
+    for (int i = tid; i < N; i += FF_THREADS) adj_pos[i] = adj_off[i];
+    for (int idx = tid; idx < FF_ROWS * NWc; idx += FF_THREADS) {
+      const int r = idx / NWc, w = idx % NWc;
+      uint64_t bits = 0;
+      for (int b = 0; b < 64; ++b) {
+        const int c = 64 * w + b;
+        if (c < ncl && xs[r * N + cls[2 * c]] != xs[r * N + cls[2 * c + 1]]) bits |= 1ull << b;
+      }
+      dmask[idx] = bits;
+    }
+    __syncthreads();
+    for (int c = tid; c < ncl; c += FF_THREADS) {
+      const int u = cls[2 * c], w = cls[2 * c + 1];
+      int at = atomicAdd(&adj_pos[u], 1);
+      adj_t[at] = w;
+      adj_c[at] = c;
+      at = atomicAdd(&adj_pos[w], 1);
+      adj_t[at] = u;
+      adj_c[at] = c;
+    }
+    if (tid < FF_ROWS) {
+      int cnt = 0;
+      for (int w = 0; w < NWc; ++w) cnt += __popcll(dmask[tid * NWc + w]);
+      ndiff[tid] = cnt;
+    }
+  }
   __syncthreads();
   T th[HPT];
 #pragma unroll
@@ -726,12 +827,11 @@ __global__ void __launch_bounds__(FF_THREADS) ffnn_metropolis_kernel(nka_ffnn m,
   const T* W2 = v.W2;
   const T* b2 = v.b2;
   auto epi = [=](int, int c, T z) { return log_cosh<T>(z + (b2 ? b2[c] : (T)0)); };
-  rows_gemm(hA, lda, H1, H2, W2, H2, 1, epi, part, rowsum);
+  rows_gemm(hA, lda, H1, H2, W2, H2, 1, epi, part, rowsum, rbuf);
   const T pw = (T)sa.machine_pow;
   if (tid < FF_ROWS) lp[tid] = pw * rowsum[tid];
 
-  // cluster slots of this lane (exchange rule): cl = lane + 64 r
-  const bool exch = sa.rule == NKA_RULE_EXCHANGE;
+  // cluster slots of this lane (weighted exchange rule): cl = lane + 64 r
   const int RS = exch ? (sa.n_clusters + 63) / 64 : 0;
   int cu[FF_MAXCL], cv[FF_MAXCL];
 #pragma unroll
@@ -751,7 +851,66 @@ __global__ void __launch_bounds__(FF_THREADS) ffnn_metropolis_kernel(nka_ffnn m,
   const int n_sweeps = sa.n_discard + sa.chain_length;
   for (int sw = 0; sw < n_sweeps; ++sw) {
     for (int p = 0; p < sa.sweep_size; ++p, ++q) {
-      // proposals of chains 8w .. 8w+7, one at a time per wavefront
+      if (!exch || fast_exch) {
+        // proposals of chains 8w .. 8w+7 by lanes 0..7 of wavefront w.  An
+        // exchange picks the k-th set bit of the chain's differing-cluster
+        // set; swapping the two (different) spins of that cluster flips the
+        // state of every other cluster at either site, so the count after the
+        // move follows from the sites' clusters alone
+        if (lane < FF_ROWS / 4) {
+          const int r = wid * (FF_ROWS / 4) + lane;
+          const int ch = c0 + r;
+          const u32x4 rnd = philox4x32_10({(uint32_t)q, (uint32_t)(q >> 32), (uint32_t)ch, TAG_METROPOLIS},
+                                          sa.k0, sa.k1);
+          const int8_t* x = xs + r * N;
+          int s0 = 0, s1 = 0, nf = 1, nn = 0;
+          T corr = (T)0;
+          if (!exch) {
+            s0 = (int)bounded(rnd.x, (uint32_t)N);
+          } else {
+            const int nd = ndiff[r];
+            nf = 0;
+            if (nd > 0) {
+              int k = (int)bounded(rnd.x, (uint32_t)nd);
+              const uint64_t* dm = dmask + r * NWc;
+              int w = 0;
+              uint64_t word = dm[0];
+              int cnt = __popcll(word);
+              while (k >= cnt && w + 1 < NWc) {
+                k -= cnt;
+                ++w;
+                word = dm[w];
+                cnt = __popcll(word);
+              }
+              if (k < cnt) {
+                const int c = 64 * w + kth_set_bit(word, k);
+                s0 = cls[2 * c];
+                s1 = cls[2 * c + 1];
+                nf = 2;
+                const int8_t x0 = x[s0], x1 = x[s1];
+                int delta = 0;
+                for (int e = adj_off[s0]; e < adj_off[s0 + 1]; ++e) {
+                  const int t = adj_t[e];
+                  if (t != s0 && t != s1) delta += x[t] != x0 ? -1 : 1;
+                }
+                for (int e = adj_off[s1]; e < adj_off[s1 + 1]; ++e) {
+                  const int t = adj_t[e];
+                  if (t != s0 && t != s1) delta += x[t] != x1 ? -1 : 1;
+                }
+                nn = nd + delta;
+                corr = log_t<T>((T)nd) - log_t<T>((T)nn);
+              }
+            }
+          }
+          psite[2 * r] = s0;
+          psite[2 * r + 1] = s1;
+          pnf[r] = nf;
+          pcorr[r] = corr;
+          pu[r] = uniform01<T>(rnd);
+          pnew[r] = nn;
+        }
+      } else
+      // weighted exchange: proposals of chains 8w .. 8w+7, one at a time per wavefront
       for (int cc = 0; cc < FF_ROWS / 4; ++cc) {
         const int r = wid * (FF_ROWS / 4) + cc;
         const int ch = c0 + r;
@@ -779,8 +938,7 @@ __global__ void __launch_bounds__(FF_THREADS) ffnn_metropolis_kernel(nka_ffnn m,
             const uint32_t k = bounded(uniform_value<uint32_t>(rnd.x), tot);
             int sel = uniform_value<int>(weighted_pick<FF_MAXCL>(wd, RS, k));
             if (sel < 0 || sel >= sa.n_clusters) sel = 0;  // unreachable: k < tot
-            s0 = sa.clusters[2 * sel];
-            s1 = sa.clusters[2 * sel + 1];
+            cluster_sites<FF_MAXCL>(cu, cv, sel, s0, s1);
             nf = 2;
             uint32_t tot_new = 0;
 #pragma unroll
@@ -824,8 +982,7 @@ __global__ void __launch_bounds__(FF_THREADS) ffnn_metropolis_kernel(nka_ffnn m,
             }
             sel = uniform_value<int>(sel);
             if (sel < 0 || sel >= sa.n_clusters) sel = 0;  // unreachable: k < n_diff
-            s0 = sa.clusters[2 * sel];
-            s1 = sa.clusters[2 * sel + 1];
+            cluster_sites<FF_MAXCL>(cu, cv, sel, s0, s1);
             nf = 2;
             int n_new = 0;
 #pragma unroll
@@ -880,7 +1037,7 @@ __global__ void __launch_bounds__(FF_THREADS) ffnn_metropolis_kernel(nka_ffnn m,
         }
       }
       __syncthreads();
-      rows_gemm(hA, lda, H1, H2, W2, H2, 1, epi, part, rowsum);
+      rows_gemm(hA, lda, H1, H2, W2, H2, 1, epi, part, rowsum, rbuf);
       if (tid < FF_ROWS) {
         const int r = tid;
         int acc = 0;
@@ -894,6 +1051,7 @@ __global__ void __launch_bounds__(FF_THREADS) ffnn_metropolis_kernel(nka_ffnn m,
               xs[r * N + site] = (int8_t)(-xs[r * N + site]);
             }
             nacc[r] += 1;
+            if (fast_exch) ndiff[r] = pnew[r];
           }
         }
         pacc[r] = acc;
@@ -902,7 +1060,22 @@ __global__ void __launch_bounds__(FF_THREADS) ffnn_metropolis_kernel(nka_ffnn m,
       if (pacc[cr]) {
 #pragma unroll
         for (int j = 0; j < HPT; ++j) th[j] = thn[j];
+        if (fast_exch) {
+          // every other cluster at the two exchanged sites changes state; the
+          // chain's FF_GROUPS threads share its clusters
+          const int s0 = psite[2 * cr], s1 = psite[2 * cr + 1];
+          unsigned long long* dm = (unsigned long long*)(dmask + cr * NWc);
+          for (int e = adj_off[s0] + g; e < adj_off[s0 + 1]; e += FF_GROUPS) {
+            const int t = adj_t[e], c = adj_c[e];
+            if (t != s0 && t != s1) atomicXor(&dm[c >> 6], 1ull << (c & 63));
+          }
+          for (int e = adj_off[s1] + g; e < adj_off[s1 + 1]; e += FF_GROUPS) {
+            const int t = adj_t[e], c = adj_c[e];
+            if (t != s0 && t != s1) atomicXor(&dm[c >> 6], 1ull << (c & 63));
+          }
+        }
       }
+      if (fast_exch) __syncthreads();  // the next proposals read the updated sets
     }
     if (sw >= sa.n_discard) {
       const int t = sw - sa.n_discard;
@@ -926,11 +1099,15 @@ __global__ void __launch_bounds__(FF_THREADS) ffnn_metropolis_kernel(nka_ffnn m,
   (void)chain;
 }
 
+// ncl: clusters of an exchange rule with equal weights (0 otherwise)
 template <class T>
-inline size_t metropolis_lds(const nka_ffnn& m) {
+inline size_t metropolis_lds(const nka_ffnn& m, int ncl) {
+  const int64_t nw = (ncl + 63) / 64, N = m.n_visible;
   return ffpad((size_t)FF_ROWS * lda_of(m.n_hidden1) * sizeof(T)) +
          ffpad((size_t)FF_ROWS * FF_MAXBLK * sizeof(T)) + ffpad((size_t)4 * FF_ROWS * sizeof(T)) +
-         ffpad((size_t)5 * FF_ROWS * sizeof(int)) + ffpad((size_t)FF_ROWS * m.n_visible);
+         ffpad((size_t)5 * FF_ROWS * sizeof(int)) + ffpad((size_t)FF_ROWS * m.n_visible) +
+         ffpad(FF_ROWS * nw * 8) + ffpad(((size_t)6 * ncl + 2 * N + 1 + 2 * FF_ROWS) * sizeof(int)) +
+         (sizeof(T) == 4 ? split_scratch_bytes(m.n_hidden2) : 0);
 }
 
 // Dynamic LDS above the default 64 KB needs the function attribute.
@@ -944,9 +1121,11 @@ template <class T, int HPT>
 static void launch_ffnn_sweep(const nka_ffnn& m, const FfSweepArgs& sa, int8_t* sigma, void* lp, int8_t* smp,
                               void* lps, int64_t* nacc, hipStream_t s) {
   const unsigned g = (unsigned)((sa.C + FF_ROWS - 1) / FF_ROWS);
-  allow_lds(ffnn_metropolis_kernel<T, HPT>, (int64_t)metropolis_lds<T>(m));
-  hipLaunchKernelGGL((ffnn_metropolis_kernel<T, HPT>), dim3(g), dim3(FF_THREADS), metropolis_lds<T>(m), s, m,
-                     sa, sigma, (T*)lp, smp, (T*)lps, nacc);
+  const int ncl = (sa.rule == NKA_RULE_EXCHANGE && !sa.weights) ? sa.n_clusters : 0;
+  const size_t lds = metropolis_lds<T>(m, ncl);
+  allow_lds(ffnn_metropolis_kernel<T, HPT>, (int64_t)lds);
+  hipLaunchKernelGGL((ffnn_metropolis_kernel<T, HPT>), dim3(g), dim3(FF_THREADS), lds, s, m, sa, sigma, (T*)lp,
+                     smp, (T*)lps, nacc);
 }
 
 template <class T>
@@ -969,9 +1148,9 @@ static int check_model(const nka_ffnn* m, const char* who) {
   NKA_CHECK_ARG(m->dtype == NKA_F32 || m->dtype == NKA_F64, "%s: bad dtype", who);
   const int64_t lds = m->dtype == NKA_F32
                           ? std::max<int64_t>(std::max<int64_t>(forward_lds<float>(*m), backprop_lds<float>(*m)),
-                                              metropolis_lds<float>(*m))
+                                              metropolis_lds<float>(*m, 0))
                           : std::max<int64_t>(std::max<int64_t>(forward_lds<double>(*m), backprop_lds<double>(*m)),
-                                              metropolis_lds<double>(*m));
+                                              metropolis_lds<double>(*m, 0));
   NKA_CHECK_ARG(lds <= FF_LDS_MAX, "%s: layer sizes need %lld bytes of LDS per workgroup (max %lld)", who,
                 (long long)lds, (long long)FF_LDS_MAX);
   return NKA_OK;
@@ -1195,6 +1374,11 @@ int nka_ffnn_metropolis(const nka_ffnn* m, int rule, const int32_t* clusters, in
   sa.k1 = (uint32_t)(seed >> 32);
   sa.counter = counter;
   sa.counter_dev = counter_dev;
+  const int ncl = (rule == NKA_RULE_EXCHANGE && !cluster_weights) ? n_clusters : 0;
+  const int64_t lds = m->dtype == NKA_F32 ? (int64_t)metropolis_lds<float>(*m, ncl)
+                                          : (int64_t)metropolis_lds<double>(*m, ncl);
+  NKA_CHECK_ARG(lds <= FF_LDS_MAX, "nka_ffnn_metropolis: %d clusters need %lld bytes of LDS (max %lld)",
+                n_clusters, (long long)lds, (long long)FF_LDS_MAX);
   hipStream_t s = (hipStream_t)stream;
   if (m->dtype == NKA_F32)
     dispatch_ffnn_sweep<float>(*m, sa, sigma, log_prob, samples, log_prob_samples, n_accepted, s);

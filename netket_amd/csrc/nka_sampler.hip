@@ -156,7 +156,8 @@ __device__ __forceinline__ void rbm_metropolis_kernel_body(const nka_rbm& m, con
           const uint32_t k = bounded(uniform_value<uint32_t>(rnd.x), tot);
           int sel = uniform_value<int>(weighted_pick<kMaxClusterSlots>(wd, RS, k));
           if (sel < 0 || sel >= sa.n_clusters) sel = 0;  // unreachable: k < tot
-          const int su = sa.clusters[2 * sel], sv = sa.clusters[2 * sel + 1];
+          int su, sv;
+          cluster_sites<kMaxClusterSlots>(cu, cv, sel, su, sv);
           sites[0] = su;
           sites[1] = sv;
           nf = 2;
@@ -211,7 +212,8 @@ __device__ __forceinline__ void rbm_metropolis_kernel_body(const nka_rbm& m, con
           }
           sel = uniform_value<int>(sel);
           if (sel < 0 || sel >= sa.n_clusters) sel = 0;  // unreachable: k < n_diff
-          const int su = sa.clusters[2 * sel], sv = sa.clusters[2 * sel + 1];
+          int su, sv;
+          cluster_sites<kMaxClusterSlots>(cu, cv, sel, su, sv);
           sites[0] = su;
           sites[1] = sv;
           nf = 2;
