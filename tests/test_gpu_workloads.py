@@ -231,6 +231,31 @@ def test_j1j2_10x10_ffnn_fp32_log_prob(gpu):
     np.testing.assert_allclose(st.log_prob.cpu().numpy(), ref, rtol=1e-5, atol=2e-4)
 
 
+def test_j1j2_10x10_ffnn_fp32_chains_follow_fp64(gpu):
+    # the FP32 sampler on the same (FP32-representable) parameters and random
+    # stream as the FP64 one: FP32 rounding of log psi changes an acceptance
+    # only when u falls within ~1e-6 of the ratio, so almost every chain takes
+    # the FP64 chain's moves and the acceptance counts agree
+    print("case j1j2 10x10 ffnn fp32 vs fp64 chains", flush=True)
+    g, hi, H, terms = _j1j2_10()
+    out = {}
+    for dtype in (np.float64, np.float32):
+        sa = nk.sampler.MetropolisExchange(hi, graph=g, n_chains=8192)
+        model, flat, p = _ffnn(100, 400, 400, np.float32, 57, 0.05)
+        model = nk.models.FFNN(hidden_dims=(400, 400), param_dtype=dtype)
+        flat = flat.to(torch.float64 if dtype == np.float64 else torch.float32)
+        st = sa.init_state(model, flat, seed=8)
+        samples, st = sa.sample_fused(model, flat, st, 2, 2)
+        out[dtype] = (samples.cpu().numpy(), st.n_accepted_proc.cpu().numpy())
+    s64, a64 = out[np.float64]
+    s32, a32 = out[np.float32]
+    assert np.all((s32 < 0).sum(axis=-1) == 50)
+    same = np.all(s32 == s64, axis=(1, 2))
+    print(f"chains identical: {same.mean():.4f}; acceptance {a64.sum()} vs {a32.sum()}", flush=True)
+    assert same.mean() > 0.98
+    assert abs(int(a32.sum()) - int(a64.sum())) <= 5e-3 * a64.sum()
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_j1j2_10x10_minsr_update_ns512(gpu, dtype):
     # VMC_SR(use_ntk=True), structured NTK 512 x 512, diag_shift 1e-4, Cholesky,
