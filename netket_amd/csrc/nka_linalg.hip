@@ -125,9 +125,11 @@ __global__ void __launch_bounds__(256, 1) gram_bf16x3_kernel(const uint16_t* __r
     // four planes.  The next slab is fetched into registers while the current
     // one is used.
     uint4 pf[NIT][4];
-    auto fetch = [&](int64_t k0) {
+    // chunks it0 <= it < it1 of the slab at k0
+    auto fetch_part = [&](int64_t k0, int it0, int it1) {
   #pragma unroll
       for (int it = 0; it < NIT; ++it) {
+        if (it < it0 || it >= it1) continue;
         const int chunk = tid + it * 256;
         const int r = chunk / GB_CPR, kc = (chunk % GB_CPR) * 8;
         const int64_t kk = k0 + kc;
@@ -146,7 +148,7 @@ __global__ void __launch_bounds__(256, 1) gram_bf16x3_kernel(const uint16_t* __r
         }
       }
     };
-    fetch(0);
+    fetch_part(0, 0, NIT);
     for (int64_t k0 = 0; k0 < K_p; k0 += GB_K) {
       __syncthreads();
   #pragma unroll
@@ -161,11 +163,19 @@ __global__ void __launch_bounds__(256, 1) gram_bf16x3_kernel(const uint16_t* __r
         }
       }
       __syncthreads();
-      if (k0 + GB_K < K_p) fetch(k0 + GB_K);
       const uint16_t* Bh = diag_tile ? sA[0] : sB[0];
       const uint16_t* Bl = diag_tile ? sA[1] : sB[1];
+      const bool more = k0 + GB_K < K_p;
   #pragma unroll
       for (int ks = 0; ks < GB_K; ks += 16) {
+        // the next slab's loads are spread over the first K sub-steps, so the
+        // memory pipeline works through them while the products run
+        if (more) {
+          constexpr int per = (NIT + GB_K / 16 - 2) / (GB_K / 16 - 1);
+          const int q = ks / 16;
+          if (q < GB_K / 16 - 1) fetch_part(k0 + GB_K, q * per, (q + 1) * per < NIT ? (q + 1) * per : NIT);
+        }
+        __builtin_amdgcn_sched_barrier(0);
         const int kq = ks + (lane >> 5) * 8;
         bf16x8 ah[MI], al[MI], bh[MI], bl[MI];
   #pragma unroll
