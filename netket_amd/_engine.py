@@ -165,6 +165,30 @@ def gram(bufs: JacobianBuffers, shift: float, offset: float = 0.0,
     return out
 
 
+def qgt_structured_available(model, flat) -> bool:
+    """Whether the dense S of ``model`` can be formed from its layer structure
+    (FP32 RBM: spin-pair x activation-pair products)."""
+    return model.kind == "rbm" and flat.dtype == torch.float32
+
+
+def qgt_structured(model, flat, sigma2d: torch.Tensor, tT: torch.Tensor, mean: torch.Tensor, scale: float,
+                   shift: float, out: torch.Tensor | None = None) -> torch.Tensor:
+    """scale (O - mean)^T (O - mean) + shift I on the lower triangle (FP64,
+    n_params x n_params) from the RBM's Khatri-Rao structure, without a
+    Jacobian; ``mean`` must be the sample means of O and ``tT`` the
+    (n_hidden, n_samples) activations."""
+    n, N = sigma2d.shape
+    np_ = model.layout(N)["n_params"]
+    if out is None or out.shape != (np_, np_):
+        out = torch.empty((np_, np_), dtype=torch.float64, device=sigma2d.device)
+    m = model.native(flat, N)
+    ws = _ops.WORKSPACE.get("rbm_qgt", nat.lib().nka_rbm_qgt_workspace(C.byref(m), n))
+    with _ops.TIMER.region("qgt_structured"):
+        nat.call("nka_rbm_qgt", C.byref(m), _ops.ptr(sigma2d), _ops.ptr(tT), n, _ops.ptr(mean), float(scale),
+                 float(shift), _ops.ptr(out), out.stride(0), _ops.ptr(ws), _ops.stream())
+    return out
+
+
 def ntk_structured(model, flat, sigma2d: torch.Tensor, t, mean: torch.Tensor, scale: float,
                    shift: float, offset: float = 0.0, out: torch.Tensor | None = None) -> torch.Tensor:
     """scale (O - mean)(O - mean)^T + shift I + offset 11^T on the lower triangle

@@ -302,6 +302,120 @@ constexpr int kGramWT = 96;
 constexpr int kGramRbmWT = 64;
 
 // ---------------------------------------------------------------------------
+// C = A (Bh + Bl)^T in FP32 (ma x nb), K contiguous in both operands (row
+// stride ldk, zero beyond the true K).  A is one bf16 plane whose entries
+// bf16 holds exactly (products of +-1 spins, 0 and 1); B is split into two
+// bf16 planes.  Each fragment pair therefore takes two products instead of
+// the Gram's three, and A's LDS slab is a single plane.  Same 2 WT x 2 WT
+// workgroup tile, K step, slab prefetch and fragment layout as the Gram
+// kernel, on a rectangular tile grid: tiles are numbered down the columns
+// (the ma side fastest), so the workgroups that share a B column panel
+// (the larger operand) sit on one XCD and read it through one L2.
+// ---------------------------------------------------------------------------
+constexpr int kExactAWT = 96;
+
+template <int WT>
+__global__ void __launch_bounds__(256, 1) gemm_exact_a_kernel(const uint16_t* __restrict__ a, int64_t ma,
+                                                           const uint16_t* __restrict__ bh,
+                                                           const uint16_t* __restrict__ bl, int64_t nb,
+                                                           int64_t K, int64_t ldk, float* __restrict__ c,
+                                                           int64_t ldc, int64_t tiles_m, int64_t n_tiles_total) {
+  constexpr int TT = 2 * WT;
+  constexpr int MI = WT / 32;
+  constexpr int NIT = TT * GB_CPR / 256;
+  __shared__ __attribute__((aligned(16))) uint16_t sA[TT * GB_LD];
+  __shared__ __attribute__((aligned(16))) uint16_t sB[2][TT * GB_LD];
+  const int64_t bid = xcd_remap(blockIdx.x, gridDim.x);
+  if (bid >= n_tiles_total) return;
+  const int64_t i0 = (bid % tiles_m) * TT, j0 = (bid / tiles_m) * TT;
+  const int tid = threadIdx.x, wid = tid / 64, lane = lane_id();
+  const int wm = (wid >> 1) * WT, wn = (wid & 1) * WT;
+
+  floatx16 acc[MI][MI];
+#pragma unroll
+  for (int x = 0; x < MI; ++x)
+#pragma unroll
+    for (int y = 0; y < MI; ++y)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[x][y][e] = 0.0f;
+
+  // chunks it0 <= it < it1 of the slab at k0: A, B hi, B lo
+  uint4 pf[NIT][3];
+  auto fetch_part = [&](int64_t k0, int it0, int it1) {
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      if (it < it0 || it >= it1) continue;
+      const int chunk = tid + it * 256;
+      const int r = chunk / GB_CPR, kc = (chunk % GB_CPR) * 8;
+      const int64_t kk = k0 + kc;
+      const uint4 z = {0, 0, 0, 0};
+      pf[it][0] = z;
+      pf[it][1] = z;
+      pf[it][2] = z;
+      if (i0 + r < ma && kk < K) pf[it][0] = *(const uint4*)(a + (i0 + r) * ldk + kk);
+      if (j0 + r < nb && kk < K) {
+        pf[it][1] = *(const uint4*)(bh + (j0 + r) * ldk + kk);
+        pf[it][2] = *(const uint4*)(bl + (j0 + r) * ldk + kk);
+      }
+    }
+  };
+  fetch_part(0, 0, NIT);
+  for (int64_t k0 = 0; k0 < K; k0 += GB_K) {
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int chunk = tid + it * 256;
+      const int r = chunk / GB_CPR, kc = (chunk % GB_CPR) * 8;
+      *(uint4*)(&sA[r * GB_LD + kc]) = pf[it][0];
+      *(uint4*)(&sB[0][r * GB_LD + kc]) = pf[it][1];
+      *(uint4*)(&sB[1][r * GB_LD + kc]) = pf[it][2];
+    }
+    __syncthreads();
+    const bool more = k0 + GB_K < K;
+#pragma unroll
+    for (int ks = 0; ks < GB_K; ks += 16) {
+      // the next slab's loads spread over the first K sub-steps
+      if (more) {
+        constexpr int per = (NIT + GB_K / 16 - 2) / (GB_K / 16 - 1);
+        const int q = ks / 16;
+        if (q < GB_K / 16 - 1) fetch_part(k0 + GB_K, q * per, (q + 1) * per < NIT ? (q + 1) * per : NIT);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      const int kq = ks + (lane >> 5) * 8;
+      bf16x8 af[MI], bhf[MI], blf[MI];
+#pragma unroll
+      for (int x = 0; x < MI; ++x) af[x] = *(const bf16x8*)(&sA[(wm + x * 32 + (lane & 31)) * GB_LD + kq]);
+#pragma unroll
+      for (int y = 0; y < MI; ++y) {
+        const int row = wn + y * 32 + (lane & 31);
+        bhf[y] = *(const bf16x8*)(&sB[0][row * GB_LD + kq]);
+        blf[y] = *(const bf16x8*)(&sB[1][row * GB_LD + kq]);
+      }
+      // small products first
+#pragma unroll
+      for (int x = 0; x < MI; ++x)
+#pragma unroll
+        for (int y = 0; y < MI; ++y) acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[x], blf[y], acc[x][y], 0, 0, 0);
+#pragma unroll
+      for (int x = 0; x < MI; ++x)
+#pragma unroll
+        for (int y = 0; y < MI; ++y) acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[x], bhf[y], acc[x][y], 0, 0, 0);
+    }
+  }
+  // element e of lane l in a 32x32 tile is (row (e/4)*8 + (l/32)*4 + e%4, col l%32)
+#pragma unroll
+  for (int x = 0; x < MI; ++x)
+#pragma unroll
+    for (int y = 0; y < MI; ++y)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int64_t gi = i0 + wm + x * 32 + (e / 4) * 8 + (lane >> 5) * 4 + (e & 3);
+        const int64_t gj = j0 + wn + y * 32 + (lane & 31);
+        if (gi < ma && gj < nb) c[gi * ldc + gj] = acc[x][y][e];
+      }
+}
+
+// ---------------------------------------------------------------------------
 // FP64 NT GEMM: C = beta C + alpha A B^T (+ shift on the diagonal),
 // 64x64 tile per workgroup, 4 waves of 32x32, K step 16.  lower_only skips
 // tiles above the diagonal and elements with j > i.  A and C may alias when
@@ -1548,6 +1662,17 @@ bool launch_gram_rbm(const uint16_t* xh, const uint16_t* xl, int64_t n, int64_t 
 // the network's NTK entry point): H, G and D from the split h1, g2 and d1
 // (hi + lo planes, n rows each), X from the packed spins.  Returns false when
 // the spins do not fit the epilogue.
+// C = A (Bh + Bl)^T (FP32) for an exact single-plane A; K and ldk multiples
+// of 8, operands 16-byte aligned (checked by the caller).
+void launch_gemm_exact_a(const uint16_t* a, int64_t ma, const uint16_t* bh, const uint16_t* bl, int64_t nb,
+                         int64_t K, int64_t ldk, float* c, int64_t ldc, hipStream_t s) {
+  constexpr int TT = 2 * kExactAWT;
+  const int64_t tm = (ma + TT - 1) / TT, tn = (nb + TT - 1) / TT;
+  if (tm == 0 || tn == 0) return;
+  hipLaunchKernelGGL((gemm_exact_a_kernel<kExactAWT>), dim3(grid_mult8(tm * tn)), dim3(256), 0, s, a, ma, bh, bl,
+                     nb, K, ldk, c, ldc, tm, tm * tn);
+}
+
 bool launch_gram_ffnn(const uint16_t* h1h, const uint16_t* h1l, int64_t K0, int64_t ld0, const uint16_t* g2h,
                       const uint16_t* g2l, int64_t K1, int64_t ld1, const uint16_t* d1h, const uint16_t* d1l,
                       int64_t K2, int64_t ld2, int64_t n, const uint32_t* bits, int nw, int N, const double* v,

@@ -142,6 +142,215 @@ bool launch_gram_rbm(const uint16_t* xh, const uint16_t* xl, int64_t n, int64_t 
                      double ha, double scale, double shift, double offset, double* out, int64_t ldo,
                      hipStream_t s);
 
+void launch_gemm_exact_a(const uint16_t* a, int64_t ma, const uint16_t* bh, const uint16_t* bl, int64_t nb,
+                         int64_t K, int64_t ldk, float* c, int64_t ldc, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Dense S of the RBM from its Khatri-Rao structure (netket/_src/ngd/sr.py:43-55:
+// matrix = O_L^T O_L + diag_shift I, O_L = (O - mean) / sqrt(Ns)).
+//
+// Every parameter's log-derivative is a product O_p = s~_u(p) t~_w(p) of an
+// augmented spin s~ = (1, sigma_0..sigma_{N-1}) and an augmented hidden
+// activation t~ = (1, tanh theta_0..tanh theta_{M-1}): hidden bias b_j =
+// (1, t_j), kernel W_ij = (sigma_i, t_j), visible bias a_i = (sigma_i, 1).  So
+//   sum_s O_sp O_sq = sum_s (s~_u s~_u')_s (t~_w t~_w')_s = C[pair(u,u'), pair(w,w')]
+// with C = P Q^T over the samples, P the distinct spin-pair products (rows:
+// 1, sigma_i, sigma_i sigma_k for i < k: 1 + N + N(N-1)/2 rows, since
+// sigma^2 = 1) and Q the distinct activation-pair products (1, t_j, t_j t_l
+// for j <= l: 1 + M + M(M+1)/2 rows).  C has about Np^2 / 4 entries against
+// the Gram's Np^2 / 2, the spin side is exact in one bf16 plane, and the
+// activation side takes two (hi + lo), so the product needs a third of the
+// bf16x3 Gram's matrix-core work.  Centring and the shift are applied in
+// FP64 when C is unpacked into S:
+//   S_pq = scale C_pq - scale ns mean_p mean_q (+ shift if p == q),
+// which is scale sum_s (O_sp - mean_p)(O_sq - mean_q) when mean holds the
+// sample means of O.
+// ---------------------------------------------------------------------------
+
+// Augmented spin index u in [-1, N) (-1: the constant 1).  Row of P holding
+// s~_u s~_u'.
+__device__ __host__ inline int64_t qgt_spin_row(int u, int v, int N) {
+  if (u == v) return 0;
+  if (u < 0) return 1 + v;
+  if (v < 0) return 1 + u;
+  const int64_t i = u < v ? u : v, k = u < v ? v : u;
+  return 1 + N + k * (k - 1) / 2 + i;
+}
+// Augmented hidden index w in [-1, M).  Row of Q holding t~_w t~_w'.
+__device__ __host__ inline int64_t qgt_hidden_row(int w, int x, int M) {
+  if (w < 0 && x < 0) return 0;
+  if (w < 0) return 1 + x;
+  if (x < 0) return 1 + w;
+  const int64_t j = w < x ? w : x, l = w < x ? x : w;
+  return 1 + M + l * (l + 1) / 2 + j;
+}
+// Largest e with e (e + 1) / 2 <= b.
+__device__ inline int64_t tri_root(int64_t b) {
+  int64_t e = (int64_t)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
+  while ((e + 1) * (e + 2) / 2 <= b) ++e;
+  while (e * (e + 1) / 2 > b) --e;
+  return e;
+}
+// Position of parameter (u, w) in the flat vector, or -1 when it is not one.
+__device__ inline int64_t qgt_param(int u, int w, const nka_rbm& m) {
+  if (u < 0) return (w < 0 || m.bias_off < 0) ? -1 : m.bias_off + w;
+  if (w < 0) return m.visible_off < 0 ? -1 : m.visible_off + u;
+  return m.kernel_off + (int64_t)u * m.n_hidden + w;
+}
+
+// sT (N x ldk int8) = sigma^T as +-1, zero for the padding samples s >= ns.
+__global__ void qgt_spins_t_kernel(const int8_t* __restrict__ sigma, int64_t ns, int N, int64_t ldk,
+                                   int8_t* __restrict__ sT) {
+  const int64_t total = (int64_t)N * ldk;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / ldk, s = e % ldk;
+    sT[e] = s < ns ? (sigma[s * N + i] > 0 ? (int8_t)1 : (int8_t)-1) : (int8_t)0;
+  }
+}
+
+__device__ inline uint32_t qgt_sign_bf16(int v) { return v > 0 ? 0x3F80u : (v < 0 ? 0xBF80u : 0u); }
+
+// P (np_rows x ldk bf16): row 0 = 1, row 1 + i = sigma_i, row 1 + N + k(k-1)/2 + i
+// = sigma_i sigma_k (i < k); zero for the padding samples.  One workgroup per
+// row, eight samples per thread and step.
+__global__ void __launch_bounds__(256) qgt_spin_pairs_kernel(const int8_t* __restrict__ sT, int64_t ns, int N,
+                                                             int64_t ldk, int64_t rows, uint16_t* __restrict__ P) {
+  for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
+    int64_t i = -1, k = -1;
+    if (r >= 1 && r <= N) {
+      i = r - 1;
+    } else if (r > N) {
+      const int64_t b = r - 1 - N;  // k(k-1)/2 + i, i < k
+      k = tri_root(b) + 1;
+      i = b - k * (k - 1) / 2;
+    }
+    for (int64_t s0 = (int64_t)threadIdx.x * 8; s0 < ldk; s0 += 256 * 8) {
+      int v[8];
+      if (i < 0) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = s0 + e < ns ? 1 : 0;
+      } else {
+        const uint2 a = *(const uint2*)(sT + i * ldk + s0);
+        const int8_t* ab = (const int8_t*)&a;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = ab[e];
+        if (k >= 0) {
+          const uint2 b = *(const uint2*)(sT + k * ldk + s0);
+          const int8_t* bb = (const int8_t*)&b;
+#pragma unroll
+          for (int e = 0; e < 8; ++e) v[e] *= bb[e];
+        }
+      }
+      uint4 o;
+      o.x = qgt_sign_bf16(v[0]) | (qgt_sign_bf16(v[1]) << 16);
+      o.y = qgt_sign_bf16(v[2]) | (qgt_sign_bf16(v[3]) << 16);
+      o.z = qgt_sign_bf16(v[4]) | (qgt_sign_bf16(v[5]) << 16);
+      o.w = qgt_sign_bf16(v[6]) | (qgt_sign_bf16(v[7]) << 16);
+      *(uint4*)(P + r * ldk + s0) = o;
+    }
+  }
+}
+
+// Q hi + lo (rows x ldk bf16 each): row 0 = 1, row 1 + j = t_j, row
+// 1 + M + l(l+1)/2 + j = t_j t_l (j <= l, FP32 product), split as
+// hi = bf16(q), lo = bf16(q - hi); zero for the padding samples.  tT is
+// (M x ns) FP32.
+__global__ void __launch_bounds__(256) qgt_hidden_pairs_kernel(const float* __restrict__ tT, int64_t ns, int M,
+                                                               int64_t ldk, int64_t rows, uint16_t* __restrict__ qh,
+                                                               uint16_t* __restrict__ ql) {
+  const bool vec = (ns % 4) == 0 && (((uintptr_t)tT) % 16) == 0;
+  for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
+    int64_t j = -1, l = -1;
+    if (r >= 1 && r <= M) {
+      j = r - 1;
+    } else if (r > M) {
+      const int64_t b = r - 1 - M;  // l(l+1)/2 + j, j <= l
+      l = tri_root(b);
+      j = b - l * (l + 1) / 2;
+    }
+    for (int64_t s0 = (int64_t)threadIdx.x * 8; s0 < ldk; s0 += 256 * 8) {
+      float f[8];
+      if (j < 0) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) f[e] = s0 + e < ns ? 1.0f : 0.0f;
+      } else {
+        const float* a = tT + j * ns + s0;
+        if (vec && s0 + 8 <= ns) {
+          const float4 x = *(const float4*)a, y = *(const float4*)(a + 4);
+          f[0] = x.x; f[1] = x.y; f[2] = x.z; f[3] = x.w;
+          f[4] = y.x; f[5] = y.y; f[6] = y.z; f[7] = y.w;
+        } else {
+#pragma unroll
+          for (int e = 0; e < 8; ++e) f[e] = s0 + e < ns ? a[e] : 0.0f;
+        }
+        if (l >= 0) {
+          const float* b = tT + l * ns + s0;
+          if (vec && s0 + 8 <= ns) {
+            const float4 x = *(const float4*)b, y = *(const float4*)(b + 4);
+            f[0] *= x.x; f[1] *= x.y; f[2] *= x.z; f[3] *= x.w;
+            f[4] *= y.x; f[5] *= y.y; f[6] *= y.z; f[7] *= y.w;
+          } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) f[e] *= s0 + e < ns ? b[e] : 0.0f;
+          }
+        }
+      }
+      uint32_t h[8], lo[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const uint16_t hb = bf16_rne(f[e]);
+        h[e] = hb;
+        lo[e] = bf16_rne(f[e] - bf16_to_f(hb));
+      }
+      const uint4 oh = {h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16)};
+      const uint4 ol = {lo[0] | (lo[1] << 16), lo[2] | (lo[3] << 16), lo[4] | (lo[5] << 16), lo[6] | (lo[7] << 16)};
+      *(uint4*)(qh + r * ldk + s0) = oh;
+      *(uint4*)(ql + r * ldk + s0) = ol;
+    }
+  }
+}
+
+// S from C, one workgroup per augmented spin pair (u_hi >= u_lo in [-1, N)):
+// all entries S[p][q] with p = (u_hi, w), q = (u_lo, w') for w, w' in
+// [-1, M), each written once into the lower triangle (at [max][min]).  The
+// C row of the pair is staged in LDS when it fits (LDS_ROW).  Threads run
+// along w', whose positions are consecutive in S, so the writes of a row
+// segment are contiguous.
+template <bool LDS_ROW>
+__global__ void __launch_bounds__(256) qgt_unpack_kernel(nka_rbm m, const float* __restrict__ c, int64_t ldc,
+                                                         int64_t nb, const double* __restrict__ mean,
+                                                         double scale, double cs, double shift,
+                                                         double* __restrict__ out, int64_t ldo) {
+  extern __shared__ float crow[];
+  const int N = m.n_visible, M = m.n_hidden;
+  const int64_t n_pairs = (int64_t)(N + 1) * (N + 2) / 2;
+  const int W = M + 1;
+  for (int64_t b = blockIdx.x; b < n_pairs; b += gridDim.x) {
+    const int64_t ah = tri_root(b);
+    const int uh = (int)ah - 1, ul = (int)(b - ah * (ah + 1) / 2) - 1;
+    const float* src = c + qgt_spin_row(uh, ul, N) * ldc;
+    if (LDS_ROW) {
+      __syncthreads();
+      for (int64_t e = threadIdx.x; e < nb; e += blockDim.x) crow[e] = src[e];
+      __syncthreads();
+    }
+    const float* row = LDS_ROW ? crow : src;
+    for (int64_t e = threadIdx.x; e < (int64_t)W * W; e += blockDim.x) {
+      const int w = (int)(e / W) - 1, x = (int)(e % W) - 1;
+      const int64_t p = qgt_param(uh, w, m), q = qgt_param(ul, x, m);
+      if (p < 0 || q < 0) continue;
+      if (uh == ul && p < q) continue;  // the symmetric diagonal pair: each entry once
+      double v = scale * (double)row[qgt_hidden_row(w, x, M)] - cs * mean[p] * mean[q];
+      if (p == q) v += shift;
+      if (p >= q)
+        out[p * ldo + q] = v;
+      else
+        out[q * ldo + p] = v;
+    }
+  }
+}
+
 constexpr int NT_T = 64;
 constexpr int NT_K = 16;
 constexpr int NT_LD = 17;
@@ -358,6 +567,61 @@ int nka_rbm_ntk(const nka_rbm* m, const int8_t* sigma, const void* t, int64_t ns
   hipLaunchKernelGGL(rbm_ntk_kernel, dim3(grid), dim3(256), 0, s, t64, M, sigma, N, ns, v, c, hb, ha,
                      scale, shift, offset, out, ldo, tiles);
   NKA_CHECK_LAUNCH("nka_rbm_ntk");
+  return NKA_OK;
+}
+
+// Workspace of nka_rbm_qgt: transposed spins, P, Q hi and lo, and C.
+static int64_t qgt_ldk(int64_t ns) { return (ns + 63) / 64 * 64; }
+static int64_t qgt_rows_p(const nka_rbm& m) { return 1 + (int64_t)m.n_visible + (int64_t)m.n_visible * (m.n_visible - 1) / 2; }
+static int64_t qgt_rows_q(const nka_rbm& m) { return 1 + (int64_t)m.n_hidden + (int64_t)m.n_hidden * (m.n_hidden + 1) / 2; }
+static int64_t qgt_ldc(const nka_rbm& m) { return (qgt_rows_q(m) + 63) / 64 * 64; }
+static int64_t qgt_round(int64_t b) { return (b + 255) / 256 * 256; }
+
+int64_t nka_rbm_qgt_workspace(const nka_rbm* m, int64_t ns) {
+  if (!m || ns <= 0) return 0;
+  const int64_t ldk = qgt_ldk(ns);
+  return qgt_round((int64_t)m->n_visible * ldk) + qgt_round(qgt_rows_p(*m) * ldk * 2) +
+         2 * qgt_round(qgt_rows_q(*m) * ldk * 2) + qgt_round(qgt_rows_p(*m) * qgt_ldc(*m) * 4);
+}
+
+int nka_rbm_qgt(const nka_rbm* m, const int8_t* sigma, const void* tT, int64_t ns, const double* mean,
+                double scale, double shift, double* out, int64_t ldo, void* workspace, void* stream) {
+  NKA_CHECK_ARG(m && m->params && sigma && tT && mean && out && workspace && ns > 0,
+                "nka_rbm_qgt: bad arguments");
+  NKA_CHECK_ARG(m->dtype == NKA_F32, "nka_rbm_qgt: FP32 models only (FP64 models take the FP64 Jacobian)");
+  const int N = m->n_visible, M = m->n_hidden;
+  const int64_t np = (m->bias_off >= 0 ? M : 0) + (int64_t)N * M + (m->visible_off >= 0 ? N : 0);
+  NKA_CHECK_ARG(ldo >= np, "nka_rbm_qgt: ldo < n_params");
+  NKA_CHECK_ARG((int64_t)(N + 1) * (N + 2) / 2 < (1LL << 31), "nka_rbm_qgt: too many visible units");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t ldk = qgt_ldk(ns), rp = qgt_rows_p(*m), rq = qgt_rows_q(*m), ldc = qgt_ldc(*m);
+  unsigned char* w = (unsigned char*)workspace;
+  int8_t* sT = (int8_t*)w;
+  w += qgt_round((int64_t)N * ldk);
+  uint16_t* P = (uint16_t*)w;
+  w += qgt_round(rp * ldk * 2);
+  uint16_t* qh = (uint16_t*)w;
+  w += qgt_round(rq * ldk * 2);
+  uint16_t* ql = (uint16_t*)w;
+  w += qgt_round(rq * ldk * 2);
+  float* C = (float*)w;
+  const unsigned g0 = (unsigned)std::min<int64_t>(((int64_t)N * ldk + 255) / 256, 65536);
+  hipLaunchKernelGGL(qgt_spins_t_kernel, dim3(g0), dim3(256), 0, s, sigma, ns, N, ldk, sT);
+  hipLaunchKernelGGL(qgt_spin_pairs_kernel, dim3((unsigned)std::min<int64_t>(rp, 65536)), dim3(256), 0, s, sT,
+                     ns, N, ldk, rp, P);
+  hipLaunchKernelGGL(qgt_hidden_pairs_kernel, dim3((unsigned)std::min<int64_t>(rq, 65536)), dim3(256), 0, s,
+                     (const float*)tT, ns, M, ldk, rq, qh, ql);
+  launch_gemm_exact_a(P, rp, qh, ql, rq, ldk, ldk, C, ldc, s);
+  const int64_t pairs = (int64_t)(N + 1) * (N + 2) / 2;
+  const unsigned gu = (unsigned)std::min<int64_t>(pairs, 65536);
+  const size_t row_bytes = (size_t)rq * sizeof(float);
+  if (row_bytes <= 64 * 1024)
+    hipLaunchKernelGGL(qgt_unpack_kernel<true>, dim3(gu), dim3(256), row_bytes, s, *m, C, ldc, rq, mean, scale,
+                       scale * (double)ns, shift, out, ldo);
+  else
+    hipLaunchKernelGGL(qgt_unpack_kernel<false>, dim3(gu), dim3(256), 0, s, *m, C, ldc, rq, mean, scale,
+                       scale * (double)ns, shift, out, ldo);
+  NKA_CHECK_LAUNCH("nka_rbm_qgt");
   return NKA_OK;
 }
 

@@ -214,10 +214,12 @@ class VMC_SR(AbstractVariationalDriver):
     def __init__(self, hamiltonian, optimizer, *, diag_shift, variational_state, proj_reg=None,
                  momentum=None, linear_solver=solvers.cholesky_with_fallback, linear_solver_fn=None,
                  chunk_size_bwd=None, mode=None, use_ntk=None, on_the_fly=None,
-                 ntk_method="structured"):
+                 ntk_method="structured", qgt_method="structured"):
         super().__init__(variational_state, optimizer)
         if ntk_method not in ("structured", "jacobian"):
             raise ValueError("ntk_method must be 'structured' or 'jacobian'")
+        if qgt_method not in ("structured", "jacobian"):
+            raise ValueError("qgt_method must be 'structured' or 'jacobian'")
         if linear_solver_fn is not None:
             linear_solver = linear_solver_fn
         if mode not in (None, "real"):
@@ -244,6 +246,9 @@ class VMC_SR(AbstractVariationalDriver):
         # NTK T from the RBM's structure (FP64, inner sizes n_hidden and n_visible)
         # or from the explicit Jacobian (bf16x3 / FP64 Gram over n_params)
         self.ntk_method = ntk_method
+        # S of an FP32 RBM from its spin-pair x activation-pair structure, or
+        # from the explicit Jacobian (bf16x3 / FP64 Gram over n_samples)
+        self.qgt_method = qgt_method
         self._old_updates = None
         self._jac = None
         self.info = None
@@ -279,7 +284,9 @@ class VMC_SR(AbstractVariationalDriver):
         # contraction weights dv / sqrt(Ns) = 2 (E - <E>) / Ns
         w_dv = eng.centre_scale(e, st.mean_device, 2.0 / ns)
         structured = self.use_ntk and self.ntk_method == "structured"
-        if not structured:
+        structured_s = (not self.use_ntk and self.qgt_method == "structured"
+                        and eng.qgt_structured_available(model, flat))
+        if not structured and not structured_s:
             kind = nat.OUT_BF16X2 if flat.dtype == torch.float32 else nat.OUT_F64
             layout = nat.LAYOUT_SP if self.use_ntk else nat.LAYOUT_PS
             self._jac = eng.jacobian(model, flat, sig, t, tT, mean, 1.0 / math.sqrt(ns), layout,
@@ -297,7 +304,11 @@ class VMC_SR(AbstractVariationalDriver):
             if momentum is not None:
                 # F + eps mu old_updates (netket/_src/ngd/sr.py:45-47)
                 eng.axpy_(F, diag_shift * float(momentum), self._old_updates)
-            system = solvers.DenseSystem(builder=lambda: eng.gram(X, diag_shift))
+            if structured_s:
+                system = solvers.DenseSystem(builder=lambda: eng.qgt_structured(
+                    model, flat, sig, tT, mean, 1.0 / ns, diag_shift))
+            else:
+                system = solvers.DenseSystem(builder=lambda: eng.gram(X, diag_shift))
             rhs = F
 
             def finish(x):

@@ -85,8 +85,10 @@ def make_stream(n_jobs: int = 256, rate: float = 4.0, seed: int = 0, n_steps: in
 
 def footprint_bytes(spec: JobSpec) -> int:
     """Device memory of one job: parameters, chains, samples, tanh tables,
-    the centred Jacobian (bf16 hi/lo), the FP64 S or T matrix and workspaces,
-    doubled for the graph pool kept next to the eager warm-up buffers."""
+    the operands of S (spin pairs and activation pairs hi/lo with their
+    product) or of T (centred Jacobian bf16 hi/lo), the FP64 S or T matrix and
+    workspaces, doubled for the graph pool kept next to the eager warm-up
+    buffers."""
     N = spec.n_sites
     M = spec.alpha * N
     Np = M + N * M + N
@@ -98,7 +100,11 @@ def footprint_bytes(spec: JobSpec) -> int:
     b = 4 * Np * 4                        # parameters, gradient, update, means (f32/f64)
     b += Ns * N * 2                       # samples + chain state
     b += Ns * M * 4 * 2                   # t and its transpose
-    b += rows * ld * 4                    # X hi + lo
+    if ntk:
+        b += rows * ld * 4                # X hi + lo
+    else:
+        na, nb = 1 + N + N * (N - 1) // 2, 1 + M + M * (M + 1) // 2
+        b += N * ld + na * ld * 2 + nb * ld * 4 + na * ((nb + 63) // 64 * 64) * 4  # spins^T, P, Q, C
     b += n * n * 8                        # S or T
     b += (Ns // 256 + 1) * (Np + 1) * 8   # contraction partials
     b += n * 16 + (n // 64 + 1) * 64 * 64 * 8

@@ -146,6 +146,91 @@ def test_rbm_ntk_structured(gpu, dtype, ns, biases, N):
     assert np.all(err <= bound)
 
 
+@pytest.mark.parametrize("ns,biases,N,alpha", [(300, (True, True), 14, 3),
+                                               (129, (False, True), 14, 3),
+                                               (64, (True, False), 16, 2),
+                                               (1, (True, True), 8, 1),
+                                               (1000, (False, False), 10, 2),
+                                               (257, (True, True), 70, 3),
+                                               (4099, (True, True), 32, 2)])
+def test_rbm_qgt_structured(gpu, ns, biases, N, alpha):
+    # S = (O - mean)^T (O - mean) / ns + shift I from the RBM's spin-pair x
+    # activation-pair products equals the explicit product of the oracle's
+    # FP64 Jacobian (netket/_src/ngd/sr.py:50-55).  M = 210 at N = 70 puts a
+    # C row (22366 floats) past the LDS staging of the unpack.
+    print(f"case rbm_qgt ns={ns} biases={biases} N={N} alpha={alpha}", flush=True)
+    M = alpha * N
+    model = nk.models.RBM(alpha=alpha, param_dtype=np.float32, use_hidden_bias=biases[0],
+                          use_visible_bias=biases[1])
+    rng = np.random.default_rng(ns + N)
+    n_par = model.layout(N)["n_params"]
+    flat = rng.normal(scale=0.3, size=n_par).astype(np.float32)
+    x = rng.choice(np.array([-1, 1], dtype=np.int8), size=(ns, N))
+    sig = torch.as_tensor(x, device="cuda")
+    fl = torch.as_tensor(flat, device="cuda")
+    t, tT = eng.prepare(model, fl, sig, row_major=True, transposed=True)
+    ones = torch.full((1, ns), 1.0 / ns, dtype=torch.float64, device="cuda")
+    mean = eng.contract(model, fl, sig, t, ones)[0]
+    shift = 1e-2
+    out = torch.full((n_par, n_par), 7.0, dtype=torch.float64, device="cuda")
+    S = eng.qgt_structured(model, fl, sig, tT, mean, 1.0 / ns, shift, out=out).cpu().numpy()
+    p = orbm.RBMParams(flat.astype(np.float64), N, M, hidden_bias=biases[0], visible_bias=biases[1])
+    Ou = orbm.jacobian(p, x)
+    Oc = Ou - Ou.mean(axis=0)
+    ref = Oc.T @ Oc / ns + shift * np.eye(n_par)
+    lo = np.tril_indices(n_par)
+    # the centring is applied to the uncentred products: rounding scales with
+    # the uncentred column norms.  Each activation pair keeps ~16 mantissa bits
+    # (bf16 hi + lo), the spin pairs are exact, accumulation is FP32.
+    du = np.sqrt(np.sum(Ou * Ou, axis=0) / ns)
+    bound = (3e-5 * np.outer(du, du))[lo] + 1e-12
+    err = np.abs(S[lo] - ref[lo])
+    print(f"max |dS| / bound = {(err / bound).max():.2e}", flush=True)
+    assert np.all(err <= bound)
+    # the strict upper triangle is not written
+    up = np.triu_indices(n_par, 1)
+    assert np.all(S[up] == 7.0)
+
+
+def test_rbm_qgt_structured_full_size(gpu):
+    # the benchmark's shape (N = 64, alpha = 2, 16384 samples, S 8384^2) against
+    # the FP64 product of the same Jacobian formed on the GPU
+    print("case rbm_qgt full size", flush=True)
+    N, alpha, ns = 64, 2, 16384
+    model = nk.models.RBM(alpha=alpha, param_dtype=np.float32)
+    rng = np.random.default_rng(3)
+    n_par = model.layout(N)["n_params"]
+    flat = rng.normal(scale=0.05, size=n_par).astype(np.float32)
+    x = np.ones((ns, N), dtype=np.int8)
+    for s in range(ns):
+        x[s, rng.permutation(N)[: N // 2]] = -1
+    sig = torch.as_tensor(x, device="cuda")
+    fl = torch.as_tensor(flat, device="cuda")
+    t, tT = eng.prepare(model, fl, sig, row_major=True, transposed=True)
+    ones = torch.full((1, ns), 1.0 / ns, dtype=torch.float64, device="cuda")
+    mean = eng.contract(model, fl, sig, t, ones)[0]
+    S = eng.qgt_structured(model, fl, sig, tT, mean, 1.0 / ns, 0.01)
+    lay = model.layout(N)
+    M = lay["M"]
+    t64 = t.double()
+    s64 = sig.double()
+    O = torch.empty((ns, n_par), dtype=torch.float64, device="cuda")
+    O[:, lay["bias"][0]: lay["bias"][0] + M] = t64
+    O[:, lay["kernel"][0]: lay["kernel"][0] + N * M] = (s64[:, :, None] * t64[:, None, :]).reshape(ns, N * M)
+    O[:, lay["visible_bias"][0]: lay["visible_bias"][0] + N] = s64
+    du = torch.sqrt((O * O).sum(0) / ns)
+    O -= O.mean(0)
+    ref = O.T @ O / ns
+    ref.diagonal().add_(0.01)
+    del O
+    L = torch.tril(torch.ones(n_par, n_par, dtype=torch.bool, device="cuda"))
+    err = (S - ref).abs()
+    bound = 3e-5 * du[:, None] * du[None, :] + 1e-12
+    ratio = torch.where(L, err / bound, torch.zeros_like(err)).max().item()
+    print(f"max |dS| / bound = {ratio:.2e}", flush=True)
+    assert ratio <= 1.0
+
+
 @pytest.mark.parametrize("solver_name", ["cholesky", "cg"])
 def test_qgt_diag_scale(gpu, solver_name):
     # scale-invariant regularisation (netket/optimizer/qgt/qgt_jacobian_common.py:21-60,
