@@ -480,7 +480,19 @@ __global__ void __launch_bounds__(256) gemm_nt_f64_kernel(const double* a, int64
     }
   }
   __syncthreads();
-  // element e of lane l in a 16x16 tile is (row l/16 + 4e, col l%16)
+  // element e of lane l in a 16x16 tile is (row l/16 + 4e, col l%16); C is
+  // read before any of it is written, so the loads' latencies overlap
+  double cv[2][2][4];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t gi = i0 + wm + x * 16 + (lane >> 4) + 4 * e;
+        const int64_t gj = j0 + wn + y * 16 + (lane & 15);
+        cv[x][y][e] = (beta != 0.0 && gi < m && gj < n && (!lower_only || gj <= gi)) ? c[gi * ldc + gj] : 0.0;
+      }
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -491,7 +503,7 @@ __global__ void __launch_bounds__(256) gemm_nt_f64_kernel(const double* a, int64
         const int64_t gj = j0 + wn + y * 16 + (lane & 15);
         if (gi < m && gj < n && (!lower_only || gj <= gi)) {
           double v = alpha * acc[x][y][e];
-          if (beta != 0.0) v += beta * c[gi * ldc + gj];
+          if (beta != 0.0) v += beta * cv[x][y][e];
           if (gi == gj) v += shift;
           c[gi * ldc + gj] = v;
         }
@@ -592,20 +604,62 @@ __device__ __forceinline__ void gemm_big_tile(const double* __restrict__ a, int6
           wout[(wm + x * 16 + (lane >> 4) + 4 * e) * G2_T + wn + y * 16 + (lane & 15)] = acc[x][y][e];
     return;
   }
+  // C is read in batches of one 16-row block (16 values per lane) before
+  // any of the batch is written, and the batch's results are all formed
+  // before its stores: the compiler cannot prove the addresses distinct for a
+  // runtime ldc, so element-wise read-modify-write waited for each store.
+  // Interior tiles (inside C and, for lower_only, below the diagonal) take
+  // unpredicated loads and stores, which keep each store's data in its own
+  // registers; edge tiles check every element.
+  const bool interior = i0 + G2_T <= m && j0 + G2_T <= n && (!lower_only || j0 + G2_T <= i0);
 #pragma unroll
-  for (int x = 0; x < 4; ++x)
+  for (int xb = 0; xb < 4; ++xb) {
+    double cv[1][4][4];
 #pragma unroll
-    for (int y = 0; y < 4; ++y)
+    for (int x = 0; x < 1; ++x)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int64_t gi = i0 + wm + x * 16 + (lane >> 4) + 4 * e;
-        const int64_t gj = j0 + wn + y * 16 + (lane & 15);
-        if (gi < m && gj < n && (!lower_only || gj <= gi)) {
-          double v = alpha * acc[x][y][e];
-          if (beta != 0.0) v += beta * c[gi * ldc + gj];
-          c[gi * ldc + gj] = v;
-        }
-      }
+      for (int y = 0; y < 4; ++y)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) cv[x][y][e] = 0.0;
+    if (beta != 0.0) {
+#pragma unroll
+      for (int x = 0; x < 1; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int64_t gi = i0 + wm + (xb + x) * 16 + (lane >> 4) + 4 * e;
+            const int64_t gj = j0 + wn + y * 16 + (lane & 15);
+            if (interior || (gi < m && gj < n && (!lower_only || gj <= gi))) cv[x][y][e] = c[gi * ldc + gj];
+          }
+    }
+#pragma unroll
+    for (int x = 0; x < 1; ++x)
+#pragma unroll
+      for (int y = 0; y < 4; ++y)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) cv[x][y][e] = alpha * acc[xb + x][y][e] + beta * cv[x][y][e];
+    if (interior) {
+#pragma unroll
+      for (int x = 0; x < 1; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            c[(i0 + wm + (xb + x) * 16 + (lane >> 4) + 4 * e) * ldc + j0 + wn + y * 16 + (lane & 15)] = cv[x][y][e];
+    } else {
+#pragma unroll
+      for (int x = 0; x < 1; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int64_t gi = i0 + wm + (xb + x) * 16 + (lane >> 4) + 4 * e;
+            const int64_t gj = j0 + wn + y * 16 + (lane & 15);
+            if (gi < m && gj < n && (!lower_only || gj <= gi)) c[gi * ldc + gj] = cv[x][y][e];
+          }
+    }
+  }
 }
 
 
