@@ -352,18 +352,18 @@ int nka_gram_bf16x3(const uint16_t* x_hi, const uint16_t* x_lo, int64_t n, int64
  * matrix = O_L^T O_L + diag_shift I of netket/_src/ngd/sr.py:43-55 and
  * QGTJacobianDenseT.to_dense, netket/optimizer/qgt/qgt_jacobian_dense.py:29-161):
  * out = scale * O_c^T O_c + shift I (lower triangle incl. diagonal, FP64,
- * row-major ldo >= n_params), O_c[s,p] = O[s,p] - mean[p], O as for
- * nka_rbm_contract, mean the sample means of O (nka_rbm_contract with
- * weights 1/ns), tT the (n_hidden x ns) activations of nka_rbm_prepare.
+ * row-major ldo >= n_params), O_c[s,p] = O[s,p] - mean_s O[s,p], O as for
+ * nka_rbm_contract, tT the (n_hidden x ns) activations of nka_rbm_prepare.
  * Each O_p is a product (augmented spin) x (augmented tanh), so
  * sum_s O_sp O_sq is one entry of C = P Q^T over the samples, P the distinct
- * spin-pair products (exact in bf16) and Q the distinct activation-pair
- * products (bf16 hi + lo): a third of the matrix-core work of the bf16x3
- * Gram of the Jacobian.  FP32 models with +-1 spins only; workspace holds
- * nka_rbm_qgt_workspace(m, ns) bytes. */
+ * spin-pair products (exact in bf16) and Q the distinct pair products of the
+ * centred activations (bf16 hi + lo): a third of the matrix-core work of the
+ * bf16x3 Gram of the Jacobian.  The column sums of O come from the same
+ * product, so the centring cancels the large terms exactly.  FP32 models
+ * with +-1 spins only; workspace holds nka_rbm_qgt_workspace(m, ns) bytes. */
 int64_t nka_rbm_qgt_workspace(const nka_rbm* m, int64_t ns);
-int nka_rbm_qgt(const nka_rbm* m, const int8_t* sigma, const void* tT, int64_t ns, const double* mean,
-                double scale, double shift, double* out, int64_t ldo, void* workspace, void* stream);
+int nka_rbm_qgt(const nka_rbm* m, const int8_t* sigma, const void* tT, int64_t ns, double scale, double shift,
+                double* out, int64_t ldo, void* workspace, void* stream);
 
 /* Centred NTK of the RBM without forming its Jacobian: out = scale * O_c O_c^T
  * + shift I + offset 11^T (lower triangle incl. diagonal, FP64, row-major ldo),

@@ -171,12 +171,12 @@ def qgt_structured_available(model, flat) -> bool:
     return model.kind == "rbm" and flat.dtype == torch.float32
 
 
-def qgt_structured(model, flat, sigma2d: torch.Tensor, tT: torch.Tensor, mean: torch.Tensor, scale: float,
-                   shift: float, out: torch.Tensor | None = None) -> torch.Tensor:
-    """scale (O - mean)^T (O - mean) + shift I on the lower triangle (FP64,
+def qgt_structured(model, flat, sigma2d: torch.Tensor, tT: torch.Tensor, scale: float, shift: float,
+                   out: torch.Tensor | None = None) -> torch.Tensor:
+    """scale (O - <O>)^T (O - <O>) + shift I on the lower triangle (FP64,
     n_params x n_params) from the RBM's Khatri-Rao structure, without a
-    Jacobian; ``mean`` must be the sample means of O and ``tT`` the
-    (n_hidden, n_samples) activations."""
+    Jacobian; ``tT`` holds the (n_hidden, n_samples) activations and <O> the
+    sample means."""
     n, N = sigma2d.shape
     np_ = model.layout(N)["n_params"]
     if out is None or out.shape != (np_, np_):
@@ -184,8 +184,8 @@ def qgt_structured(model, flat, sigma2d: torch.Tensor, tT: torch.Tensor, mean: t
     m = model.native(flat, N)
     ws = _ops.WORKSPACE.get("rbm_qgt", nat.lib().nka_rbm_qgt_workspace(C.byref(m), n))
     with _ops.TIMER.region("qgt_structured"):
-        nat.call("nka_rbm_qgt", C.byref(m), _ops.ptr(sigma2d), _ops.ptr(tT), n, _ops.ptr(mean), float(scale),
-                 float(shift), _ops.ptr(out), out.stride(0), _ops.ptr(ws), _ops.stream())
+        nat.call("nka_rbm_qgt", C.byref(m), _ops.ptr(sigma2d), _ops.ptr(tT), n, float(scale), float(shift),
+                 _ops.ptr(out), out.stride(0), _ops.ptr(ws), _ops.stream())
     return out
 
 

@@ -150,7 +150,7 @@ SIGNATURES = {
     "nka_rbm_jacobian": (i32, [C.POINTER(NkaRbm), vp, vp, vp, i64, vp, f64, i32, i32, vp, vp, i64, vp]),
     "nka_gram_bf16x3": (i32, [vp, vp, i64, i64, i64, f64, f64, vp, i64, vp]),
     "nka_rbm_qgt_workspace": (i64, [C.POINTER(NkaRbm), i64]),
-    "nka_rbm_qgt": (i32, [C.POINTER(NkaRbm), vp, vp, i64, vp, f64, f64, vp, i64, vp, vp]),
+    "nka_rbm_qgt": (i32, [C.POINTER(NkaRbm), vp, vp, i64, f64, f64, vp, i64, vp, vp]),
     "nka_rbm_ntk_workspace": (i64, [C.POINTER(NkaRbm), i64]),
     "nka_rbm_ntk": (i32, [C.POINTER(NkaRbm), vp, vp, i64, vp, f64, f64, f64, vp, i64, vp, vp]),
     "nka_syrk_f64": (i32, [vp, i64, i64, i64, f64, f64, f64, vp, i64, vp]),

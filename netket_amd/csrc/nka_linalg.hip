@@ -306,41 +306,55 @@ constexpr int kGramRbmWT = 64;
 // stride ldk, zero beyond the true K).  A is one bf16 plane whose entries
 // bf16 holds exactly (products of +-1 spins, 0 and 1); B is split into two
 // bf16 planes.  Each fragment pair therefore takes two products instead of
-// the Gram's three, and A's LDS slab is a single plane.  Same 2 WT x 2 WT
-// workgroup tile, K step, slab prefetch and fragment layout as the Gram
-// kernel, on a rectangular tile grid: tiles are numbered down the columns
-// (the ma side fastest), so the workgroups that share a B column panel
-// (the larger operand) sit on one XCD and read it through one L2.
+// the Gram's three, and A's LDS slab is a single plane.  Four waves of
+// 32 MIA x 32 MIB (v_mfma_f32_32x32x16_bf16) make a 64 MIA x 64 MIB tile;
+// K step, slab prefetch and fragment layout are the Gram kernel's.  Tiles are
+// numbered down the columns (the ma side fastest), so the workgroups that
+// share a B column panel (the larger operand) sit on one XCD and read it
+// through one L2.
+// The FP32 accumulation inside the MFMA rounds with a bias (the error of a
+// plain sum over 16384 samples grows like the number of accumulations times
+// 2^-24, not like its square root), so the products are accumulated on the
+// matrix cores for kExactAChunk K steps (128 accumulations) at a time and each
+// chunk is added into a separate FP32 total with round-to-nearest vector adds.
+// 192 x 128 tiles (3 x 2 blocks of 32 x 32 per wave) leave the registers for
+// that total; the single-plane operand takes the 192-row side, so a K step
+// reads 7 fragments per 12 products from LDS.
 // ---------------------------------------------------------------------------
-constexpr int kExactAWT = 96;
+constexpr int kExactAMIA = 3, kExactAMIB = 2;
+constexpr int kExactAChunk = 16;
 
-template <int WT>
+template <int MIA, int MIB>
 __global__ void __launch_bounds__(256, 1) gemm_exact_a_kernel(const uint16_t* __restrict__ a, int64_t ma,
                                                            const uint16_t* __restrict__ bh,
                                                            const uint16_t* __restrict__ bl, int64_t nb,
                                                            int64_t K, int64_t ldk, float* __restrict__ c,
                                                            int64_t ldc, int64_t tiles_m, int64_t n_tiles_total) {
-  constexpr int TT = 2 * WT;
-  constexpr int MI = WT / 32;
-  constexpr int NIT = TT * GB_CPR / 256;
-  __shared__ __attribute__((aligned(16))) uint16_t sA[TT * GB_LD];
-  __shared__ __attribute__((aligned(16))) uint16_t sB[2][TT * GB_LD];
+  constexpr int TA = 64 * MIA, TB = 64 * MIB;
+  constexpr int NITA = TA * GB_CPR / 256, NITB = TB * GB_CPR / 256;
+  constexpr int NIT = NITA > NITB ? NITA : NITB;
+  __shared__ __attribute__((aligned(16))) uint16_t sA[TA * GB_LD];
+  __shared__ __attribute__((aligned(16))) uint16_t sB[2][TB * GB_LD];
   const int64_t bid = xcd_remap(blockIdx.x, gridDim.x);
   if (bid >= n_tiles_total) return;
-  const int64_t i0 = (bid % tiles_m) * TT, j0 = (bid / tiles_m) * TT;
+  const int64_t i0 = (bid % tiles_m) * TA, j0 = (bid / tiles_m) * TB;
   const int tid = threadIdx.x, wid = tid / 64, lane = lane_id();
-  const int wm = (wid >> 1) * WT, wn = (wid & 1) * WT;
+  const int wm = (wid >> 1) * 32 * MIA, wn = (wid & 1) * 32 * MIB;
 
-  floatx16 acc[MI][MI];
+  floatx16 acc[MIA][MIB];
+  float tot[MIA][MIB][16];
 #pragma unroll
-  for (int x = 0; x < MI; ++x)
+  for (int x = 0; x < MIA; ++x)
 #pragma unroll
-    for (int y = 0; y < MI; ++y)
+    for (int y = 0; y < MIB; ++y)
 #pragma unroll
-      for (int e = 0; e < 16; ++e) acc[x][y][e] = 0.0f;
+      for (int e = 0; e < 16; ++e) {
+        acc[x][y][e] = 0.0f;
+        tot[x][y][e] = 0.0f;
+      }
 
   // chunks it0 <= it < it1 of the slab at k0: A, B hi, B lo
-  uint4 pf[NIT][3];
+  uint4 pa[NITA], pb[NITB][2];
   auto fetch_part = [&](int64_t k0, int it0, int it1) {
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
@@ -349,26 +363,43 @@ __global__ void __launch_bounds__(256, 1) gemm_exact_a_kernel(const uint16_t* __
       const int r = chunk / GB_CPR, kc = (chunk % GB_CPR) * 8;
       const int64_t kk = k0 + kc;
       const uint4 z = {0, 0, 0, 0};
-      pf[it][0] = z;
-      pf[it][1] = z;
-      pf[it][2] = z;
-      if (i0 + r < ma && kk < K) pf[it][0] = *(const uint4*)(a + (i0 + r) * ldk + kk);
-      if (j0 + r < nb && kk < K) {
-        pf[it][1] = *(const uint4*)(bh + (j0 + r) * ldk + kk);
-        pf[it][2] = *(const uint4*)(bl + (j0 + r) * ldk + kk);
+      if (it < NITA) {
+        pa[it] = z;
+        if (i0 + r < ma && kk < K) pa[it] = *(const uint4*)(a + (i0 + r) * ldk + kk);
+      }
+      if (it < NITB) {
+        pb[it][0] = z;
+        pb[it][1] = z;
+        if (j0 + r < nb && kk < K) {
+          pb[it][0] = *(const uint4*)(bh + (j0 + r) * ldk + kk);
+          pb[it][1] = *(const uint4*)(bl + (j0 + r) * ldk + kk);
+        }
       }
     }
   };
   fetch_part(0, 0, NIT);
   for (int64_t k0 = 0; k0 < K; k0 += GB_K) {
+    if (k0 != 0 && k0 % (GB_K * kExactAChunk) == 0) {
+#pragma unroll
+      for (int x = 0; x < MIA; ++x)
+#pragma unroll
+        for (int y = 0; y < MIB; ++y)
+#pragma unroll
+          for (int e = 0; e < 16; ++e) {
+            tot[x][y][e] += acc[x][y][e];
+            acc[x][y][e] = 0.0f;
+          }
+    }
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
       const int chunk = tid + it * 256;
       const int r = chunk / GB_CPR, kc = (chunk % GB_CPR) * 8;
-      *(uint4*)(&sA[r * GB_LD + kc]) = pf[it][0];
-      *(uint4*)(&sB[0][r * GB_LD + kc]) = pf[it][1];
-      *(uint4*)(&sB[1][r * GB_LD + kc]) = pf[it][2];
+      if (it < NITA) *(uint4*)(&sA[r * GB_LD + kc]) = pa[it];
+      if (it < NITB) {
+        *(uint4*)(&sB[0][r * GB_LD + kc]) = pb[it][0];
+        *(uint4*)(&sB[1][r * GB_LD + kc]) = pb[it][1];
+      }
     }
     __syncthreads();
     const bool more = k0 + GB_K < K;
@@ -382,36 +413,38 @@ __global__ void __launch_bounds__(256, 1) gemm_exact_a_kernel(const uint16_t* __
       }
       __builtin_amdgcn_sched_barrier(0);
       const int kq = ks + (lane >> 5) * 8;
-      bf16x8 af[MI], bhf[MI], blf[MI];
+      bf16x8 af[MIA], bhf[MIB], blf[MIB];
 #pragma unroll
-      for (int x = 0; x < MI; ++x) af[x] = *(const bf16x8*)(&sA[(wm + x * 32 + (lane & 31)) * GB_LD + kq]);
+      for (int x = 0; x < MIA; ++x) af[x] = *(const bf16x8*)(&sA[(wm + x * 32 + (lane & 31)) * GB_LD + kq]);
 #pragma unroll
-      for (int y = 0; y < MI; ++y) {
+      for (int y = 0; y < MIB; ++y) {
         const int row = wn + y * 32 + (lane & 31);
         bhf[y] = *(const bf16x8*)(&sB[0][row * GB_LD + kq]);
         blf[y] = *(const bf16x8*)(&sB[1][row * GB_LD + kq]);
       }
       // small products first
 #pragma unroll
-      for (int x = 0; x < MI; ++x)
+      for (int x = 0; x < MIA; ++x)
 #pragma unroll
-        for (int y = 0; y < MI; ++y) acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[x], blf[y], acc[x][y], 0, 0, 0);
+        for (int y = 0; y < MIB; ++y)
+          acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[x], blf[y], acc[x][y], 0, 0, 0);
 #pragma unroll
-      for (int x = 0; x < MI; ++x)
+      for (int x = 0; x < MIA; ++x)
 #pragma unroll
-        for (int y = 0; y < MI; ++y) acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[x], bhf[y], acc[x][y], 0, 0, 0);
+        for (int y = 0; y < MIB; ++y)
+          acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[x], bhf[y], acc[x][y], 0, 0, 0);
     }
   }
   // element e of lane l in a 32x32 tile is (row (e/4)*8 + (l/32)*4 + e%4, col l%32)
 #pragma unroll
-  for (int x = 0; x < MI; ++x)
+  for (int x = 0; x < MIA; ++x)
 #pragma unroll
-    for (int y = 0; y < MI; ++y)
+    for (int y = 0; y < MIB; ++y)
 #pragma unroll
       for (int e = 0; e < 16; ++e) {
         const int64_t gi = i0 + wm + x * 32 + (e / 4) * 8 + (lane >> 5) * 4 + (e & 3);
         const int64_t gj = j0 + wn + y * 32 + (lane & 31);
-        if (gi < ma && gj < nb) c[gi * ldc + gj] = acc[x][y][e];
+        if (gi < ma && gj < nb) c[gi * ldc + gj] = tot[x][y][e] + acc[x][y][e];
       }
 }
 
@@ -1720,11 +1753,11 @@ bool launch_gram_rbm(const uint16_t* xh, const uint16_t* xl, int64_t n, int64_t 
 // of 8, operands 16-byte aligned (checked by the caller).
 void launch_gemm_exact_a(const uint16_t* a, int64_t ma, const uint16_t* bh, const uint16_t* bl, int64_t nb,
                          int64_t K, int64_t ldk, float* c, int64_t ldc, hipStream_t s) {
-  constexpr int TT = 2 * kExactAWT;
-  const int64_t tm = (ma + TT - 1) / TT, tn = (nb + TT - 1) / TT;
+  constexpr int TA = 64 * kExactAMIA, TB = 64 * kExactAMIB;
+  const int64_t tm = (ma + TA - 1) / TA, tn = (nb + TB - 1) / TB;
   if (tm == 0 || tn == 0) return;
-  hipLaunchKernelGGL((gemm_exact_a_kernel<kExactAWT>), dim3(grid_mult8(tm * tn)), dim3(256), 0, s, a, ma, bh, bl,
-                     nb, K, ldk, c, ldc, tm, tm * tn);
+  hipLaunchKernelGGL((gemm_exact_a_kernel<kExactAMIA, kExactAMIB>), dim3(grid_mult8(tm * tn)), dim3(256), 0, s, a,
+                     ma, bh, bl, nb, K, ldk, c, ldc, tm, tm * tn);
 }
 
 bool launch_gram_ffnn(const uint16_t* h1h, const uint16_t* h1l, int64_t K0, int64_t ld0, const uint16_t* g2h,

@@ -160,11 +160,20 @@ void launch_gemm_exact_a(const uint16_t* a, int64_t ma, const uint16_t* bh, cons
 // for j <= l: 1 + M + M(M+1)/2 rows).  C has about Np^2 / 4 entries against
 // the Gram's Np^2 / 2, the spin side is exact in one bf16 plane, and the
 // activation side takes two (hi + lo), so the product needs a third of the
-// bf16x3 Gram's matrix-core work.  Centring and the shift are applied in
-// FP64 when C is unpacked into S:
-//   S_pq = scale C_pq - scale ns mean_p mean_q (+ shift if p == q),
-// which is scale sum_s (O_sp - mean_p)(O_sq - mean_q) when mean holds the
-// sample means of O.
+// bf16x3 Gram's matrix-core work.  Q is built from the centred activations
+// t'_j = t_j - c_j (c_j their sample means, rounded to FP32), so that the
+// rounding of the split and of the FP32 accumulation scales with |t'| rather
+// than |t|; with c~ = (0, c) the uncentred sum is recovered in FP64 as
+//   U(u,u',w,w') = C[r, (w,w')] + c~_w C[r, (-1,w')] + c~_w' C[r, (w,-1)]
+//                  + c~_w c~_w' C[r, (-1,-1)],   r = pair(u, u'),
+// and centring and the shift are applied when C is unpacked into S:
+//   S_pq = scale (U_pq - U_p U_q / ns) (+ shift if p == q),
+// U_p = sum_s O_sp = U(u_p, -1, w_p, -1) from the same entries of C.  Taking
+// the column sums from the product itself, and not from an exact FP64 mean,
+// makes the large terms (c~_w times a pair sum) cancel exactly between U_pq
+// and U_p U_q / ns, so the remaining rounding scales with the centred
+// activations.  The spin side needs no centring: its pair sums are exact
+// integers.
 // ---------------------------------------------------------------------------
 
 // Augmented spin index u in [-1, N) (-1: the constant 1).  Row of P holding
@@ -252,12 +261,29 @@ __global__ void __launch_bounds__(256) qgt_spin_pairs_kernel(const int8_t* __res
   }
 }
 
+// c_j = mean over the ns samples of tT[j][:] (FP64, fixed summation order),
+// one workgroup per hidden unit.
+__global__ void __launch_bounds__(256) qgt_hidden_means_kernel(const float* __restrict__ tT, int64_t ns, int M,
+                                                               double* __restrict__ c) {
+  __shared__ double red[4];
+  for (int j = blockIdx.x; j < M; j += gridDim.x) {
+    double acc = 0.0;
+    for (int64_t s = threadIdx.x; s < ns; s += 256) acc += (double)tT[(int64_t)j * ns + s];
+    acc = wave_sum(acc);
+    __syncthreads();
+    if (lane_id() == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) c[j] = (red[0] + red[1] + red[2] + red[3]) / (double)ns;
+  }
+}
+
 // Q hi + lo (rows x ldk bf16 each): row 0 = 1, row 1 + j = t_j, row
 // 1 + M + l(l+1)/2 + j = t_j t_l (j <= l, FP32 product), split as
 // hi = bf16(q), lo = bf16(q - hi); zero for the padding samples.  tT is
-// (M x ns) FP32.
+// (M x ns) FP32; t_j here is the centred t_j - (float) c_j.
 __global__ void __launch_bounds__(256) qgt_hidden_pairs_kernel(const float* __restrict__ tT, int64_t ns, int M,
-                                                               int64_t ldk, int64_t rows, uint16_t* __restrict__ qh,
+                                                               const double* __restrict__ cmean, int64_t ldk,
+                                                               int64_t rows, uint16_t* __restrict__ qh,
                                                                uint16_t* __restrict__ ql) {
   const bool vec = (ns % 4) == 0 && (((uintptr_t)tT) % 16) == 0;
   for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
@@ -269,6 +295,7 @@ __global__ void __launch_bounds__(256) qgt_hidden_pairs_kernel(const float* __re
       l = tri_root(b);
       j = b - l * (l + 1) / 2;
     }
+    const float cj = j >= 0 ? (float)cmean[j] : 0.0f, cl = l >= 0 ? (float)cmean[l] : 0.0f;
     for (int64_t s0 = (int64_t)threadIdx.x * 8; s0 < ldk; s0 += 256 * 8) {
       float f[8];
       if (j < 0) {
@@ -278,21 +305,21 @@ __global__ void __launch_bounds__(256) qgt_hidden_pairs_kernel(const float* __re
         const float* a = tT + j * ns + s0;
         if (vec && s0 + 8 <= ns) {
           const float4 x = *(const float4*)a, y = *(const float4*)(a + 4);
-          f[0] = x.x; f[1] = x.y; f[2] = x.z; f[3] = x.w;
-          f[4] = y.x; f[5] = y.y; f[6] = y.z; f[7] = y.w;
+          f[0] = x.x - cj; f[1] = x.y - cj; f[2] = x.z - cj; f[3] = x.w - cj;
+          f[4] = y.x - cj; f[5] = y.y - cj; f[6] = y.z - cj; f[7] = y.w - cj;
         } else {
 #pragma unroll
-          for (int e = 0; e < 8; ++e) f[e] = s0 + e < ns ? a[e] : 0.0f;
+          for (int e = 0; e < 8; ++e) f[e] = s0 + e < ns ? a[e] - cj : 0.0f;
         }
         if (l >= 0) {
           const float* b = tT + l * ns + s0;
           if (vec && s0 + 8 <= ns) {
             const float4 x = *(const float4*)b, y = *(const float4*)(b + 4);
-            f[0] *= x.x; f[1] *= x.y; f[2] *= x.z; f[3] *= x.w;
-            f[4] *= y.x; f[5] *= y.y; f[6] *= y.z; f[7] *= y.w;
+            f[0] *= x.x - cl; f[1] *= x.y - cl; f[2] *= x.z - cl; f[3] *= x.w - cl;
+            f[4] *= y.x - cl; f[5] *= y.y - cl; f[6] *= y.z - cl; f[7] *= y.w - cl;
           } else {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) f[e] *= s0 + e < ns ? b[e] : 0.0f;
+            for (int e = 0; e < 8; ++e) f[e] *= s0 + e < ns ? b[e] - cl : 0.0f;
           }
         }
       }
@@ -314,34 +341,53 @@ __global__ void __launch_bounds__(256) qgt_hidden_pairs_kernel(const float* __re
 // S from C, one workgroup per augmented spin pair (u_hi >= u_lo in [-1, N)):
 // all entries S[p][q] with p = (u_hi, w), q = (u_lo, w') for w, w' in
 // [-1, M), each written once into the lower triangle (at [max][min]).  The
-// C row of the pair is staged in LDS when it fits (LDS_ROW).  Threads run
-// along w', whose positions are consecutive in S, so the writes of a row
-// segment are contiguous.
+// C row of the pair and the first M + 1 entries of the rows of (u_hi, -1) and
+// (u_lo, -1) (the column sums of the two parameter groups) are staged in LDS
+// when they fit (LDS_ROW).  Threads run along w', whose positions are
+// consecutive in S, so the writes of a row segment are contiguous.
 template <bool LDS_ROW>
 __global__ void __launch_bounds__(256) qgt_unpack_kernel(nka_rbm m, const float* __restrict__ c, int64_t ldc,
-                                                         int64_t nb, const double* __restrict__ mean,
-                                                         double scale, double cs, double shift,
-                                                         double* __restrict__ out, int64_t ldo) {
+                                                         int64_t nb, const double* __restrict__ cmean, double ns,
+                                                         double scale, double shift, double* __restrict__ out,
+                                                         int64_t ldo) {
   extern __shared__ float crow[];
   const int N = m.n_visible, M = m.n_hidden;
   const int64_t n_pairs = (int64_t)(N + 1) * (N + 2) / 2;
   const int W = M + 1;
+  const double rns = 1.0 / ns;
   for (int64_t b = blockIdx.x; b < n_pairs; b += gridDim.x) {
     const int64_t ah = tri_root(b);
     const int uh = (int)ah - 1, ul = (int)(b - ah * (ah + 1) / 2) - 1;
     const float* src = c + qgt_spin_row(uh, ul, N) * ldc;
+    const float* srch = c + qgt_spin_row(uh, -1, N) * ldc;  // entries 0..M: sums of (u_hi, w)
+    const float* srcl = c + qgt_spin_row(ul, -1, N) * ldc;
     if (LDS_ROW) {
       __syncthreads();
       for (int64_t e = threadIdx.x; e < nb; e += blockDim.x) crow[e] = src[e];
+      for (int e = threadIdx.x; e < W; e += blockDim.x) {
+        crow[nb + e] = srch[e];
+        crow[nb + W + e] = srcl[e];
+      }
       __syncthreads();
     }
     const float* row = LDS_ROW ? crow : src;
+    const float* rowh = LDS_ROW ? crow + nb : srch;
+    const float* rowl = LDS_ROW ? crow + nb + W : srcl;
     for (int64_t e = threadIdx.x; e < (int64_t)W * W; e += blockDim.x) {
       const int w = (int)(e / W) - 1, x = (int)(e % W) - 1;
       const int64_t p = qgt_param(uh, w, m), q = qgt_param(ul, x, m);
       if (p < 0 || q < 0) continue;
       if (uh == ul && p < q) continue;  // the symmetric diagonal pair: each entry once
-      double v = scale * (double)row[qgt_hidden_row(w, x, M)] - cs * mean[p] * mean[q];
+      // uncentred sums from the centred activations' pair sums
+      const double cw = w >= 0 ? (double)(float)cmean[w] : 0.0, cx = x >= 0 ? (double)(float)cmean[x] : 0.0;
+      double u = (double)row[qgt_hidden_row(w, x, M)];
+      if (w >= 0) u += cw * (double)row[qgt_hidden_row(-1, x, M)];
+      if (x >= 0) u += cx * (double)row[qgt_hidden_row(w, -1, M)];
+      if (w >= 0 && x >= 0) u += cw * cx * (double)row[0];
+      double up = (double)rowh[w + 1], uq = (double)rowl[x + 1];
+      if (w >= 0) up += cw * (double)rowh[0];
+      if (x >= 0) uq += cx * (double)rowl[0];
+      double v = scale * (u - up * uq * rns);
       if (p == q) v += shift;
       if (p >= q)
         out[p * ldo + q] = v;
@@ -581,12 +627,13 @@ int64_t nka_rbm_qgt_workspace(const nka_rbm* m, int64_t ns) {
   if (!m || ns <= 0) return 0;
   const int64_t ldk = qgt_ldk(ns);
   return qgt_round((int64_t)m->n_visible * ldk) + qgt_round(qgt_rows_p(*m) * ldk * 2) +
-         2 * qgt_round(qgt_rows_q(*m) * ldk * 2) + qgt_round(qgt_rows_p(*m) * qgt_ldc(*m) * 4);
+         2 * qgt_round(qgt_rows_q(*m) * ldk * 2) + qgt_round(qgt_rows_p(*m) * qgt_ldc(*m) * 4) +
+         qgt_round((int64_t)m->n_hidden * 8);
 }
 
-int nka_rbm_qgt(const nka_rbm* m, const int8_t* sigma, const void* tT, int64_t ns, const double* mean,
-                double scale, double shift, double* out, int64_t ldo, void* workspace, void* stream) {
-  NKA_CHECK_ARG(m && m->params && sigma && tT && mean && out && workspace && ns > 0,
+int nka_rbm_qgt(const nka_rbm* m, const int8_t* sigma, const void* tT, int64_t ns, double scale, double shift,
+                double* out, int64_t ldo, void* workspace, void* stream) {
+  NKA_CHECK_ARG(m && m->params && sigma && tT && out && workspace && ns > 0,
                 "nka_rbm_qgt: bad arguments");
   NKA_CHECK_ARG(m->dtype == NKA_F32, "nka_rbm_qgt: FP32 models only (FP64 models take the FP64 Jacobian)");
   const int N = m->n_visible, M = m->n_hidden;
@@ -605,22 +652,26 @@ int nka_rbm_qgt(const nka_rbm* m, const int8_t* sigma, const void* tT, int64_t n
   uint16_t* ql = (uint16_t*)w;
   w += qgt_round(rq * ldk * 2);
   float* C = (float*)w;
+  w += qgt_round(rp * ldc * 4);
+  double* cmean = (double*)w;
   const unsigned g0 = (unsigned)std::min<int64_t>(((int64_t)N * ldk + 255) / 256, 65536);
   hipLaunchKernelGGL(qgt_spins_t_kernel, dim3(g0), dim3(256), 0, s, sigma, ns, N, ldk, sT);
   hipLaunchKernelGGL(qgt_spin_pairs_kernel, dim3((unsigned)std::min<int64_t>(rp, 65536)), dim3(256), 0, s, sT,
                      ns, N, ldk, rp, P);
+  hipLaunchKernelGGL(qgt_hidden_means_kernel, dim3((unsigned)std::min<int64_t>(M, 65536)), dim3(256), 0, s,
+                     (const float*)tT, ns, M, cmean);
   hipLaunchKernelGGL(qgt_hidden_pairs_kernel, dim3((unsigned)std::min<int64_t>(rq, 65536)), dim3(256), 0, s,
-                     (const float*)tT, ns, M, ldk, rq, qh, ql);
+                     (const float*)tT, ns, M, (const double*)cmean, ldk, rq, qh, ql);
   launch_gemm_exact_a(P, rp, qh, ql, rq, ldk, ldk, C, ldc, s);
   const int64_t pairs = (int64_t)(N + 1) * (N + 2) / 2;
   const unsigned gu = (unsigned)std::min<int64_t>(pairs, 65536);
-  const size_t row_bytes = (size_t)rq * sizeof(float);
+  const size_t row_bytes = (size_t)(rq + 2 * (M + 1)) * sizeof(float);
   if (row_bytes <= 64 * 1024)
-    hipLaunchKernelGGL(qgt_unpack_kernel<true>, dim3(gu), dim3(256), row_bytes, s, *m, C, ldc, rq, mean, scale,
-                       scale * (double)ns, shift, out, ldo);
+    hipLaunchKernelGGL(qgt_unpack_kernel<true>, dim3(gu), dim3(256), row_bytes, s, *m, C, ldc, rq,
+                       (const double*)cmean, (double)ns, scale, shift, out, ldo);
   else
-    hipLaunchKernelGGL(qgt_unpack_kernel<false>, dim3(gu), dim3(256), 0, s, *m, C, ldc, rq, mean, scale,
-                       scale * (double)ns, shift, out, ldo);
+    hipLaunchKernelGGL(qgt_unpack_kernel<false>, dim3(gu), dim3(256), 0, s, *m, C, ldc, rq, (const double*)cmean,
+                       (double)ns, scale, shift, out, ldo);
   NKA_CHECK_LAUNCH("nka_rbm_qgt");
   return NKA_OK;
 }

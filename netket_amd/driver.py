@@ -306,7 +306,7 @@ class VMC_SR(AbstractVariationalDriver):
                 eng.axpy_(F, diag_shift * float(momentum), self._old_updates)
             if structured_s:
                 system = solvers.DenseSystem(builder=lambda: eng.qgt_structured(
-                    model, flat, sig, tT, mean, 1.0 / ns, diag_shift))
+                    model, flat, sig, tT, 1.0 / ns, diag_shift))
             else:
                 system = solvers.DenseSystem(builder=lambda: eng.gram(X, diag_shift))
             rhs = F

@@ -169,11 +169,9 @@ def test_rbm_qgt_structured(gpu, ns, biases, N, alpha):
     sig = torch.as_tensor(x, device="cuda")
     fl = torch.as_tensor(flat, device="cuda")
     t, tT = eng.prepare(model, fl, sig, row_major=True, transposed=True)
-    ones = torch.full((1, ns), 1.0 / ns, dtype=torch.float64, device="cuda")
-    mean = eng.contract(model, fl, sig, t, ones)[0]
     shift = 1e-2
     out = torch.full((n_par, n_par), 7.0, dtype=torch.float64, device="cuda")
-    S = eng.qgt_structured(model, fl, sig, tT, mean, 1.0 / ns, shift, out=out).cpu().numpy()
+    S = eng.qgt_structured(model, fl, sig, tT, 1.0 / ns, shift, out=out).cpu().numpy()
     p = orbm.RBMParams(flat.astype(np.float64), N, M, hidden_bias=biases[0], visible_bias=biases[1])
     Ou = orbm.jacobian(p, x)
     Oc = Ou - Ou.mean(axis=0)
@@ -192,6 +190,39 @@ def test_rbm_qgt_structured(gpu, ns, biases, N, alpha):
     assert np.all(S[up] == 7.0)
 
 
+def test_rbm_qgt_structured_large_mean_activations(gpu):
+    # hidden biases of 2 put every tanh near 0.96 with a small spread, so
+    # E[t_j t_l] is far larger than S's entries: the activations are centred
+    # before the bf16 split, and the error must stay small against the
+    # centred diagonal, not only against the uncentred norms
+    print("case rbm_qgt large-mean activations", flush=True)
+    N, alpha, ns = 16, 2, 2048
+    M = alpha * N
+    model = nk.models.RBM(alpha=alpha, param_dtype=np.float32)
+    lay = model.layout(N)
+    rng = np.random.default_rng(11)
+    flat = rng.normal(scale=0.05, size=lay["n_params"]).astype(np.float32)
+    b0 = lay["bias"][0]
+    flat[b0:b0 + M] = 2.0
+    x = rng.choice(np.array([-1, 1], dtype=np.int8), size=(ns, N))
+    sig = torch.as_tensor(x, device="cuda")
+    fl = torch.as_tensor(flat, device="cuda")
+    t, tT = eng.prepare(model, fl, sig, row_major=True, transposed=True)
+    S = eng.qgt_structured(model, fl, sig, tT, 1.0 / ns, 0.0).cpu().numpy()
+    p = orbm.RBMParams(flat.astype(np.float64), N, M)
+    Ou = orbm.jacobian(p, x)
+    Oc = Ou - Ou.mean(axis=0)
+    ref = Oc.T @ Oc / ns
+    lo = np.tril_indices(lay["n_params"])
+    d = np.sqrt(np.diag(ref))
+    ratio = (np.abs(S - ref) / np.outer(d, d))[lo].max()
+    spread = (np.sqrt(np.sum(Ou * Ou, axis=0) / ns) / d)[b0:b0 + M].min()
+    print(f"uncentred / centred norm of the hidden-bias columns >= {spread:.1f}; "
+          f"max |dS| / sqrt(S_ii S_jj) = {ratio:.2e}", flush=True)
+    assert spread > 20.0
+    assert ratio < 3e-5
+
+
 def test_rbm_qgt_structured_full_size(gpu):
     # the benchmark's shape (N = 64, alpha = 2, 16384 samples, S 8384^2) against
     # the FP64 product of the same Jacobian formed on the GPU
@@ -207,9 +238,7 @@ def test_rbm_qgt_structured_full_size(gpu):
     sig = torch.as_tensor(x, device="cuda")
     fl = torch.as_tensor(flat, device="cuda")
     t, tT = eng.prepare(model, fl, sig, row_major=True, transposed=True)
-    ones = torch.full((1, ns), 1.0 / ns, dtype=torch.float64, device="cuda")
-    mean = eng.contract(model, fl, sig, t, ones)[0]
-    S = eng.qgt_structured(model, fl, sig, tT, mean, 1.0 / ns, 0.01)
+    S = eng.qgt_structured(model, fl, sig, tT, 1.0 / ns, 0.01)
     lay = model.layout(N)
     M = lay["M"]
     t64 = t.double()
@@ -221,14 +250,20 @@ def test_rbm_qgt_structured_full_size(gpu):
     du = torch.sqrt((O * O).sum(0) / ns)
     O -= O.mean(0)
     ref = O.T @ O / ns
-    ref.diagonal().add_(0.01)
     del O
+    d = torch.sqrt(ref.diagonal().clone())
+    ref.diagonal().add_(0.01)
     L = torch.tril(torch.ones(n_par, n_par, dtype=torch.bool, device="cuda"))
     err = (S - ref).abs()
     bound = 3e-5 * du[:, None] * du[None, :] + 1e-12
     ratio = torch.where(L, err / bound, torch.zeros_like(err)).max().item()
-    print(f"max |dS| / bound = {ratio:.2e}", flush=True)
+    # the centred activations, the column sums taken from the same product
+    # and the chunked accumulation keep the error small against the centred
+    # diagonal too
+    rel = torch.where(L, err / (d[:, None] * d[None, :]), torch.zeros_like(err)).max().item()
+    print(f"max |dS| / bound = {ratio:.2e}; max |dS| / sqrt(S_ii S_jj) = {rel:.2e}", flush=True)
     assert ratio <= 1.0
+    assert rel < 3e-5
 
 
 @pytest.mark.parametrize("solver_name", ["cholesky", "cg"])
