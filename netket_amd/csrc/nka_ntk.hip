@@ -355,6 +355,7 @@ __global__ void __launch_bounds__(256) qgt_unpack_kernel(nka_rbm m, const float*
   const int64_t n_pairs = (int64_t)(N + 1) * (N + 2) / 2;
   const int W = M + 1;
   const double rns = 1.0 / ns;
+  const int wid = threadIdx.x >> 6, lane = lane_id();
   for (int64_t b = blockIdx.x; b < n_pairs; b += gridDim.x) {
     const int64_t ah = tri_root(b);
     const int uh = (int)ah - 1, ul = (int)(b - ah * (ah + 1) / 2) - 1;
@@ -373,19 +374,17 @@ __global__ void __launch_bounds__(256) qgt_unpack_kernel(nka_rbm m, const float*
     const float* row = LDS_ROW ? crow : src;
     const float* rowh = LDS_ROW ? crow + nb : srch;
     const float* rowl = LDS_ROW ? crow + nb + W : srcl;
-    for (int64_t e = threadIdx.x; e < (int64_t)W * W; e += blockDim.x) {
-      const int w = (int)(e / W) - 1, x = (int)(e % W) - 1;
-      const int64_t p = qgt_param(uh, w, m), q = qgt_param(ul, x, m);
-      if (p < 0 || q < 0) continue;
-      if (uh == ul && p < q) continue;  // the symmetric diagonal pair: each entry once
-      // uncentred sums from the centred activations' pair sums
-      const double cw = w >= 0 ? (double)(float)cmean[w] : 0.0, cx = x >= 0 ? (double)(float)cmean[x] : 0.0;
+    // one element: p = (u_hi, w), q = (u_lo, x); the row's terms are passed in
+    auto entry = [&](int w, int64_t p, double cw, double up, int x) {
+      const int64_t q = qgt_param(ul, x, m);
+      if (q < 0 || (uh == ul && p < q)) return;  // the symmetric diagonal pair: each entry once
+      const double cx = x >= 0 ? (double)(float)cmean[x] : 0.0;
+      // the uncentred sum from the centred activations' pair sums
       double u = (double)row[qgt_hidden_row(w, x, M)];
       if (w >= 0) u += cw * (double)row[qgt_hidden_row(-1, x, M)];
       if (x >= 0) u += cx * (double)row[qgt_hidden_row(w, -1, M)];
       if (w >= 0 && x >= 0) u += cw * cx * (double)row[0];
-      double up = (double)rowh[w + 1], uq = (double)rowl[x + 1];
-      if (w >= 0) up += cw * (double)rowh[0];
+      double uq = (double)rowl[x + 1];
       if (x >= 0) uq += cx * (double)rowl[0];
       double v = scale * (u - up * uq * rns);
       if (p == q) v += shift;
@@ -393,6 +392,25 @@ __global__ void __launch_bounds__(256) qgt_unpack_kernel(nka_rbm m, const float*
         out[p * ldo + q] = v;
       else
         out[q * ldo + p] = v;
+    };
+    // rows w of the block, one wavefront each; lanes along x >= 0, whose
+    // positions q are consecutive in S
+    for (int w = wid - 1; w < M; w += 4) {
+      const int64_t p = qgt_param(uh, w, m);
+      if (p < 0) continue;
+      const double cw = w >= 0 ? (double)(float)cmean[w] : 0.0;
+      double up = (double)rowh[w + 1];
+      if (w >= 0) up += cw * (double)rowh[0];
+      for (int x = lane; x < M; x += 64) entry(w, p, cw, up, x);
+    }
+    // the column x = -1 (the visible bias of u_lo), one row per thread
+    for (int w = (int)threadIdx.x - 1; w < M; w += blockDim.x) {
+      const int64_t p = qgt_param(uh, w, m);
+      if (p < 0) continue;
+      const double cw = w >= 0 ? (double)(float)cmean[w] : 0.0;
+      double up = (double)rowh[w + 1];
+      if (w >= 0) up += cw * (double)rowh[0];
+      entry(w, p, cw, up, -1);
     }
   }
 }
