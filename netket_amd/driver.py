@@ -251,6 +251,7 @@ class VMC_SR(AbstractVariationalDriver):
         self.qgt_method = qgt_method
         self._old_updates = None
         self._jac = None
+        self._side = None  # second stream of the structured S
         self.info = None
         # when a dict, each step leaves its local energies, right-hand side,
         # solution, update and linear system there (for error reports)
@@ -264,19 +265,39 @@ class VMC_SR(AbstractVariationalDriver):
         the local energies of the current samples were computed elsewhere (the
         serving loop's shared launches); otherwise they are computed here."""
         vs = self.state
+        diag_shift = float(self._hyper(self.diag_shift))
+        proj_reg = self._hyper(self.proj_reg)
+        momentum = self._hyper(self.momentum)
+        model, flat = vs.model, vs.parameters_flat
+        structured_s = (not self.use_ntk and self.qgt_method == "structured"
+                        and eng.qgt_structured_available(model, flat))
+        pc = vs._prepared()
+        sig, t = pc["sigma"], pc["t"]
+        ns = sig.shape[0]
+        need_tT = not self.use_ntk
+        tT = eng.prepare(model, flat, sig, row_major=False, transposed=True)[1] if need_tT else None
+        S_pre = None
+        if structured_s and eloc is None:
+            # S depends only on the samples: it is formed on a second stream
+            # while this stream computes the local energies, the statistics and
+            # the contractions (matrix cores against the scalar unit and
+            # latency-bound work), and joined before the solve
+            main = torch.cuda.current_stream()
+            if self._side is None:
+                self._side = torch.cuda.Stream()
+            np_ = model.layout(sig.shape[1])["n_params"]
+            S_pre = torch.empty((np_, np_), dtype=torch.float64, device=sig.device)
+            self._side.wait_stream(main)
+            with torch.cuda.stream(self._side):
+                eng.qgt_structured(model, flat, sig, tT, 1.0 / ns, diag_shift, out=S_pre)
+            # the allocator keeps these blocks until the second stream's work on
+            # them is done, also if this step stops before the join
+            for x in (S_pre, sig, tT, flat):
+                x.record_stream(self._side)
         if eloc is None:
             eloc = vs.local_estimators(self._ham)
         st = statistics(eloc)
         e = eloc.reshape(-1)
-        ns = e.numel()
-        diag_shift = float(self._hyper(self.diag_shift))
-        proj_reg = self._hyper(self.proj_reg)
-        momentum = self._hyper(self.momentum)
-        pc = vs._prepared()
-        sig, t = pc["sigma"], pc["t"]
-        model, flat = vs.model, vs.parameters_flat
-        need_tT = not self.use_ntk
-        tT = eng.prepare(model, flat, sig, row_major=False, transposed=True)[1] if need_tT else None
         ones = torch.full((1, ns), 1.0 / ns, dtype=torch.float64, device=e.device)
         mean = eng.contract(model, flat, sig, t, ones)[0]
         # dv = 2 (E - <E>) / sqrt(Ns)   (sr_srt_common.py:75-77)
@@ -284,8 +305,6 @@ class VMC_SR(AbstractVariationalDriver):
         # contraction weights dv / sqrt(Ns) = 2 (E - <E>) / Ns
         w_dv = eng.centre_scale(e, st.mean_device, 2.0 / ns)
         structured = self.use_ntk and self.ntk_method == "structured"
-        structured_s = (not self.use_ntk and self.qgt_method == "structured"
-                        and eng.qgt_structured_available(model, flat))
         if not structured and not structured_s:
             kind = nat.OUT_BF16X2 if flat.dtype == torch.float32 else nat.OUT_F64
             layout = nat.LAYOUT_SP if self.use_ntk else nat.LAYOUT_PS
@@ -305,8 +324,11 @@ class VMC_SR(AbstractVariationalDriver):
                 # F + eps mu old_updates (netket/_src/ngd/sr.py:45-47)
                 eng.axpy_(F, diag_shift * float(momentum), self._old_updates)
             if structured_s:
-                system = solvers.DenseSystem(builder=lambda: eng.qgt_structured(
-                    model, flat, sig, tT, 1.0 / ns, diag_shift))
+                build_s = lambda: eng.qgt_structured(model, flat, sig, tT, 1.0 / ns, diag_shift)
+                if S_pre is not None:
+                    torch.cuda.current_stream().wait_stream(self._side)
+                system = solvers.DenseSystem(S_pre, builder=build_s) if S_pre is not None else \
+                    solvers.DenseSystem(builder=build_s)
             else:
                 system = solvers.DenseSystem(builder=lambda: eng.gram(X, diag_shift))
             rhs = F
