@@ -526,6 +526,33 @@ def measured_traffic(kernel: str):
         return None
 
 
+def measured_traffic_per_call(kernels, per_call_kernel: str):
+    """HBM bytes per call of an entry point that launches several kernels: the
+    sum over ``kernels`` of bytes per launch x launches, divided by the number
+    of calls, which is the dispatch count of ``per_call_kernel`` (launched once
+    per call), from profiles/hbm_traffic.json; None when a kernel is missing."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            recs = json.load(f)["kernels"]
+    except (OSError, KeyError, ValueError):
+        return None
+    calls = sum(r["dispatches"] for n, r in recs.items() if n.split("<")[0] == per_call_kernel)
+    if not calls:
+        return None
+    total = 0.0
+    for k in kernels:
+        found = [r for n, r in recs.items() if n.split("<")[0] == k]
+        if not found:
+            return None
+        total += sum(r["hbm_bytes_per_launch"] * r["dispatches"] for r in found)
+    return int(total / calls)
+
+
+POTRF_KERNELS = ("panel_step_kernel", "gemm_nt_f64_big_kernel", "panel_factor_kernel", "gemm_big_reduce_kernel",
+                 "trtri_blocks_kernel", "potf2_kernel", "panel_update_kernel")
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -619,6 +646,7 @@ def main():
     # roofline of the dominant stage (the Cholesky factorisation of S); the
     # other stages are listed beside it
     roof = roofline_for("potrf", stages["potrf"]["mean_ms"])
+    roof["traffic"] = measured_traffic_per_call(POTRF_KERNELS, "potf2_kernel")
     others = {k: roofline_for(k, stages[k]["mean_ms"]) for k in ("qgt_structured", "metropolis", "local_energy")
               if k in stages}
     # one more eager step with its results kept for the oracle comparison
