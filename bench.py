@@ -551,6 +551,8 @@ def measured_traffic_per_call(kernels, per_call_kernel: str):
 
 POTRF_KERNELS = ("panel_step_kernel", "gemm_nt_f64_big_kernel", "panel_factor_kernel", "gemm_big_reduce_kernel",
                  "trtri_blocks_kernel", "potf2_kernel", "panel_update_kernel")
+QGT_KERNELS = ("qgt_spins_t_kernel", "qgt_spin_pairs_kernel", "qgt_hidden_means_kernel", "qgt_hidden_pairs_kernel",
+               "gemm_exact_a_kernel", "qgt_unpack_kernel")
 
 
 def log(msg):
@@ -649,6 +651,8 @@ def main():
     roof["traffic"] = measured_traffic_per_call(POTRF_KERNELS, "potf2_kernel")
     others = {k: roofline_for(k, stages[k]["mean_ms"]) for k in ("qgt_structured", "metropolis", "local_energy")
               if k in stages}
+    if others.get("qgt_structured"):
+        others["qgt_structured"]["traffic"] = measured_traffic_per_call(QGT_KERNELS, "qgt_spins_t_kernel")
     # one more eager step with its results kept for the oracle comparison
     rec = None if args.no_cpu_baseline else record_step(drv)
     del drv, vs
