@@ -6,10 +6,14 @@ netket/optimizer/qgt/qgt_jacobian_dense.py:29-161, 251-270) for real
 parameters ("real" mode): O_L = (O - <O>) / sqrt(Ns) with FP64 centring,
 S = O_L^T O_L + diag_shift I.
 
-On the GPU O_L is stored parameter-major (X = O_L^T, Np x Ns); for FP32
-models as a bf16 hi/lo split, and S is formed with three BF16 MFMA products
-per tile (FP32 accumulate); FP64 models keep O_L and S in FP64 (FP64 MFMA).
-S is built on demand and not kept after a Cholesky solve consumed it.
+For FP32 RBMs S is formed from the model's structure without a Jacobian: each
+log-derivative is (augmented spin) x (augmented tanh), so O^T O is unpacked
+from the product of the spin-pair and activation-pair tables over the samples
+(nka_rbm_qgt, bf16 matrix cores, FP64 centring).  Other FP32 models store O_L
+parameter-major (X = O_L^T, Np x Ns) as a bf16 hi/lo split and form S with
+three BF16 MFMA products per tile (FP32 accumulate); FP64 models keep O_L and
+S in FP64 (FP64 MFMA).  S is built on demand and not kept after a Cholesky
+solve consumed it.
 """
 
 from __future__ import annotations
@@ -53,14 +57,27 @@ class QGTJacobianDenseT(LinearOperator):
         sig = s.reshape(C * L, N)
         model, flat = vstate.model, vstate.parameters_flat
         t, tT = eng.prepare(model, flat, sig, row_major=True, transposed=True)
-        ones = torch.full((1, self.n_samples), 1.0 / self.n_samples, dtype=torch.float64,
-                          device=sig.device)
-        self.mean = eng.contract(model, flat, sig, t, ones)[0]
-        kind = nat.OUT_BF16X2 if flat.dtype == torch.float32 else nat.OUT_F64
-        self.X = eng.jacobian(model, flat, sig, t, tT, self.mean, eng.inv_sqrt(self.n_samples),
-                              nat.LAYOUT_PS, kind)
-        self.n_params = self.X.rows
+        self.X = None
+        if eng.qgt_structured_available(model, flat):
+            self._structured = (model, flat, sig, tT)
+            self.n_params = model.layout(N)["n_params"]
+        else:
+            self._structured = None
+            ones = torch.full((1, self.n_samples), 1.0 / self.n_samples, dtype=torch.float64,
+                              device=sig.device)
+            self.mean = eng.contract(model, flat, sig, t, ones)[0]
+            kind = nat.OUT_BF16X2 if flat.dtype == torch.float32 else nat.OUT_F64
+            self.X = eng.jacobian(model, flat, sig, t, tT, self.mean, eng.inv_sqrt(self.n_samples),
+                                  nat.LAYOUT_PS, kind)
+            self.n_params = self.X.rows
         self._dense = None
+
+    def _gram(self, shift: float) -> torch.Tensor:
+        """S + shift I on the lower triangle (FP64)."""
+        if self._structured is not None:
+            model, flat, sig, tT = self._structured
+            return eng.qgt_structured(model, flat, sig, tT, 1.0 / self.n_samples, shift)
+        return eng.gram(self.X, shift)
 
     # -- matrices ---------------------------------------------------------------
     def lower(self) -> torch.Tensor:
@@ -68,8 +85,8 @@ class QGTJacobianDenseT(LinearOperator):
         solved: S + diag_shift I, or D^-1 S D^-1 + diag_shift I with a
         diag_scale (which also sets ``self.scale`` = diag(D))."""
         if self._offset is None:
-            return eng.gram(self.X, self.diag_shift)
-        a = eng.gram(self.X, 0.0)
+            return self._gram(self.diag_shift)
+        a = self._gram(0.0)
         self.scale = torch.empty(a.shape[0], dtype=torch.float64, device=a.device)
         _ops.qgt_rescale_(a, self._offset, self.diag_shift, 0, self.scale)
         return a
@@ -81,7 +98,7 @@ class QGTJacobianDenseT(LinearOperator):
             if self._offset is None:
                 a = self.lower()
             else:
-                a = eng.gram(self.X, 0.0)
+                a = self._gram(0.0)
                 self.scale = torch.empty(a.shape[0], dtype=torch.float64, device=a.device)
                 _ops.qgt_rescale_(a, self._offset, self.diag_shift, 1, self.scale)
             _ops.symmetrize_(a)

@@ -88,8 +88,10 @@ def test_qgt_dense(gpu, dtype):
     if dtype == np.float64:
         np.testing.assert_allclose(S, S_ref, rtol=1e-12, atol=1e-14)
     else:
-        # bf16x3: each product keeps ~16 mantissa bits of X (hi + lo), so
-        # |dS_pq| <= c 2^-16 sum_s |X_ps X_qs| <= c 2^-16 sqrt(S_pp S_qq)
+        # FP32 RBM: S from the spin-pair x centred-activation-pair product
+        # (bf16 hi + lo keeps ~16 mantissa bits of each activation pair, the
+        # accumulation is chunked); the same bound as for the bf16x3 Gram of
+        # the Jacobian, |dS_pq| <= c 2^-16 sqrt(S_pp S_qq)
         d = np.sqrt(np.diag(S_ref - 0.01 * np.eye(len(S_ref))))
         bound = 3e-5 * np.outer(d, d) + 1e-9
         err = np.abs(S - S_ref)
