@@ -1142,85 +1142,96 @@ __device__ __forceinline__ void pu_tile(const double* __restrict__ Pa, const dou
       for (int e = 0; e < 4; ++e) out[x][y][e] = vc[x][y][e] - acc[x][y][e];
 }
 
-// Two tiles of C - P P^T with K = 64 and one set of loads: the diagonal tile
-// (rows and columns 0..63, both operands P rows 0..63, which must all be
-// valid rows) and the tile of rows i0.. (P rows i0.. clamped to ra rows,
-// against P rows 0..63).  Both share the B operand, so the loads of the two
-// tiles are in flight together and one staging pass feeds both products.
-// dg and ow are laid out as pu_tile's out; the sums run in pu_tile's order,
-// so the results equal two pu_tile calls.  All threads of the workgroup call it.
-__device__ __forceinline__ void pu_tile_pair(const double* __restrict__ P, const double* __restrict__ C,
-                                             int64_t lda, int64_t i0, int64_t ra, double* sA, double* sB,
-                                             double dg[2][2][4], double ow[2][2][4]) {
-  const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
-  const int wm = (wid >> 1) * 32, wn = (wid & 1) * 32;
-  double va[16], vb[16], cd[2][2][4], co[2][2][4];
+// The previous step's rank-64 update of the diagonal block, C - P P^T with
+// K = 64 (P rows 0..63, which must all be valid rows), on its ten lower 16x16
+// blocks only: block (bi, bj), bj <= bi, is number bi(bi+1)/2 + bj, and
+// wavefront w takes blocks [b_lo, b_hi) = [0,3), [3,6), [6,8), [8,10) so that
+// no wavefront runs more than 48 MFMAs.  P's rows are staged in Pd (64 rows,
+// stride TS_LD); the result goes to Ls with zeros above the diagonal.  The
+// sums run in ascending K steps of 4, the order of pu_tile.  All threads of
+// the workgroup call it; it ends with a barrier.
+__device__ __forceinline__ void diag_lower_update(const double* __restrict__ P, const double* __restrict__ C,
+                                                  int64_t lda, double* Pd, double* Ls) {
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int b_lo = (w == 0) ? 0 : (w == 1) ? 3 : (w == 2) ? 6 : 8;
+  const int b_hi = (w == 0) ? 3 : (w == 1) ? 6 : (w == 2) ? 8 : 10;
+  double vb[16], cv[3][4];
 #pragma unroll
-  for (int it = 0; it < 16; ++it) {
-    const int r = wid + 4 * it;
-    va[it] = P[min<int64_t>(i0 + r, ra - 1) * lda + lane];
-    vb[it] = P[(int64_t)r * lda + lane];
-  }
+  for (int it = 0; it < 16; ++it) vb[it] = P[(int64_t)(w + 4 * it) * lda + lane];
 #pragma unroll
-  for (int x = 0; x < 2; ++x)
+  for (int t = 0; t < 3; ++t) {
+    const int b = b_lo + t;
+    if (b < b_hi) {
+      const int bi = (b >= 6) ? 3 : (b >= 3) ? 2 : (b >= 1) ? 1 : 0;
+      const int bj = b - bi * (bi + 1) / 2;
 #pragma unroll
-    for (int y = 0; y < 2; ++y)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int rr = wm + x * 16 + (lane >> 4) + 4 * e, cc = wn + y * 16 + (lane & 15);
-        cd[x][y][e] = C[(int64_t)rr * lda + cc];
-        co[x][y][e] = C[min<int64_t>(i0 + rr, ra - 1) * lda + cc];
-      }
-  doublex4 ad[2][2], ao[2][2];
-#pragma unroll
-  for (int x = 0; x < 2; ++x)
-#pragma unroll
-    for (int y = 0; y < 2; ++y) {
-      ad[x][y] = (doublex4){0.0, 0.0, 0.0, 0.0};
-      ao[x][y] = (doublex4){0.0, 0.0, 0.0, 0.0};
-    }
-  __syncthreads();  // the staging buffers may still be read by earlier code
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    if (h) __syncthreads();
-    if ((lane >> 5) == h) {
-#pragma unroll
-      for (int it = 0; it < 16; ++it) {
-        const int r = wid + 4 * it;
-        sA[r * PU_LD + (lane & 31)] = va[it];
-        sB[r * PU_LD + (lane & 31)] = vb[it];
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int ks = 0; ks < 32; ks += 4) {
-      const int kq = ks + (lane >> 4);
-      double avd[2], avo[2], bv[2];
-#pragma unroll
-      for (int x = 0; x < 2; ++x) {
-        avd[x] = sB[(wm + x * 16 + (lane & 15)) * PU_LD + kq];
-        avo[x] = sA[(wm + x * 16 + (lane & 15)) * PU_LD + kq];
-      }
-#pragma unroll
-      for (int y = 0; y < 2; ++y) bv[y] = sB[(wn + y * 16 + (lane & 15)) * PU_LD + kq];
-#pragma unroll
-      for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int y = 0; y < 2; ++y) {
-          ad[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(avd[x], bv[y], ad[x][y], 0, 0, 0);
-          ao[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(avo[x], bv[y], ao[x][y], 0, 0, 0);
-        }
+      for (int e = 0; e < 4; ++e)
+        cv[t][e] = C[(int64_t)(16 * bi + (lane >> 4) + 4 * e) * lda + 16 * bj + (lane & 15)];
     }
   }
 #pragma unroll
-  for (int x = 0; x < 2; ++x)
+  for (int it = 0; it < 16; ++it) Pd[(w + 4 * it) * TS_LD + lane] = vb[it];
+  // rows 16w..16w+15 right of their diagonal sub-block hold zeros
+  if (lane >= 16 * (w + 1)) {
 #pragma unroll
-    for (int y = 0; y < 2; ++y)
+    for (int r = 0; r < 16; ++r) Ls[(16 * w + r) * TS_LD + lane] = 0.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    const int b = b_lo + t;
+    if (b < b_hi) {
+      const int bi = (b >= 6) ? 3 : (b >= 3) ? 2 : (b >= 1) ? 1 : 0;
+      const int bj = b - bi * (bi + 1) / 2;
+      doublex4 acc = (doublex4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kb = 0; kb < CH_NB; kb += 4) {
+        const int kk = kb + (lane >> 4);
+        const double av = Pd[(16 * bi + (lane & 15)) * TS_LD + kk];
+        const double bv = Pd[(16 * bj + (lane & 15)) * TS_LD + kk];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+      }
+      // element e of lane l is (row l/16 + 4e, col l%16) of the block
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        dg[x][y][e] = cd[x][y][e] - ad[x][y][e];
-        ow[x][y][e] = co[x][y][e] - ao[x][y][e];
+        const int R = 16 * bi + (lane >> 4) + 4 * e, c = 16 * bj + (lane & 15);
+        Ls[R * TS_LD + c] = c <= R ? cv[t][e] - acc[e] : 0.0;
       }
+    }
+  }
+  __syncthreads();
+}
+
+// Rows 16s..16s+15 of the tile of rows i0.. of C - P P^T (K = 64; P rows
+// i0.. clamped to ra rows, against P rows 0..63 staged in Pd), for one
+// wavefront: out[y][e] is element ((l/16) + 4e, 16y + l%16) of the 16-row
+// strip, lane l.  The A operand is read from global memory in the MFMA's
+// layout; the sums run in the order of pu_tile.
+__device__ __forceinline__ void own_strip_update(const double* __restrict__ P, const double* __restrict__ C,
+                                                 int64_t lda, int64_t i0, int64_t ra, int s, const double* Pd,
+                                                 double out[4][4]) {
+  const int lane = threadIdx.x & 63;
+  double av[16], cv[4][4];
+  const double* pa = P + min<int64_t>(i0 + 16 * s + (lane & 15), ra - 1) * lda + (lane >> 4);
+#pragma unroll
+  for (int q = 0; q < 16; ++q) av[q] = pa[4 * q];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const double* pc = C + min<int64_t>(i0 + 16 * s + (lane >> 4) + 4 * e, ra - 1) * lda + (lane & 15);
+#pragma unroll
+    for (int y = 0; y < 4; ++y) cv[y][e] = pc[16 * y];
+  }
+#pragma unroll
+  for (int y = 0; y < 4; ++y) {
+    doublex4 acc = (doublex4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const double bv = Pd[(16 * y + (lane & 15)) * TS_LD + 4 * q + (lane >> 4)];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], bv, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) out[y][e] = cv[y][e] - acc[e];
+  }
 }
 
 // C[i][j] -= sum_{k<64} Pa[i][k] Pb[j][k] over a grid of 64x64 tiles, stored
@@ -1260,10 +1271,11 @@ __global__ void __launch_bounds__(256) panel_update_kernel(double* __restrict__ 
 
 // One 64-column step k0 of a panel after its first, in one launch with two
 // roles.  Workgroups [0, n_factor) work as panel_factor_kernel, except that
-// the previous step's rank-64 update of this step's columns is applied here,
-// to the diagonal block and to each workgroup's own 64 rows (the operands
-// and C tiles go through the same tile code as the panel update), before the
-// factorisation.  The other workgroups apply the previous step's update to
+// the previous step's rank-64 update of this step's columns is applied here:
+// to the diagonal block's lower 16x16 blocks by all four wavefronts before
+// the factorisation, and to the workgroup's own 64 rows by wavefronts 1-3
+// while wavefront 0 factors (same sums, in the same order, as the panel
+// update's tiles).  The other workgroups apply the previous step's update to
 // the panel's columns right of this step (panel_update_kernel's tiles, shifted
 // by 64 columns).  The roles write disjoint columns and read only what earlier
 // launches wrote, so the panel update no longer sits between two steps.
@@ -1289,31 +1301,41 @@ __global__ void __launch_bounds__(256, 2) panel_step_kernel(double* __restrict__
     pu_store_tile(P, P + CH_NB * lda, A + k0 * lda + k0 + CH_NB, lda, tile, tiles_n_b, rest_p, cols_b, sA, sB);
     return;
   }
-  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
-  double dg[2][2][4], ow[2][2][4];
-  // diagonal block (rows and columns k0 .. k0+63) and this workgroup's rows
-  // (the launch has rows below this step, so rest_p > 64: P rows 0..63 are valid)
-  const int64_t own0 = CH_NB + (int64_t)blockIdx.x * (4 * PF_ROWS);   // relative to k0
-  pu_tile_pair(P, A + k0 * lda + k0, lda, own0, rest_p, sA, sB, dg, ow);
-  __syncthreads();  // staging done: Ls and Xs take the updated tiles
-#pragma unroll
-  for (int x = 0; x < 2; ++x)
-#pragma unroll
-    for (int y = 0; y < 2; ++y)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int R = wm + x * 16 + (lane >> 4) + 4 * e, c = wn + y * 16 + (lane & 15);
-        Ls[R * TS_LD + c] = c <= R ? dg[x][y][e] : 0.0;
-        Xs[R >> 4][(R & 15) * TS_LD + c] = ow[x][y][e];
-      }
-  __syncthreads();
+  // diagonal block (rows and columns k0 .. k0+63) first: the launch has rows
+  // below this step, so rest_p > 64 and P rows 0..63 are valid.  P rows
+  // 0..63 are staged in the Xs region, which the factorisation does not use.
+  double* Pd = &Xs[0][0];
+  const double* Ck = A + k0 * lda + k0;
+  diag_lower_update(P, Ck, lda, Pd, Ls);
   double* X = Xs[w];
-  const int64_t r0 = k0 + CH_NB + (int64_t)blockIdx.x * (4 * PF_ROWS) + PF_ROWS * w;
+  const int64_t own0 = CH_NB + (int64_t)blockIdx.x * (4 * PF_ROWS);   // relative to k0
+  const int64_t r0 = k0 + own0 + PF_ROWS * w;
+  // wavefront 0 factors the diagonal block while wavefronts 1-3 update this
+  // workgroup's 16-row strips (w - 1, and 3 for wavefront 1) in registers
+  double ow[2][4][4];
   if (w == 0) {
     const int bad = factor_block_wave(Ls, colbuf, rdiag, lane, k0);
     if (blockIdx.x == 0 && lane == 0 && bad) atomicCAS(info, 0, bad);
+  } else {
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int s = w - 1 + 3 * t;
+      if (s < 4) own_strip_update(P, Ck, lda, own0, rest_p, s, Pd, ow[t]);
+    }
   }
-  __syncthreads();
+  __syncthreads();  // the factor is done and Pd is no longer read
+  if (w != 0) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int s = w - 1 + 3 * t;
+      if (s < 4) {
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) Xs[s][((lane >> 4) + 4 * e) * TS_LD + 16 * y + (lane & 15)] = ow[t][y][e];
+      }
+    }
+  }
   if (blockIdx.x == 0) {
 #pragma unroll
     for (int r = 0; r < PF_ROWS; ++r) {
