@@ -794,12 +794,13 @@ __device__ inline double readlane_f64(double v, int src) {
 }
 
 // 1/sqrt(a) from the hardware estimate (about 2^-23 relative error) refined by
-// two Newton-Raphson steps y <- y (3 - a y^2) / 2, each of which squares the
-// relative error.
+// two Newton-Raphson steps y <- y + y (1/2 - (a/2) y^2), each of which squares
+// the relative error.
 __device__ inline double rsqrt_refined(double a) {
+  const double h = 0.5 * a;
   double y = __builtin_amdgcn_rsq(a);
-  y = y * (1.5 - 0.5 * a * y * y);
-  y = y * (1.5 - 0.5 * a * y * y);
+  y = fma(y, fma(-(h * y), y, 0.5), y);
+  y = fma(y, fma(-(h * y), y, 0.5), y);
   return y;
 }
 
@@ -850,9 +851,15 @@ __device__ __forceinline__ int factor_block_wave(double* Ab, double* colbuf, dou
       // the next pivot: lane jj + 1 forms x[j+1] - (x[j] y^2) x[j] below with
       // these same operations (one rounding for the multiply-subtract), so
       // the value is the one that lane holds after this step
-      if (j < 15) ajj_next = fma(-(a1 * (y * y)), a1, b1);
-      const double sij = (lane > jj) ? x[j] * (y * y) : 0.0;
-      x[j] = (lane > jj) ? x[j] * y : (lane == jj ? piv * y : 0.0);
+      const double yy = y * y;
+      if (j < 15) ajj_next = fma(-(a1 * yy), a1, b1);
+      // every lane scales and updates: in rows jj and above the entries
+      // changed lie above the diagonal, which no later pivot or column
+      // broadcast reads and the write-back below replaces by zeros; row jj
+      // holds a_jj itself (the value the pivot was formed from)
+      const double sij = x[j] * yy;
+      x[j] *= y;
+      if (!ok && lane == jj) x[j] = piv * y;
       if (lane == jj) yv[jj] = y;
 #pragma unroll
       for (int q = j + 1; q < 16; ++q) x[q] -= sij * cj[q];
