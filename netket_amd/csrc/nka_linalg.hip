@@ -784,6 +784,7 @@ constexpr int CH_NB = 64;
 
 constexpr int CH_LT = CH_NB + 2;  // LDS row stride of the transposed factor (16-byte rows)
 constexpr int TS_LD = CH_NB + 1;  // LDS row stride of a 64x64 block (doubles)
+constexpr int PF_ROWS = 16;  // rows of the panel per wavefront
 
 // Lane `src`'s value of v in every lane (src a compile-time constant after
 // unrolling).
@@ -815,11 +816,13 @@ __device__ inline double rsqrt_refined(double a) {
 //  * the block's remaining lower part (rows and columns >= 16(p+1)) then
 //    receives the rank-16 update on FP64 MFMA (v_mfma_f64_16x16x4).
 // On return Ab holds L (exact zeros above the diagonal) and yv[j] = 1/L_jj.
+// With `done` (LDS, zeroed beforehand), done[p] is set with a workgroup-scope
+// release once columns 16p..16p+15 of L and their yv entries are final.
 // A non-positive or non-finite pivot is replaced by 1 so the factor
 // completes; the return value is then the first failing column + 1 + k0
 // (0 when every pivot is positive).
 __device__ __forceinline__ int factor_block_wave(double* Ab, double* colbuf, double* yv, int lane,
-                                                 int64_t k0) {
+                                                 int64_t k0, int* done = nullptr) {
   int bad = 0;
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
@@ -868,6 +871,9 @@ __device__ __forceinline__ int factor_block_wave(double* Ab, double* colbuf, dou
 #pragma unroll
     for (int q = 0; q < 16; ++q) Ab[lane * TS_LD + 16 * p + q] = (16 * p + q <= lane) ? x[q] : 0.0;
     wave_lds_sync();
+    // columns 16p..16p+15 of L and yv[16p..16p+15] are final from here on
+    // (the update below writes only later columns)
+    if (done != nullptr) __hip_atomic_store(&done[p], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     // rank-16 update of the lower blocks (bi, bj), p < bj <= bi < 4
 #pragma unroll
     for (int bi = p + 1; bi < 4; ++bi)
@@ -918,43 +924,84 @@ __device__ __forceinline__ void invert_diag16_wave(const double* Ls, const doubl
   }
 }
 
-// Solve 16 rows of X (LDS, row-major) in place, X <- X L^{-T}, with one
-// wavefront by 16-column blocks on FP64 MFMA (v_mfma_f64_16x16x4, one 16x16
-// accumulator): block p first receives X_p -= X_{<p} L_{p,<p}^T, then
-// X_p <- X_p (L_pp^{-1})^T with the inverse of the diagonal sub-block
-// (Mi + p * MI_SZ, from invert_diag16_wave).  Ls = L (row-major, LDS).
-__device__ __forceinline__ void solve_rows16_wave(const double* Ls, const double* Mi, double* X, int lane) {
+// Block p of the solve of 16 rows of X (LDS, row-major) in place,
+// X <- X L^{-T}, with one wavefront on FP64 MFMA (v_mfma_f64_16x16x4, one
+// 16x16 accumulator): X_p -= X_{<p} L_{p,<p}^T, then X_p <- X_p (L_pp^{-1})^T
+// with the inverse of the diagonal sub-block (Mi + p * MI_SZ, from
+// invert_diag16_wave).  Ls = L (row-major, LDS).  Blocks must be applied in
+// order p = 0..3; columns 16p.. of L's rows 16p..16p+15 must be final.
+__device__ __forceinline__ void solve_block16_wave(const double* Ls, const double* Mi, double* X, int p,
+                                                   int lane) {
   const int row = lane & 15;
-#pragma unroll
-  for (int p = 0; p < 4; ++p) {
-    if (p > 0) {
-      // X[:, 16p..16p+16) -= X[:, 0..16p) L[16p..16p+16, 0..16p)^T
-      doublex4 acc = (doublex4){0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int kb = 0; kb < 16 * p; kb += 4) {
-        const int kk = kb + (lane >> 4);
-        const double av = X[row * TS_LD + kk];
-        const double bv = Ls[(16 * p + row) * TS_LD + kk];
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
-      }
-      // element e of lane l is (row l/16 + 4e, col l%16)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) X[((lane >> 4) + 4 * e) * TS_LD + 16 * p + row] -= acc[e];
-      wave_lds_sync();
-    }
-    // X_p <- X_p Mi_p^T: operand B[k][n] = Mi_p[n][k]
+  if (p > 0) {
+    // X[:, 16p..16p+16) -= X[:, 0..16p) L[16p..16p+16, 0..16p)^T
     doublex4 acc = (doublex4){0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int kb = 0; kb < 16; kb += 4) {
+    for (int kb = 0; kb < 16 * p; kb += 4) {
       const int kk = kb + (lane >> 4);
-      const double av = X[row * TS_LD + 16 * p + kk];
-      const double bv = Mi[p * MI_SZ + row * MI_LD + kk];
+      const double av = X[row * TS_LD + kk];
+      const double bv = Ls[(16 * p + row) * TS_LD + kk];
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
     }
-    wave_lds_sync();  // every lane's operand reads of X_p precede the writes
+    // element e of lane l is (row l/16 + 4e, col l%16)
 #pragma unroll
-    for (int e = 0; e < 4; ++e) X[((lane >> 4) + 4 * e) * TS_LD + 16 * p + row] = acc[e];
+    for (int e = 0; e < 4; ++e) X[((lane >> 4) + 4 * e) * TS_LD + 16 * p + row] -= acc[e];
     wave_lds_sync();
+  }
+  // X_p <- X_p Mi_p^T: operand B[k][n] = Mi_p[n][k]
+  doublex4 acc = (doublex4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int kb = 0; kb < 16; kb += 4) {
+    const int kk = kb + (lane >> 4);
+    const double av = X[row * TS_LD + 16 * p + kk];
+    const double bv = Mi[p * MI_SZ + row * MI_LD + kk];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+  }
+  wave_lds_sync();  // every lane's operand reads of X_p precede the writes
+#pragma unroll
+  for (int e = 0; e < 4; ++e) X[((lane >> 4) + 4 * e) * TS_LD + 16 * p + row] = acc[e];
+  wave_lds_sync();
+}
+
+// Bounded wait of a wavefront for an LDS flag of its own workgroup set by
+// another wavefront (workgroup-scope acquire).  The setter always runs to
+// its publication, so the wait ends; past the bound the factorisation is
+// reported as failed at this step (*info = k0 + 1) and the wavefront goes on.
+constexpr int64_t kLdsSpinLimit = 1LL << 24;
+__device__ inline void lds_wait(int* flag, int32_t* info, int64_t k0) {
+  int64_t spins = 0;
+  while (__builtin_amdgcn_readfirstlane(
+             __hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) == 0) {
+    __builtin_amdgcn_s_sleep(1);
+    if (++spins > kLdsSpinLimit) {
+      if ((threadIdx.x & 63) == 0) atomicCAS(info, 0, (int32_t)(k0 + 1));
+      break;
+    }
+  }
+}
+
+// Wavefronts 1-3 while wavefront 0 factors the diagonal block with
+// done = pflags[0..3]: wavefront p+1 inverts sub-block p (p < 3) of L11 once
+// it is published and announces the inverse in pflags[4+p]; every one of them
+// then applies block p of the solve to its 16-row strips (w - 1, and 3 for
+// wavefront 1).  Block 3 follows once the factor is done; the operations and
+// their order are those of applying blocks 0..3 one after another.
+__device__ __forceinline__ void solve_during_factor(const double* Ls, const double* rdiag, double* Mi,
+                                                    double (*Xs)[PF_ROWS * TS_LD], int* pflags, int w,
+                                                    int lane, int32_t* info, int64_t k0) {
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    if (w == p + 1) {
+      lds_wait(&pflags[p], info, k0);
+      invert_diag16_wave(Ls, rdiag, Mi + p * MI_SZ, p, lane);
+      __hip_atomic_store(&pflags[4 + p], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    lds_wait(&pflags[4 + p], info, k0);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int s = w - 1 + 3 * t;
+      if (s < 4) solve_block16_wave(Ls, Mi, Xs[s], p, lane);
+    }
   }
 }
 
@@ -1010,7 +1057,6 @@ __global__ void __launch_bounds__(64) potf2_kernel(double* __restrict__ A, int64
 // kernel at the end, since a workgroup that starts late still reads the
 // unfactored block from A.  All global loads of a workgroup are issued before
 // its first LDS store.  Rows past n re-read the last row and are not stored.
-constexpr int PF_ROWS = 16;  // rows of the panel per wavefront
 
 __global__ void __launch_bounds__(256) panel_factor_kernel(double* __restrict__ A, int64_t lda, int64_t n,
                                                            int64_t k0, double* __restrict__ slot,
@@ -1021,8 +1067,10 @@ __global__ void __launch_bounds__(256) panel_factor_kernel(double* __restrict__ 
   __shared__ double rdiag[CH_NB];
   __shared__ double Xs[4][PF_ROWS * TS_LD];      // 16 rows per wavefront
   __shared__ double Mi[4 * MI_SZ];               // inverses of L11's diagonal 16x16 blocks
+  __shared__ int pflags[8];                      // sub-panels factored, inverses ready
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   double* X = Xs[w];
+  if (tid < 8) pflags[tid] = 0;                  // ordered by the barrier below
   const int64_t r0 = k0 + CH_NB + (int64_t)blockIdx.x * (4 * PF_ROWS) + PF_ROWS * w;
   {
     // wavefront w reads rows 16w..16w+15 of the diagonal block and its own
@@ -1042,9 +1090,13 @@ __global__ void __launch_bounds__(256) panel_factor_kernel(double* __restrict__ 
     for (int r = 0; r < PF_ROWS; ++r) X[r * TS_LD + lane] = v[r];
   }
   __syncthreads();
+  // wavefront 0 factors the diagonal block while wavefronts 1-3 solve blocks
+  // 0..2 of the four strips as its sub-panels are published
   if (w == 0) {
-    const int bad = factor_block_wave(Ls, colbuf, rdiag, lane, k0);
+    const int bad = factor_block_wave(Ls, colbuf, rdiag, lane, k0, pflags);
     if (blockIdx.x == 0 && lane == 0 && bad) atomicCAS(info, 0, bad);
+  } else {
+    solve_during_factor(Ls, rdiag, Mi, Xs, pflags, w, lane, info, k0);
   }
   __syncthreads();
   if (blockIdx.x == 0) {
@@ -1057,11 +1109,10 @@ __global__ void __launch_bounds__(256) panel_factor_kernel(double* __restrict__ 
       slot[R * CH_NB + lane] = Ls[R * TS_LD + lane];
     }
   }
-  // wavefront w inverts diagonal sub-block w of L11; then each wavefront
-  // solves its rows with all four
-  invert_diag16_wave(Ls, rdiag, Mi + w * MI_SZ, w, lane);
+  // the last diagonal sub-block's inverse, then block 3 of each strip
+  if (w == 0) invert_diag16_wave(Ls, rdiag, Mi + 3 * MI_SZ, 3, lane);
   __syncthreads();
-  solve_rows16_wave(Ls, Mi, X, lane);
+  solve_block16_wave(Ls, Mi, X, 3, lane);
 #pragma unroll
   for (int r = 0; r < PF_ROWS; ++r) {
     if (r0 + r < n) A[(r0 + r) * lda + k0 + lane] = X[r * TS_LD + lane];
@@ -1296,6 +1347,7 @@ __global__ void __launch_bounds__(256, 2) panel_step_kernel(double* __restrict__
   __shared__ double rdiag[CH_NB];
   __shared__ double Xs[4][PF_ROWS * TS_LD];      // 16 rows per wavefront
   __shared__ double Mi[4 * MI_SZ];               // inverses of L11's diagonal 16x16 blocks
+  __shared__ int pflags[8];                      // sub-panels factored, inverses ready, strips stored
   double* sA = Ls;                               // staging of the update tiles (before Ls is filled)
   double* sB = &Xs[0][0];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
@@ -1313,25 +1365,35 @@ __global__ void __launch_bounds__(256, 2) panel_step_kernel(double* __restrict__
   // 0..63 are staged in the Xs region, which the factorisation does not use.
   double* Pd = &Xs[0][0];
   const double* Ck = A + k0 * lda + k0;
+  if (tid < 8) pflags[tid] = 0;                  // ordered by the barriers below
   diag_lower_update(P, Ck, lda, Pd, Ls);
   double* X = Xs[w];
   const int64_t own0 = CH_NB + (int64_t)blockIdx.x * (4 * PF_ROWS);   // relative to k0
   const int64_t r0 = k0 + own0 + PF_ROWS * w;
-  // wavefront 0 factors the diagonal block while wavefronts 1-3 update this
-  // workgroup's 16-row strips (w - 1, and 3 for wavefront 1) in registers
-  double ow[2][4][4];
+  // wavefront 0 factors the diagonal block; meanwhile wavefronts 1-3 update
+  // this workgroup's 16-row strips (w - 1, and 3 for wavefront 1) in
+  // registers, store them once none of them reads Pd any more, and solve
+  // their blocks 0..2 as the factor publishes its sub-panels
   if (w == 0) {
-    const int bad = factor_block_wave(Ls, colbuf, rdiag, lane, k0);
+    const int bad = factor_block_wave(Ls, colbuf, rdiag, lane, k0, pflags);
     if (blockIdx.x == 0 && lane == 0 && bad) atomicCAS(info, 0, bad);
   } else {
+    double ow[2][4][4];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
       const int s = w - 1 + 3 * t;
       if (s < 4) own_strip_update(P, Ck, lda, own0, rest_p, s, Pd, ow[t]);
     }
-  }
-  __syncthreads();  // the factor is done and Pd is no longer read
-  if (w != 0) {
+    if (lane == 0) __hip_atomic_fetch_add(&pflags[7], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    int64_t spins = 0;
+    while (__builtin_amdgcn_readfirstlane(
+               __hip_atomic_load(&pflags[7], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) < 3) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > kLdsSpinLimit) {
+        if (lane == 0) atomicCAS(info, 0, (int32_t)(k0 + 1));
+        break;
+      }
+    }
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
       const int s = w - 1 + 3 * t;
@@ -1342,7 +1404,10 @@ __global__ void __launch_bounds__(256, 2) panel_step_kernel(double* __restrict__
           for (int e = 0; e < 4; ++e) Xs[s][((lane >> 4) + 4 * e) * TS_LD + 16 * y + (lane & 15)] = ow[t][y][e];
       }
     }
+    wave_lds_sync();
+    solve_during_factor(Ls, rdiag, Mi, Xs, pflags, w, lane, info, k0);
   }
+  __syncthreads();  // the factor and blocks 0..2 of every strip are done
   if (blockIdx.x == 0) {
 #pragma unroll
     for (int r = 0; r < PF_ROWS; ++r) {
@@ -1350,11 +1415,9 @@ __global__ void __launch_bounds__(256, 2) panel_step_kernel(double* __restrict__
       slot[R * CH_NB + lane] = Ls[R * TS_LD + lane];
     }
   }
-  // wavefront w inverts diagonal sub-block w of L11; then each wavefront
-  // solves its rows with all four
-  invert_diag16_wave(Ls, rdiag, Mi + w * MI_SZ, w, lane);
+  if (w == 0) invert_diag16_wave(Ls, rdiag, Mi + 3 * MI_SZ, 3, lane);
   __syncthreads();
-  solve_rows16_wave(Ls, Mi, X, lane);
+  solve_block16_wave(Ls, Mi, X, 3, lane);
 #pragma unroll
   for (int r = 0; r < PF_ROWS; ++r) {
     if (r0 + r < n) A[(r0 + r) * lda + k0 + lane] = X[r * TS_LD + lane];
