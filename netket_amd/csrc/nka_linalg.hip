@@ -2043,8 +2043,11 @@ static void potrf_run(double* a, int64_t n, int64_t lda, void* workspace, double
                       int32_t* info, hipStream_t s) {
   double* inv = ws_inv(workspace);
   const int64_t nblk = (n + CH_NB - 1) / CH_NB;
-  // panel widths measured best on MI355X: n = 8384 -> (512, 512), n = 32768 -> (4096, 512)
-  const int64_t NB0 = n >= 16384 ? 4096 : 512, NB1 = 512;
+  // panel widths measured best on MI355X: (768, 768) for n = 4096, 8384 and
+  // 12288 (against 512-wide panels 2.02 / 6.64 / 15.1 ms against 2.06 / 6.94 /
+  // 15.8 ms), (4096, 512) from n = 16384 (30.8 ms at 16384, 200 ms at 32768)
+  const bool large = n >= 16384;
+  const int64_t NB0 = large ? 4096 : 768, NB1 = large ? 512 : 768;
   for (int64_t K0 = 0; K0 < n; K0 += NB0) {
     const int64_t n0 = std::min<int64_t>(NB0, n - K0);
     for (int64_t K1 = K0; K1 < K0 + n0; K1 += NB1) {
@@ -2082,8 +2085,8 @@ static void potrf_run(double* a, int64_t n, int64_t lda, void* workspace, double
                              k0, (int64_t)grid, cols_b, tn, slot, info);
         }
       }
-      // update of the remaining columns of this 1024-column panel (K = 256):
-      // the lower triangle of its square top, then the rows below it
+      // update of the outer panel's columns right of this NB1-column panel
+      // (K = NB1): the lower triangle of its square top, then the rows below it
       const int64_t cols1 = K0 + n0 - (K1 + n1);
       if (cols1 > 0) {
         const int64_t rows1 = n - (K1 + n1);
@@ -2095,7 +2098,7 @@ static void potrf_run(double* a, int64_t n, int64_t lda, void* workspace, double
                           1.0, 0, info, s, split, n_split);
       }
     }
-    // trailing update (K = 1024)
+    // trailing update (K = NB0)
     const int64_t rest2 = n - K0 - n0;
     if (rest2 > 0) {
       const double* l21 = a + (K0 + n0) * lda + K0;
