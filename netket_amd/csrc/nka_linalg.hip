@@ -805,6 +805,30 @@ __device__ inline double rsqrt_refined(double a) {
   return y;
 }
 
+// Bounded wait of a wavefront for an LDS flag of its own workgroup set by
+// another wavefront (workgroup-scope acquire; every lane of the setter stores
+// the flag, so each lane's earlier LDS accesses are released).  The setter
+// always runs to its publication, so the wait ends; past the bound the
+// factorisation is reported as failed at this step (*info = k0 + 1) and the
+// wavefront goes on.
+constexpr int64_t kLdsSpinLimit = 1LL << 24;
+__device__ inline void lds_wait(const int* flag, int32_t* info, int64_t k0) {
+  int64_t spins = 0;
+  while (__builtin_amdgcn_readfirstlane(__hip_atomic_load(const_cast<int*>(flag), __ATOMIC_ACQUIRE,
+                                                          __HIP_MEMORY_SCOPE_WORKGROUP)) == 0) {
+    __builtin_amdgcn_s_sleep(1);
+    if (++spins > kLdsSpinLimit) {
+      if ((threadIdx.x & 63) == 0) atomicCAS(info, 0, (int32_t)(k0 + 1));
+      break;
+    }
+  }
+}
+
+// Publish an LDS flag of the workgroup (every lane stores it).
+__device__ inline void lds_publish(int* flag) {
+  __hip_atomic_store(flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Factor a 64x64 block held in LDS (Ab, row-major, zeros above the diagonal)
 // in place with one wavefront, lane i = row i, by 16-column sub-panels:
 //  * sub-panel p (columns 16p..16p+15, rows 16p..63) is factored
@@ -818,11 +842,15 @@ __device__ inline double rsqrt_refined(double a) {
 // On return Ab holds L (exact zeros above the diagonal) and yv[j] = 1/L_jj.
 // With `done` (LDS, zeroed beforehand), done[p] is set with a workgroup-scope
 // release once columns 16p..16p+15 of L and their yv entries are final.
+// With `ready`, the blocks right of the first sub-panel are written by other
+// wavefronts during its factorisation: ready[0..2] are awaited before they
+// are first touched (the rank-16 update after sub-panel 0).
 // A non-positive or non-finite pivot is replaced by 1 so the factor
 // completes; the return value is then the first failing column + 1 + k0
 // (0 when every pivot is positive).
 __device__ __forceinline__ int factor_block_wave(double* Ab, double* colbuf, double* yv, int lane,
-                                                 int64_t k0, int* done = nullptr) {
+                                                 int64_t k0, int* done = nullptr,
+                                                 const int* ready = nullptr, int32_t* info = nullptr) {
   int bad = 0;
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
@@ -873,7 +901,11 @@ __device__ __forceinline__ int factor_block_wave(double* Ab, double* colbuf, dou
     wave_lds_sync();
     // columns 16p..16p+15 of L and yv[16p..16p+15] are final from here on
     // (the update below writes only later columns)
-    if (done != nullptr) __hip_atomic_store(&done[p], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (done != nullptr) lds_publish(&done[p]);
+    if (p == 0 && ready != nullptr) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) lds_wait(&ready[i], info, k0);
+    }
     // rank-16 update of the lower blocks (bi, bj), p < bj <= bi < 4
 #pragma unroll
     for (int bi = p + 1; bi < 4; ++bi)
@@ -963,23 +995,6 @@ __device__ __forceinline__ void solve_block16_wave(const double* Ls, const doubl
   wave_lds_sync();
 }
 
-// Bounded wait of a wavefront for an LDS flag of its own workgroup set by
-// another wavefront (workgroup-scope acquire).  The setter always runs to
-// its publication, so the wait ends; past the bound the factorisation is
-// reported as failed at this step (*info = k0 + 1) and the wavefront goes on.
-constexpr int64_t kLdsSpinLimit = 1LL << 24;
-__device__ inline void lds_wait(int* flag, int32_t* info, int64_t k0) {
-  int64_t spins = 0;
-  while (__builtin_amdgcn_readfirstlane(
-             __hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) == 0) {
-    __builtin_amdgcn_s_sleep(1);
-    if (++spins > kLdsSpinLimit) {
-      if ((threadIdx.x & 63) == 0) atomicCAS(info, 0, (int32_t)(k0 + 1));
-      break;
-    }
-  }
-}
-
 // Wavefronts 1-3 while wavefront 0 factors the diagonal block with
 // done = pflags[0..3]: wavefront p+1 inverts sub-block p (p < 3) of L11 once
 // it is published and announces the inverse in pflags[4+p]; every one of them
@@ -994,7 +1009,7 @@ __device__ __forceinline__ void solve_during_factor(const double* Ls, const doub
     if (w == p + 1) {
       lds_wait(&pflags[p], info, k0);
       invert_diag16_wave(Ls, rdiag, Mi + p * MI_SZ, p, lane);
-      __hip_atomic_store(&pflags[4 + p], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      lds_publish(&pflags[4 + p]);
     }
     lds_wait(&pflags[4 + p], info, k0);
 #pragma unroll
@@ -1200,31 +1215,56 @@ __device__ __forceinline__ void pu_tile(const double* __restrict__ Pa, const dou
       for (int e = 0; e < 4; ++e) out[x][y][e] = vc[x][y][e] - acc[x][y][e];
 }
 
-// The previous step's rank-64 update of the diagonal block, C - P P^T with
-// K = 64 (P rows 0..63, which must all be valid rows), on its ten lower 16x16
-// blocks only: block (bi, bj), bj <= bi, is number bi(bi+1)/2 + bj, and
-// wavefront w takes blocks [b_lo, b_hi) = [0,3), [3,6), [6,8), [8,10) so that
-// no wavefront runs more than 48 MFMAs.  P's rows are staged in Pd (64 rows,
-// stride TS_LD); the result goes to Ls with zeros above the diagonal.  The
-// sums run in ascending K steps of 4, the order of pu_tile.  All threads of
-// the workgroup call it; it ends with a barrier.
-__device__ __forceinline__ void diag_lower_update(const double* __restrict__ P, const double* __restrict__ C,
-                                                  int64_t lda, double* Pd, double* Ls) {
+// Block (bi, bj) of the previous step's rank-64 update of the diagonal block,
+// C - P P^T with K = 64, from P's rows staged in Pd (64 rows, stride TS_LD)
+// and the block's C values cv in the accumulator layout; written to Ls with
+// zeros above the diagonal.  The sums run in ascending K steps of 4, the
+// order of pu_tile.  One wavefront.
+__device__ __forceinline__ void diag_block_update(const double* Pd, double* Ls, int bi, int bj,
+                                                  const double cv[4], int lane) {
+  doublex4 acc = (doublex4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int kb = 0; kb < CH_NB; kb += 4) {
+    const int kk = kb + (lane >> 4);
+    const double av = Pd[(16 * bi + (lane & 15)) * TS_LD + kk];
+    const double bv = Pd[(16 * bj + (lane & 15)) * TS_LD + kk];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+  }
+  // element e of lane l is (row l/16 + 4e, col l%16) of the block
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int R = 16 * bi + (lane >> 4) + 4 * e, c = 16 * bj + (lane & 15);
+    Ls[R * TS_LD + c] = c <= R ? cv[e] - acc[e] : 0.0;
+  }
+}
+
+// The lower blocks of the diagonal block right of its first 16 columns,
+// (1,1) (2,1) | (2,2) (3,1) | (3,2) (3,3), two for each of wavefronts 1-3.
+__device__ inline int rest_bi(int w, int t) { return w == 1 ? 1 + t : (w == 2 ? 2 + t : 3); }
+__device__ inline int rest_bj(int w, int t) { return w == 1 ? 1 : (w == 2 ? 2 - t : 2 + t); }
+
+// The previous step's update of the diagonal block (P rows 0..63, which must
+// all be valid rows) for its first 16 columns, which the factorisation needs
+// first: wavefront w updates block (w, 0).  P's rows are staged in Pd, the
+// zeros above the diagonal are written, and wavefronts 1-3 also load the C
+// values of their two further blocks into cr (applied by diag_block_update
+// during the factorisation of the first sub-panel).  All threads of the
+// workgroup call it; it ends with a barrier.
+__device__ __forceinline__ void diag_col0_update(const double* __restrict__ P, const double* __restrict__ C,
+                                                 int64_t lda, double* Pd, double* Ls, double cr[2][4]) {
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-  const int b_lo = (w == 0) ? 0 : (w == 1) ? 3 : (w == 2) ? 6 : 8;
-  const int b_hi = (w == 0) ? 3 : (w == 1) ? 6 : (w == 2) ? 8 : 10;
-  double vb[16], cv[3][4];
+  double vb[16], c0[4];
 #pragma unroll
   for (int it = 0; it < 16; ++it) vb[it] = P[(int64_t)(w + 4 * it) * lda + lane];
 #pragma unroll
-  for (int t = 0; t < 3; ++t) {
-    const int b = b_lo + t;
-    if (b < b_hi) {
-      const int bi = (b >= 6) ? 3 : (b >= 3) ? 2 : (b >= 1) ? 1 : 0;
-      const int bj = b - bi * (bi + 1) / 2;
+  for (int e = 0; e < 4; ++e) c0[e] = C[(int64_t)(16 * w + (lane >> 4) + 4 * e) * lda + (lane & 15)];
+  if (w > 0) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int bi = rest_bi(w, t), bj = rest_bj(w, t);
 #pragma unroll
       for (int e = 0; e < 4; ++e)
-        cv[t][e] = C[(int64_t)(16 * bi + (lane >> 4) + 4 * e) * lda + 16 * bj + (lane & 15)];
+        cr[t][e] = C[(int64_t)(16 * bi + (lane >> 4) + 4 * e) * lda + 16 * bj + (lane & 15)];
     }
   }
 #pragma unroll
@@ -1235,28 +1275,7 @@ __device__ __forceinline__ void diag_lower_update(const double* __restrict__ P, 
     for (int r = 0; r < 16; ++r) Ls[(16 * w + r) * TS_LD + lane] = 0.0;
   }
   __syncthreads();
-#pragma unroll
-  for (int t = 0; t < 3; ++t) {
-    const int b = b_lo + t;
-    if (b < b_hi) {
-      const int bi = (b >= 6) ? 3 : (b >= 3) ? 2 : (b >= 1) ? 1 : 0;
-      const int bj = b - bi * (bi + 1) / 2;
-      doublex4 acc = (doublex4){0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int kb = 0; kb < CH_NB; kb += 4) {
-        const int kk = kb + (lane >> 4);
-        const double av = Pd[(16 * bi + (lane & 15)) * TS_LD + kk];
-        const double bv = Pd[(16 * bj + (lane & 15)) * TS_LD + kk];
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
-      }
-      // element e of lane l is (row l/16 + 4e, col l%16) of the block
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int R = 16 * bi + (lane >> 4) + 4 * e, c = 16 * bj + (lane & 15);
-        Ls[R * TS_LD + c] = c <= R ? cv[t][e] - acc[e] : 0.0;
-      }
-    }
-  }
+  diag_block_update(Pd, Ls, w, 0, c0, lane);
   __syncthreads();
 }
 
@@ -1330,10 +1349,10 @@ __global__ void __launch_bounds__(256) panel_update_kernel(double* __restrict__ 
 // One 64-column step k0 of a panel after its first, in one launch with two
 // roles.  Workgroups [0, n_factor) work as panel_factor_kernel, except that
 // the previous step's rank-64 update of this step's columns is applied here:
-// to the diagonal block's lower 16x16 blocks by all four wavefronts before
-// the factorisation, and to the workgroup's own 64 rows by wavefronts 1-3
-// while wavefront 0 factors (same sums, in the same order, as the panel
-// update's tiles).  The other workgroups apply the previous step's update to
+// to the diagonal block's first 16 columns by all four wavefronts before the
+// factorisation, to its further lower blocks and to the workgroup's own 64
+// rows by wavefronts 1-3 while wavefront 0 factors (same sums, in the same
+// order, as the panel update's tiles).  The other workgroups apply the previous step's update to
 // the panel's columns right of this step (panel_update_kernel's tiles, shifted
 // by 64 columns).  The roles write disjoint columns and read only what earlier
 // launches wrote, so the panel update no longer sits between two steps.
@@ -1347,7 +1366,10 @@ __global__ void __launch_bounds__(256, 2) panel_step_kernel(double* __restrict__
   __shared__ double rdiag[CH_NB];
   __shared__ double Xs[4][PF_ROWS * TS_LD];      // 16 rows per wavefront
   __shared__ double Mi[4 * MI_SZ];               // inverses of L11's diagonal 16x16 blocks
-  __shared__ int pflags[8];                      // sub-panels factored, inverses ready, strips stored
+  // LDS flags: [0..3] sub-panels factored, [4..6] their inverses, [8..10]
+  // the diagonal block's further blocks updated and [12..14] the own strips
+  // updated (wavefronts 1-3)
+  __shared__ int pflags[16];
   double* sA = Ls;                               // staging of the update tiles (before Ls is filled)
   double* sB = &Xs[0][0];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
@@ -1365,35 +1387,33 @@ __global__ void __launch_bounds__(256, 2) panel_step_kernel(double* __restrict__
   // 0..63 are staged in the Xs region, which the factorisation does not use.
   double* Pd = &Xs[0][0];
   const double* Ck = A + k0 * lda + k0;
-  if (tid < 8) pflags[tid] = 0;                  // ordered by the barriers below
-  diag_lower_update(P, Ck, lda, Pd, Ls);
+  if (tid < 16) pflags[tid] = 0;                 // ordered by the barriers below
+  double cr[2][4];
+  diag_col0_update(P, Ck, lda, Pd, Ls, cr);
   double* X = Xs[w];
   const int64_t own0 = CH_NB + (int64_t)blockIdx.x * (4 * PF_ROWS);   // relative to k0
   const int64_t r0 = k0 + own0 + PF_ROWS * w;
   // wavefront 0 factors the diagonal block; meanwhile wavefronts 1-3 update
-  // this workgroup's 16-row strips (w - 1, and 3 for wavefront 1) in
+  // its further lower blocks (awaited before the factor first touches them),
+  // update this workgroup's 16-row strips (w - 1, and 3 for wavefront 1) in
   // registers, store them once none of them reads Pd any more, and solve
   // their blocks 0..2 as the factor publishes its sub-panels
   if (w == 0) {
-    const int bad = factor_block_wave(Ls, colbuf, rdiag, lane, k0, pflags);
+    const int bad = factor_block_wave(Ls, colbuf, rdiag, lane, k0, pflags, &pflags[8], info);
     if (blockIdx.x == 0 && lane == 0 && bad) atomicCAS(info, 0, bad);
   } else {
+#pragma unroll
+    for (int t = 0; t < 2; ++t) diag_block_update(Pd, Ls, rest_bi(w, t), rest_bj(w, t), cr[t], lane);
+    lds_publish(&pflags[7 + w]);
     double ow[2][4][4];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
       const int s = w - 1 + 3 * t;
       if (s < 4) own_strip_update(P, Ck, lda, own0, rest_p, s, Pd, ow[t]);
     }
-    if (lane == 0) __hip_atomic_fetch_add(&pflags[7], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-    int64_t spins = 0;
-    while (__builtin_amdgcn_readfirstlane(
-               __hip_atomic_load(&pflags[7], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) < 3) {
-      __builtin_amdgcn_s_sleep(1);
-      if (++spins > kLdsSpinLimit) {
-        if (lane == 0) atomicCAS(info, 0, (int32_t)(k0 + 1));
-        break;
-      }
-    }
+    lds_publish(&pflags[11 + w]);
+#pragma unroll
+    for (int i = 12; i < 15; ++i) lds_wait(&pflags[i], info, k0);
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
       const int s = w - 1 + 3 * t;
