@@ -810,14 +810,21 @@ __device__ inline double rsqrt_refined(double a) {
 // the flag, so each lane's earlier LDS accesses are released).  The setter
 // always runs to its publication, so the wait ends; past the bound the
 // factorisation is reported as failed at this step (*info = k0 + 1) and the
-// wavefront goes on.
+// wavefront goes on.  A factorisation already reported as failed (*info set,
+// possibly by another workgroup of the launch, after which a wavefront that
+// read it at its start has left) ends the wait too: its result is discarded.
 constexpr int64_t kLdsSpinLimit = 1LL << 24;
 __device__ inline void lds_wait(const int* flag, int32_t* info, int64_t k0) {
   int64_t spins = 0;
   while (__builtin_amdgcn_readfirstlane(__hip_atomic_load(const_cast<int*>(flag), __ATOMIC_ACQUIRE,
                                                           __HIP_MEMORY_SCOPE_WORKGROUP)) == 0) {
     __builtin_amdgcn_s_sleep(1);
-    if (++spins > kLdsSpinLimit) {
+    ++spins;
+    // (a normal wait ends within a few hundred polls, before *info is read)
+    if (spins > 4096 && (spins & 255) == 0 &&
+        __builtin_amdgcn_readfirstlane(__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0)
+      break;
+    if (spins > kLdsSpinLimit) {
       if ((threadIdx.x & 63) == 0) atomicCAS(info, 0, (int32_t)(k0 + 1));
       break;
     }

@@ -363,12 +363,13 @@ def test_cholesky_solve(gpu, n):
     np.testing.assert_allclose(x.cpu().numpy(), ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max())
 
 
-@pytest.mark.parametrize("n,lda", [(1536, 1536), (2047, 2048), (2560, 2561), (4192, 4192), (8384, 8384),
-                                   (16382, 16382)])
+@pytest.mark.parametrize("n,lda", [(769, 769), (1536, 1536), (1600, 1601), (2047, 2048), (2560, 2561),
+                                   (4192, 4192), (8384, 8384), (16382, 16382)])
 def test_cholesky_many_panels(gpu, n, lda):
-    # several 512-column panels, a narrow last panel, even and odd row
-    # strides and padded rows: L L^T against A on sampled rows, the solve's
-    # residual, and L against rocSOLVER's factor
+    # several panels (768 columns below n = 16384), a narrow last panel (one
+    # column at n = 769), even and odd row strides and padded rows: L L^T
+    # against A on sampled rows, the solve's residual, and L against
+    # rocSOLVER's factor
     print(f"case cholesky panels n={n} lda={lda}", flush=True)
     g = torch.Generator(device="cuda").manual_seed(n)
     X = torch.randn(n, n // 2 + 7, dtype=torch.float64, device="cuda", generator=g)
