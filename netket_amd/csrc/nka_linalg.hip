@@ -2070,9 +2070,10 @@ static void potrf_run(double* a, int64_t n, int64_t lda, void* workspace, double
                       int32_t* info, hipStream_t s) {
   double* inv = ws_inv(workspace);
   const int64_t nblk = (n + CH_NB - 1) / CH_NB;
-  // panel widths measured best on MI355X: (768, 768) for n = 4096, 8384 and
-  // 12288 (against 512-wide panels 2.02 / 6.64 / 15.1 ms against 2.06 / 6.94 /
-  // 15.8 ms), (4096, 512) from n = 16384 (30.8 ms at 16384, 200 ms at 32768)
+  // panel widths measured best on MI355X: (768, 768) below n = 16384 (at
+  // n = 4096, 8384 and 12288: 2.02, 6.64 and 15.1 ms, against 2.06, 6.94 and
+  // 15.8 ms with (512, 512)); (4096, 512) from n = 16384 (30.8 ms at 16384,
+  // 200 ms at 32768)
   const bool large = n >= 16384;
   const int64_t NB0 = large ? 4096 : 768, NB1 = large ? 512 : 768;
   for (int64_t K0 = 0; K0 < n; K0 += NB0) {
