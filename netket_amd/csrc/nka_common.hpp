@@ -221,6 +221,21 @@ __device__ inline uint16_t bf16_rne(float f) {
 }
 __device__ inline float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+// Two FP32 values rounded to nearest even bf16 by the hardware conversion
+// (v_cvt_pk_bf16_f32), packed low-first; for finite values the same bits as
+// bf16_rne.  And the hi + lo split of two values: hi = bf16(f),
+// lo = bf16(f - hi), each pair packed low-first.
+typedef float nka_f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 nka_bf16x2 __attribute__((ext_vector_type(2)));
+__device__ inline uint32_t bf16x2_rne(float a, float b) {
+  const nka_f32x2 v = {a, b};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, nka_bf16x2));
+}
+__device__ inline void bf16x2_split(float a, float b, uint32_t& hi, uint32_t& lo) {
+  hi = bf16x2_rne(a, b);
+  lo = bf16x2_rne(a - __uint_as_float(hi << 16), b - __uint_as_float(hi & 0xFFFF0000u));
+}
+
 // Lower-triangular tile pair (ti >= tj) of linear id b in an order made of
 // SxS super-blocks of tiles (row-major over the super-blocks of the lower
 // triangle, then row-major inside each; diagonal super-blocks are

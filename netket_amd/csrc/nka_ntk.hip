@@ -323,15 +323,11 @@ __global__ void __launch_bounds__(256) qgt_hidden_pairs_kernel(const float* __re
           }
         }
       }
-      uint32_t h[8], lo[8];
+      uint32_t h[4], lo[4];
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const uint16_t hb = bf16_rne(f[e]);
-        h[e] = hb;
-        lo[e] = bf16_rne(f[e] - bf16_to_f(hb));
-      }
-      const uint4 oh = {h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16)};
-      const uint4 ol = {lo[0] | (lo[1] << 16), lo[2] | (lo[3] << 16), lo[4] | (lo[5] << 16), lo[6] | (lo[7] << 16)};
+      for (int e = 0; e < 4; ++e) bf16x2_split(f[2 * e], f[2 * e + 1], h[e], lo[e]);
+      const uint4 oh = {h[0], h[1], h[2], h[3]};
+      const uint4 ol = {lo[0], lo[1], lo[2], lo[3]};
       *(uint4*)(qh + r * ldk + s0) = oh;
       *(uint4*)(ql + r * ldk + s0) = ol;
     }
